@@ -1,0 +1,244 @@
+// gemm_f64.hip -- FP64 GEMM / SYRK on CDNA4 matrix cores.
+//
+// C = alpha * op(A) * op(B) + beta * C, column-major with leading dimensions.
+// This is the MI355X replacement for general_matrix_matrix_product::run
+// (Eigen/src/Core/products/GeneralMatrixMatrix.h:49-199) and, with the
+// lower-triangle filter, for general_matrix_matrix_triangular_product
+// (GeneralMatrixMatrixTriangular.h:56-125).
+//
+// Design: a 128x128 C tile per 256-thread workgroup (4 waves in 2x2, each
+// wave 64x64 = 4x4 tiles of v_mfma_f64_16x16x4f64), K staged through LDS in
+// slabs of 16 with double buffering.  The MFMA is fed the transposed product
+// (op(B)^T op(A)^T) so that each lane's four accumulator rows map to four
+// *columns* of C and the sixteen lanes of a row group cover sixteen
+// consecutive rows: the epilogue writes 128-byte runs of a column.
+// Workgroups are remapped so that consecutive tiles share an XCD (the
+// dispatcher deals workgroups round-robin over the 8 XCDs, each with its own
+// L2) and swizzled in groups of 8 tile-rows for L2 reuse of B.
+#include "common.h"
+#include "kernels.h"
+
+namespace eamd {
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
+constexpr int LDS_LD = BM + PAD;  // row length of the LDS slabs (doubles)
+
+__device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, int tiles_n,
+                                            int& tm, int& tn) {
+  // XCD-aware remap: workgroup bid runs on XCD bid % 8.
+  int xcd = bid & 7, loc = bid >> 3;
+  int per = nblocks >> 3, rem = nblocks & 7;
+  int logical = (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+  const int G = 8;
+  int group = G * tiles_n;
+  int g = logical / group;
+  int first_m = g * G;
+  int gm = min(tiles_m - first_m, G);
+  int r = logical - g * group;
+  tm = first_m + r % gm;
+  tn = r / gm;
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void load_slab(const double* __restrict__ A, int64_t lda,
+                                          const double* __restrict__ B, int64_t ldb, int m0,
+                                          int n0, int k0, int m, int n, int k, double ra[8],
+                                          double rb[8]) {
+  const int t = threadIdx.x;
+  // ---- op(A): BM x BK
+  if (!TA) {
+    int kk = t >> 4, mm = (t & 15) * 8;
+    int gk = k0 + kk;
+    const double* p = A + (int64_t)gk * lda + (m0 + mm);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ra[q] = (gk < k && m0 + mm + q < m) ? p[q] : 0.0;
+  } else {
+    int mm = t >> 1, kk = (t & 1) * 8;
+    int gm = m0 + mm;
+    const double* p = A + (int64_t)gm * lda + (k0 + kk);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ra[q] = (gm < m && k0 + kk + q < k) ? p[q] : 0.0;
+  }
+  // ---- op(B): BK x BN
+  if (!TB) {
+    int nn = t >> 1, kk = (t & 1) * 8;
+    int gn = n0 + nn;
+    const double* p = B + (int64_t)gn * ldb + (k0 + kk);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rb[q] = (gn < n && k0 + kk + q < k) ? p[q] : 0.0;
+  } else {
+    int kk = t >> 4, nn = (t & 15) * 8;
+    int gk = k0 + kk;
+    const double* p = B + (int64_t)gk * ldb + (n0 + nn);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rb[q] = (gk < k && n0 + nn + q < n) ? p[q] : 0.0;
+  }
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void store_slab(double* As, double* Bs, const double ra[8],
+                                           const double rb[8]) {
+  const int t = threadIdx.x;
+  if (!TA) {
+    int kk = t >> 4, mm = (t & 15) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[kk * LDS_LD + mm + q] = ra[q];
+  } else {
+    int mm = t >> 1, kk = (t & 1) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[(kk + q) * LDS_LD + mm] = ra[q];
+  }
+  if (!TB) {
+    int nn = t >> 1, kk = (t & 1) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Bs[(kk + q) * LDS_LD + nn] = rb[q];
+  } else {
+    int kk = t >> 4, nn = (t & 15) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Bs[kk * LDS_LD + nn + q] = rb[q];
+  }
+}
+
+// MODE 0: full C.  MODE 1: lower triangle only (i >= j), tiles strictly above
+// the diagonal are skipped.  MODE 2: upper triangle only (i <= j).
+template <bool TA, bool TB, int MODE>
+__global__ __launch_bounds__(256) void dgemm_kernel(int m, int n, int k, double alpha,
+                                                    const double* __restrict__ A, int64_t lda,
+                                                    const double* __restrict__ B, int64_t ldb,
+                                                    double beta, double* __restrict__ C,
+                                                    int64_t ldc, const int* __restrict__ abort_flag) {
+  __shared__ double As[2][BK * LDS_LD];
+  if (abort_flag && *abort_flag >= 0) return;
+  __shared__ double Bs[2][BK * LDS_LD];
+
+  const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  if (MODE == 1 && m0 + BM <= n0) return;  // tile entirely above the diagonal
+  if (MODE == 2 && n0 + BN <= m0) return;  // tile entirely below the diagonal
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const int ktiles = (k + BK - 1) / BK;
+  double ra[8], rb[8];
+  if (ktiles > 0) {
+    load_slab<TA, TB>(A, lda, B, ldb, m0, n0, 0, m, n, k, ra, rb);
+    store_slab<TA, TB>(As[0], Bs[0], ra, rb);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < ktiles; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < ktiles) load_slab<TA, TB>(A, lda, B, ldb, m0, n0, (kt + 1) * BK, m, n, k, ra, rb);
+    const double* as = As[buf];
+    const double* bs = Bs[buf];
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int krow = (kk * 4 + lk) * LDS_LD;
+      double af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = as[krow + wm + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = bs[krow + wn + j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < ktiles) store_slab<TA, TB>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+    __syncthreads();
+  }
+
+  // epilogue: v_mfma_f64_16x16x4f64 leaves result row lk + 4r of the 16x16
+  // tile in register r of lane (lk, l16): acc[i][j][r] = C(m0+wm+16i+l16,
+  // n0+wn+16j+lk+4r)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm + i * 16 + l16;
+    if (gi >= m) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn + j * 16 + lk + 4 * r;
+        if (gj >= n) continue;
+        if (MODE == 1 && gi < gj) continue;
+        if (MODE == 2 && gi > gj) continue;
+        double* c = C + (int64_t)gj * ldc + gi;
+        double v = alpha * acc[i][j][r];
+        if (beta != 0.0) v += beta * *c;
+        *c = v;
+      }
+  }
+}
+
+template <bool TA, bool TB, int MODE>
+void launch_t(hipStream_t s, int m, int n, int k, double alpha, const double* A, int64_t lda,
+              const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+              const int* abort_flag) {
+  int tiles = ceil_div(m, BM) * ceil_div(n, BN);
+  hipLaunchKernelGGL((dgemm_kernel<TA, TB, MODE>), dim3(tiles), dim3(256), 0, s, m, n, k, alpha,
+                     A, lda, B, ldb, beta, C, ldc, abort_flag);
+}
+
+// C = beta*C for k == 0 or alpha == 0 (BLAS semantics: beta == 0 clears C,
+// NaNs included)
+__global__ void dscale_kernel(int m, int n, double beta, double* C, int64_t ldc, int tri) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)m * n;
+  if (idx >= total) return;
+  int i = (int)(idx % m), j = (int)(idx / m);
+  if ((tri == 1 && i < j) || (tri == 2 && i > j)) return;
+  double* c = C + (int64_t)j * ldc + i;
+  *c = beta == 0.0 ? 0.0 : beta * *c;
+}
+
+}  // namespace
+
+int dgemm_device(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc, int tri, const int* abort_flag) {
+  if (m <= 0 || n <= 0) return kOk;
+  if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) {
+    set_error("dgemm: dimensions beyond int32 are not supported");
+    return kErrUnsupported;
+  }
+  if (k == 0 || alpha == 0.0) {
+    if (beta == 1.0) return kOk;
+    int64_t total = m * n;
+    hipLaunchKernelGGL(dscale_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       (int)m, (int)n, beta, C, ldc, tri);
+    EAMD_HIP(hipGetLastError());
+    return kOk;
+  }
+  int M = (int)m, N = (int)n, K = (int)k;
+#define EAMD_GEMM_DISPATCH(MODE)                                                              \
+  if (!ta && !tb) launch_t<false, false, MODE>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag); \
+  else if (!ta && tb) launch_t<false, true, MODE>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag); \
+  else if (ta && !tb) launch_t<true, false, MODE>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag); \
+  else launch_t<true, true, MODE>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  if (tri == 1) {
+    EAMD_GEMM_DISPATCH(1)
+  } else if (tri == 2) {
+    EAMD_GEMM_DISPATCH(2)
+  } else {
+    EAMD_GEMM_DISPATCH(0)
+  }
+#undef EAMD_GEMM_DISPATCH
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace eamd

@@ -1,0 +1,430 @@
+// mixed.hip -- low-precision trailing update of the FP32 LU used by the
+// mixed-precision solver.
+//
+// The LU of the refinement solver keeps its working matrix in FP32 (row-major)
+// and factors panels in FP32 (panel_lu_kernel<float>).  The trailing update
+// A22 -= A21 * A12 runs on FP16 / BF16 matrix cores with FP32 accumulation:
+//
+//   * both operands are packed once per block step into K-contiguous FP16
+//     (or BF16) copies, scaled per 128-row block by a power of two chosen so
+//     that the block maximum lands just below 2^14 (exact scaling, no range
+//     loss in FP16);
+//   * FP16X3 splits each scaled value x into hi = fp16(x) and
+//     lo = fp16((x - hi) * 2^11) and accumulates hi*hi + 2^-11 (hi*lo + lo*hi)
+//     in FP32 (two accumulator sets): 22 significant bits per operand, an
+//     update accurate to about FP32 unit roundoff on three MFMA products;
+//   * the MFMA tile computes the transposed product so each lane's four
+//     results are four columns of one row run of C (64-byte coalesced FP32
+//     read-modify-write).
+// The update is HBM-bound (C is read and written once per block step); see
+// the roofline accounting in the benchmark.
+#include "../../include/eigen_amd.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace eamd {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int MT = 128;   // C tile (M and N), also the scale block
+constexpr int KS = 32;    // K per MFMA slab (v_mfma_f32_16x16x32_{f16,bf16})
+constexpr int LROW = 40;  // LDS row length in halves (64 B of data + 16 B pad)
+
+enum { kModeF16 = 0, kModeF16x3 = 1, kModeBF16 = 2 };
+
+__device__ __forceinline__ float pow2_scale(float amax) {
+  if (!(amax > 0.0f) || !isfinite(amax)) return 1.0f;
+  int ex;
+  frexpf(amax, &ex);  // amax < 2^ex
+  return ldexpf(1.0f, 14 - ex);
+}
+
+template <int MODE>
+__device__ __forceinline__ void split_store(float x, float s, unsigned short* hi,
+                                            unsigned short* lo) {
+  float y = x * s;
+  if (MODE == kModeBF16) {
+    __bf16 h = (__bf16)y;
+    *hi = *reinterpret_cast<unsigned short*>(&h);
+  } else {
+    _Float16 h = (_Float16)y;
+    *hi = *reinterpret_cast<unsigned short*>(&h);
+    if (MODE == kModeF16x3) {
+      _Float16 l = (_Float16)((y - (float)h) * 2048.0f);
+      *lo = *reinterpret_cast<unsigned short*>(&l);
+    }
+  }
+}
+
+// Pack rows of a row-major source (N rows x K, element (j,k) at S[j*ld+k])
+// into P[j*KP + k] (+ lo copy), scale per 128-row block.  One workgroup per
+// 128-row block; first pass: block max, second: convert.
+template <int MODE>
+__global__ __launch_bounds__(256) void pack_rows_kernel(const float* __restrict__ S, int64_t ld,
+                                                        int N, int K, int KP,
+                                                        unsigned short* __restrict__ Ph,
+                                                        unsigned short* __restrict__ Pl,
+                                                        float* __restrict__ scale) {
+  __shared__ float red[256];
+  const int j0 = blockIdx.x * MT;
+  const int rows = min(MT, N - j0);
+  float m = 0.0f;
+  for (int e = threadIdx.x; e < rows * K; e += 256) {
+    int jj = e / K, k = e - jj * K;
+    m = fmaxf(m, fabsf(S[(int64_t)(j0 + jj) * ld + k]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const float s = pow2_scale(red[0]);
+  if (threadIdx.x == 0) scale[blockIdx.x] = s;
+  for (int e = threadIdx.x; e < MT * KP; e += 256) {
+    int jj = e / KP, k = e - jj * KP;
+    float x = (jj < rows && k < K) ? S[(int64_t)(j0 + jj) * ld + k] : 0.0f;
+    size_t o = (size_t)(j0 + jj) * KP + k;
+    unsigned short hbits = 0, lbits = 0;
+    split_store<MODE>(x, s, &hbits, &lbits);
+    Ph[o] = hbits;
+    if (MODE == kModeF16x3) Pl[o] = lbits;
+  }
+}
+
+// Pack columns of a row-major source (K rows x M, element (k,i) at
+// S[k*ld+i]) into P[i*KP + k]: a transpose through LDS, 128 columns per
+// workgroup, K in chunks of 32.
+template <int MODE>
+__global__ __launch_bounds__(256) void pack_cols_kernel(const float* __restrict__ S, int64_t ld,
+                                                        int M, int K, int KP,
+                                                        unsigned short* __restrict__ Ph,
+                                                        unsigned short* __restrict__ Pl,
+                                                        float* __restrict__ scale) {
+  __shared__ float red[256];
+  __shared__ float tile[32][MT + 1];
+  const int i0 = blockIdx.x * MT;
+  const int cols = min(MT, M - i0);
+  float m = 0.0f;
+  for (int e = threadIdx.x; e < K * MT; e += 256) {
+    int k = e / MT, ii = e - k * MT;
+    if (ii < cols) m = fmaxf(m, fabsf(S[(int64_t)k * ld + i0 + ii]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const float s = pow2_scale(red[0]);
+  if (threadIdx.x == 0) scale[blockIdx.x] = s;
+  for (int k0 = 0; k0 < KP; k0 += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * MT; e += 256) {
+      int kk = e / MT, ii = e - kk * MT;
+      tile[kk][ii] = (k0 + kk < K && ii < cols) ? S[(int64_t)(k0 + kk) * ld + i0 + ii] : 0.0f;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * MT; e += 256) {
+      int ii = e / 32, kk = e - ii * 32;
+      size_t o = (size_t)(i0 + ii) * KP + k0 + kk;
+      unsigned short hbits = 0, lbits = 0;
+      split_store<MODE>(tile[kk][ii], s, &hbits, &lbits);
+      Ph[o] = hbits;
+      if (MODE == kModeF16x3) Pl[o] = lbits;
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, int tiles_n,
+                                            int& tm, int& tn) {
+  int xcd = bid & 7, loc = bid >> 3;
+  int per = nblocks >> 3, rem = nblocks & 7;
+  int logical = (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+  const int G = 8;
+  int group = G * tiles_n;
+  int g = logical / group;
+  int first_m = g * G;
+  int gm = min(tiles_m - first_m, G);
+  int r = logical - g * group;
+  tm = first_m + r % gm;
+  tn = r / gm;
+}
+
+template <int MODE>
+__device__ __forceinline__ v4f mfma16(const uint4& a, const uint4& b, v4f c) {
+  if (MODE == kModeBF16) {
+    v8bf x, y;
+    __builtin_memcpy(&x, &a, 16);
+    __builtin_memcpy(&y, &b, 16);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, c, 0, 0, 0);
+  } else {
+    v8h x, y;
+    __builtin_memcpy(&x, &a, 16);
+    __builtin_memcpy(&y, &b, 16);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(x, y, c, 0, 0, 0);
+  }
+}
+
+// C (M x N column-major, ldc) -= (Pa . Pb) / (sa * sb).  Pa: M x KP
+// K-contiguous, Pb: N x KP K-contiguous.
+template <int MODE>
+__global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
+                                                           const unsigned short* __restrict__ Pah,
+                                                           const unsigned short* __restrict__ Pal,
+                                                           const unsigned short* __restrict__ Pbh,
+                                                           const unsigned short* __restrict__ Pbl,
+                                                           const float* __restrict__ sa,
+                                                           const float* __restrict__ sb,
+                                                           float* __restrict__ C, int64_t ldc) {
+  constexpr bool X3 = MODE == kModeF16x3;
+  constexpr int NARR = X3 ? 4 : 2;
+  __shared__ __align__(16) unsigned short lds[2][NARR][MT * LROW];
+
+  const int tiles_m = (M + MT - 1) / MT, tiles_n = (N + MT - 1) / MT;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * MT, n0 = tn * MT;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  v4f acc[4][4], accx[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+      accx[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+
+  // staging: each thread moves 32 bytes of each array per slab:
+  // row = tid / 2, byte half = (tid & 1) * 32 of the 64-byte K slab
+  const int srow = tid >> 1, shalf = (tid & 1) * 16;  // in halves
+  const unsigned short* src[4] = {Pah, Pbh, Pal, Pbl};
+  uint4 stg[4][2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int a = 0; a < NARR; ++a) {
+      const bool isB = (a & 1);
+      const int base = isB ? n0 : m0;
+      const int lim = isB ? N : M;
+      const int r = base + srow;
+      if (r < lim) {
+        const uint4* p = reinterpret_cast<const uint4*>(src[a] + (size_t)r * KP + k0 + shalf);
+        stg[a][0] = p[0];
+        stg[a][1] = p[1];
+      } else {
+        stg[a][0] = make_uint4(0, 0, 0, 0);
+        stg[a][1] = make_uint4(0, 0, 0, 0);
+      }
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int a = 0; a < NARR; ++a) {
+      uint4* d = reinterpret_cast<uint4*>(&lds[buf][a][srow * LROW + shalf]);
+      d[0] = stg[a][0];
+      d[1] = stg[a][1];
+    }
+  };
+
+  const int kslabs = KP / KS;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ks = 0; ks < kslabs; ++ks) {
+    const int buf = ks & 1;
+    if (ks + 1 < kslabs) gload((ks + 1) * KS);
+    uint4 af[4], bf[4], afl[4], bfl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      af[i] = *reinterpret_cast<const uint4*>(&lds[buf][0][(wm + i * 16 + l16) * LROW + lk * 8]);
+      if (X3)
+        afl[i] = *reinterpret_cast<const uint4*>(&lds[buf][2][(wm + i * 16 + l16) * LROW + lk * 8]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bf[j] = *reinterpret_cast<const uint4*>(&lds[buf][1][(wn + j * 16 + l16) * LROW + lk * 8]);
+      if (X3)
+        bfl[j] = *reinterpret_cast<const uint4*>(&lds[buf][3][(wn + j * 16 + l16) * LROW + lk * 8]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = mfma16<MODE>(bf[j], af[i], acc[i][j]);
+        if (X3) {
+          accx[i][j] = mfma16<MODE>(bfl[j], af[i], accx[i][j]);
+          accx[i][j] = mfma16<MODE>(bf[j], afl[i], accx[i][j]);
+        }
+      }
+    if (ks + 1 < kslabs) lstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  const float inv = 1.0f / (sa[tm] * sb[tn]);
+  // result row (N index) 4*lk + r of each 16x16 tile, column (M index) l16
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm + i * 16 + l16;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn + j * 16 + lk * 4 + r;
+        if (gj >= N) continue;
+        float v = acc[i][j][r];
+        if (X3) v += accx[i][j][r] * (1.0f / 2048.0f);
+        float* c = C + (int64_t)gj * ldc + gi;
+        *c -= v * inv;
+      }
+  }
+}
+
+// FP32 update on the FP32 matrix cores (v_mfma_f32_16x16x4f32), operands
+// read straight from the row-major working matrix through LDS.
+__global__ __launch_bounds__(256) void f32_update_kernel(int M, int N, int K,
+                                                         const float* __restrict__ A12,
+                                                         const float* __restrict__ A21,
+                                                         int64_t ld, float* __restrict__ C) {
+  // Xa(i,k) = A12[k*ld + i]; Xb(k,j) = A21[j*ld + k]; C(i,j) = C[j*ld + i]
+  constexpr int BK = 16;
+  __shared__ float As[2][BK][MT + 4];
+  __shared__ float Bs[2][BK][MT + 4];
+  const int tiles_m = (M + MT - 1) / MT, tiles_n = (N + MT - 1) / MT;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * MT, n0 = tn * MT;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  float ra[8], rb[8];
+  auto gload = [&](int k0) {
+    {  // A12 rows k (16) x 128 i: thread -> k = tid/16, i = (tid%16)*8
+      int k = k0 + (tid >> 4), i = m0 + (tid & 15) * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ra[q] = (k < K && i + q < M) ? A12[(int64_t)k * ld + i + q] : 0.f;
+    }
+    {  // A21 rows j (128) x 16 k: thread -> j = tid/2, k = (tid%2)*8
+      int j = n0 + (tid >> 1), k = k0 + (tid & 1) * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) rb[q] = (j < N && k + q < K) ? A21[(int64_t)j * ld + k + q] : 0.f;
+    }
+  };
+  auto lstore = [&](int buf) {
+    int k = tid >> 4, i = (tid & 15) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[buf][k][i + q] = ra[q];
+    int j = tid >> 1, kk = (tid & 1) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Bs[buf][kk + q][j] = rb[q];
+  };
+  const int kt = (K + BK - 1) / BK;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int t = 0; t < kt; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < kt) gload((t + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      float af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[buf][kk * 4 + lk][wm + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][kk * 4 + lk][wn + j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (t + 1 < kt) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm + i * 16 + l16;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn + j * 16 + lk * 4 + r;
+        if (gj >= N) continue;
+        C[(int64_t)gj * ld + gi] -= acc[i][j][r];
+      }
+  }
+}
+
+}  // namespace
+
+// row-major A22 (mrows x ncols) -= A21 (mrows x k) * A12 (k x ncols), ld
+int sgemm_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
+                          int64_t ld, int mrows, int ncols, int k, int precision) {
+  if (mrows <= 0 || ncols <= 0 || k <= 0) return kOk;
+  hipStream_t s = h->stream;
+  const int M = ncols, N = mrows;
+  const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
+  if (precision == EAMD_PREC_FP32 || precision == EAMD_PREC_FP64) {
+    PhaseScope ps(h, kPhUpdate, 4.0 * (2.0 * M * N + (double)k * (M + N)), 2.0 * M * N * k);
+    hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, s, M, N, k, A12, A21, ld,
+                       A22);
+    EAMD_HIP(hipGetLastError());
+    return kOk;
+  }
+  const int mode = precision == EAMD_PREC_FP16X3 ? kModeF16x3
+                   : precision == EAMD_PREC_BF16 ? kModeBF16 : kModeF16;
+  const int KP = ceil_div(k, KS) * KS;
+  const int MB = ceil_div(M, MT), NB = ceil_div(N, MT);
+  const size_t na = (size_t)MB * MT * KP, nbh = (size_t)NB * MT * KP;
+  const int narr = mode == kModeF16x3 ? 2 : 1;
+  size_t need = sizeof(unsigned short) * narr * (na + nbh) + sizeof(float) * (MB + NB) + 1024;
+  void* p = nullptr;
+  EAMD_CHECK(h->ws.get(kWsSplit, need, &p));
+  unsigned short* Pah = static_cast<unsigned short*>(p);
+  unsigned short* Pbh = Pah + na;
+  unsigned short* Pal = Pbh + nbh;
+  unsigned short* Pbl = mode == kModeF16x3 ? Pal + na : Pal;
+  float* sa = reinterpret_cast<float*>(Pah + (size_t)narr * (na + nbh));
+  float* sb = sa + MB;
+  // algorithmic traffic: FP32 C read + write, packed operands read once
+  const double upd_bytes = 8.0 * M * N + 2.0 * narr * (double)KP * (M + N);
+  const double pack_bytes = (4.0 + 2.0 * narr) * (double)k * (M + N);
+#define EAMD_MIXED(MODE)                                                                        \
+  {                                                                                             \
+    PhaseScope ps(h, kPhPack, pack_bytes, 0.0);                                                 \
+    hipLaunchKernelGGL(pack_cols_kernel<MODE>, dim3(MB), dim3(256), 0, s, A12, ld, M, k, KP,    \
+                       Pah, Pal, sa);                                                           \
+    hipLaunchKernelGGL(pack_rows_kernel<MODE>, dim3(NB), dim3(256), 0, s, A21, ld, N, k, KP,    \
+                       Pbh, Pbl, sb);                                                           \
+  }                                                                                             \
+  {                                                                                             \
+    PhaseScope ps(h, kPhUpdate, upd_bytes, 2.0 * M * N * (double)k);                            \
+    hipLaunchKernelGGL(mixed_update_kernel<MODE>, dim3(tiles), dim3(256), 0, s, M, N, KP, Pah,  \
+                       Pal, Pbh, Pbl, sa, sb, A22, ld);                                         \
+  }
+  if (mode == kModeF16x3) {
+    EAMD_MIXED(kModeF16x3);
+  } else if (mode == kModeBF16) {
+    EAMD_MIXED(kModeBF16);
+  } else {
+    EAMD_MIXED(kModeF16);
+  }
+#undef EAMD_MIXED
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace eamd

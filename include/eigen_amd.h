@@ -1,0 +1,250 @@
+/*
+ * eigen_amd.h -- C ABI of the MI355X (gfx950) dense factor-and-solve library.
+ *
+ * Plain C: pointers, sizes and status codes only.  Matrices are column-major
+ * with a leading dimension (Eigen's default storage, Map<Matrix,0,
+ * OuterStride<>>).  Sizes are int64_t; pivots are int32 like Eigen's
+ * default PermutationMatrix StorageIndex.  Unless a function is named
+ * `eamd_host_*`, every matrix/vector pointer is a DEVICE pointer and the work
+ * is queued on the handle's HIP stream.
+ *
+ * Status: 0 = success; -i = argument i (1-based) is invalid, the reference's
+ * xerbla convention (lapack/lu.cpp:17-24, blas/level3_impl.h:47-57);
+ * > 0 = numerical outcome documented per function; <= -1000 = library error
+ * (eamd_last_error() has the message).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the Eigen 3.3.90 snapshot).
+ */
+#ifndef EIGEN_AMD_H
+#define EIGEN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EAMD_VERSION 10000
+
+#define EAMD_OK 0
+#define EAMD_ERR_HIP (-1000)
+#define EAMD_ERR_ALLOC (-1001)
+#define EAMD_ERR_UNSUPPORTED (-1002)
+#define EAMD_ERR_NOT_CONVERGED (-1003)
+
+typedef struct eamd_handle* eamd_handle_t;
+
+/* ------------------------------------------------------------ lifetime -- */
+int eamd_version(void);
+/* Creates a handle on the current HIP device with its own stream.  Handles
+ * are thread-confined (one per host thread), as Eigen objects are not
+ * internally synchronised (Parallelizer.h:20,47-54). */
+int eamd_create(eamd_handle_t* handle);
+int eamd_destroy(eamd_handle_t handle);
+/* Queue the handle's work on a caller-owned hipStream_t (passed as void*);
+ * NULL selects the default stream.  Pending work on the previous stream is
+ * synchronised first. */
+int eamd_set_stream(eamd_handle_t handle, void* hip_stream);
+void* eamd_get_stream(eamd_handle_t handle);
+int eamd_synchronize(eamd_handle_t handle);
+/* Thread-local message for the last failure. */
+const char* eamd_last_error(void);
+/* Frees the handle's cached device workspace. */
+int eamd_release_workspace(eamd_handle_t handle);
+
+/* Library instrumentation (no reference counterpart; the reference has
+ * only offline timers, bench/BenchTimer.h): device time of the library's
+ * own kernels per phase, from hipEvent pairs on the handle's stream, with
+ * the algorithmic bytes and flops of those launches. */
+#define EAMD_PHASE_PANEL 0     /* LU panel factorization */
+#define EAMD_PHASE_LASWP 1     /* row interchanges */
+#define EAMD_PHASE_TRSM 2      /* U12 = L11^{-1} A12 */
+#define EAMD_PHASE_UPDATE 3    /* LU trailing update A22 -= A21 A12 */
+#define EAMD_PHASE_PACK 4      /* FP16/BF16 operand packing of the mixed update */
+#define EAMD_PHASE_TRSV 5      /* triangular solves */
+#define EAMD_PHASE_RESIDUAL 6  /* FP64 residuals */
+#define EAMD_PHASE_GEMM 7      /* eamd_dgemm */
+#define EAMD_PHASE_POTF2 8     /* Cholesky diagonal blocks */
+#define EAMD_PHASE_BATCHED 9   /* batched solves */
+int eamd_profile_enable(eamd_handle_t handle, int on);
+int eamd_profile_read(eamd_handle_t handle, int phase, double* ms, double* bytes, double* flops,
+                      int64_t* count);
+
+/* --------------------------------------------------------------- level 3 -- */
+/* C = alpha*op(A)*op(B) + beta*C ('N' or 'T').
+ * Replaces general_matrix_matrix_product<...,ColMajor>::run
+ * (Eigen/src/Core/products/GeneralMatrixMatrix.h:49-199, res += alpha*lhs*rhs)
+ * and dgemm_ (blas/level3_impl.h:12-76). */
+int eamd_dgemm(eamd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+               double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+               double beta, double* C, int64_t ldc);
+
+/* B = alpha*op(A)^{-1}*B (side 'L') or alpha*B*op(A)^{-1} (side 'R');
+ * uplo 'L'/'U', transa 'N'/'T', diag 'N'/'U'.
+ * Replaces triangular_solve_matrix<...,OnTheLeft|OnTheRight,...>::run
+ * (Eigen/src/Core/products/TriangularSolverMatrix.h:38-329; dispatch
+ * SolveTriangular.h:81-104) and dtrsm_ (blas/level3_impl.h:78-179). */
+int eamd_dtrsm(eamd_handle_t h, char side, char uplo, char transa, char diag, int64_t m,
+               int64_t n, double alpha, const double* A, int64_t lda, double* B, int64_t ldb);
+
+/* C(uplo) = alpha*op(A)*op(A)^T + beta*C(uplo), trans 'N' (A n x k) or 'T'.
+ * Replaces SelfAdjointView::rankUpdate -> general_matrix_matrix_triangular_
+ * product (SelfadjointProduct.h:82-129, GeneralMatrixMatrixTriangular.h:56-125)
+ * and dsyrk_ (blas/level3_impl.h:357). */
+int eamd_dsyrk(eamd_handle_t h, char uplo, char trans, int64_t n, int64_t k, double alpha,
+               const double* A, int64_t lda, double beta, double* C, int64_t ldc);
+
+/* ------------------------------------------------------------------- LU -- */
+/* In-place LU with partial pivoting of an m x n matrix (m >= n, or the
+ * reference's square use).  transp[min(m,n)] receives 0-based row
+ * transpositions (device int32), *nb_transp (host) their non-trivial count
+ * and *first_zero_pivot (host) the first exactly-zero pivot or -1; the
+ * factorization continues past it, as the reference does.
+ * Replaces partial_lu_impl<double,ColMajor,int>::blocked_lu
+ * (Eigen/src/LU/PartialPivLU.h:426-496), i.e. PartialPivLU::compute
+ * (:514-536), and dgetrf_ (lapack/lu.cpp:14-41, which reports 1-based ipiv
+ * and info = first_zero+1). */
+int eamd_dgetrf(eamd_handle_t h, int64_t m, int64_t n, double* A, int64_t lda, int32_t* transp,
+                int32_t* nb_transp, int64_t* first_zero_pivot);
+
+/* Solves A X = B ('N') or A^T X = B ('T') with the factors of eamd_dgetrf.
+ * Replaces PartialPivLU::_solve_impl / _solve_impl_transposed
+ * (PartialPivLU.h:226-271) and dgetrs_ (lapack/lu.cpp:46-89). */
+int eamd_dgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const double* LU,
+                int64_t lda, const int32_t* transp, double* B, int64_t ldb);
+
+/* PermutationMatrix indices from a transposition sequence (device arrays):
+ * identity, then swap(perm[k], perm[t_k]) for k = n-1..0, i.e.
+ * PartialPivLU::permutationP() (PermutationMatrix.h:82-88). */
+int eamd_transpositions_to_permutation(eamd_handle_t h, int64_t n, const int32_t* transp,
+                                       int32_t* perm);
+
+/* ------------------------------------------------------------------ LLT -- */
+/* In-place Cholesky; uplo 'L' (A = L L^T, lower triangle read/written) or
+ * 'U' (A = U^T U).  *fail_col (host) = first column whose pivot is <= 0, or
+ * -1 (LLT::info() == NumericalIssue when >= 0).
+ * Replaces llt_inplace<double,Lower|Upper>::blocked (Eigen/src/Cholesky/
+ * LLT.h:330-391), LLT::compute (:422-452) and dpotrf_ (lapack/cholesky.cpp:14-36,
+ * info = fail_col+1). */
+int eamd_dpotrf(eamd_handle_t h, char uplo, int64_t n, double* A, int64_t lda,
+                int64_t* fail_col);
+
+/* Solves A X = B with the factor of eamd_dpotrf.
+ * Replaces LLT::_solve_impl (LLT.h:475-505) and dpotrs_ (lapack/cholesky.cpp:41-72). */
+int eamd_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const double* A,
+                int64_t lda, double* B, int64_t ldb);
+
+/* --------------------------------------------------- mixed precision IR -- */
+/* No reference counterpart (Eigen has no iterative refinement): a new
+ * capability behind PartialPivLU's solve() surface. */
+#define EAMD_PREC_FP64 0      /* plain FP64 LU (the drop-in) */
+#define EAMD_PREC_FP32 1      /* FP32 LU, FP32 trailing update */
+#define EAMD_PREC_FP16 2      /* FP32 panel, FP16 MFMA trailing update */
+#define EAMD_PREC_BF16 3      /* FP32 panel, BF16 MFMA trailing update */
+#define EAMD_PREC_FP16X3 4    /* FP32 panel, FP16 hi/lo split update (3 MFMA products) */
+
+typedef struct eamd_ir_opts {
+  int factor_precision;   /* EAMD_PREC_* */
+  int max_iter;           /* refinement steps (classic IR), default 30 */
+  double tol;             /* target normwise backward error, default 1e-14 */
+  int gmres;              /* 0: never, 1: fall back when IR stalls, 2: always */
+  int gmres_restart;      /* Krylov subspace size, default 30 */
+  int gmres_max_iter;     /* total GMRES iterations, default 300 */
+  int nb;                 /* LU block width, 0 = automatic */
+} eamd_ir_opts;
+
+typedef struct eamd_ir_report {
+  int ir_iterations;
+  int gmres_iterations;
+  int converged;          /* 1 when backward_error <= tol */
+  int used_gmres;
+  double backward_error;  /* ||b-Ax||_inf / (||A||_inf ||x||_inf + ||b||_inf), max over rhs */
+  double factor_ms;
+  double refine_ms;
+  int64_t first_zero_pivot;
+} eamd_ir_report;
+
+void eamd_ir_opts_default(eamd_ir_opts* opts);
+
+/* Solves A X = B: low-precision LU of A, FP64 residuals on the device,
+ * classic refinement and LU-preconditioned FP64 GMRES when it stalls.  A is
+ * not modified.  Returns EAMD_ERR_NOT_CONVERGED (report filled) when the
+ * tolerance is not met. */
+int eamd_dgesv_ir(eamd_handle_t h, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                  const double* B, int64_t ldb, double* X, int64_t ldx, const eamd_ir_opts* opts,
+                  eamd_ir_report* report);
+
+/* Normwise backward error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf
+ * + ||b_j||_inf), residual in FP64 on the device; result on the host. */
+int eamd_dbackward_error(eamd_handle_t h, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                         const double* X, int64_t ldx, const double* B, int64_t ldb,
+                         double* berr);
+
+/* -------------------------------------------------------------- batched -- */
+/* No reference counterpart as one launch; per system it is
+ * LLT<MatrixXf>(A_i).solve(b_i) (LLT.h:422-505).  Systems are packed
+ * column-major n_i x n_i at A + offA[i], right-hand sides at b + offb[i]
+ * (overwritten by x_i).  info[i] = failing column or -1.  All arrays are
+ * device arrays.  Sizes up to 32. */
+int eamd_spotrf_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
+                              const int32_t* n, const float* A, const int64_t* offb, float* b,
+                              int32_t* info);
+
+/* The same split into a plan (size bucketing of n[], one host
+ * synchronisation, autotuned lanes-per-system per size) and an execution
+ * that only queues kernels -- no host synchronisation, no memset -- so it can
+ * be captured in a HIP graph and replayed for new A/b of the same layout. */
+typedef struct eamd_batched_plan* eamd_batched_plan_t;
+int eamd_batched_plan_create(eamd_handle_t h, int64_t count, const int32_t* n, int32_t max_n,
+                             eamd_batched_plan_t* plan);
+int eamd_batched_plan_destroy(eamd_batched_plan_t plan);
+/* lanes of a wavefront per system chosen for size n (0: n not in the plan) */
+int eamd_batched_plan_lanes(eamd_batched_plan_t plan, int32_t n);
+int eamd_spotrf_solve_batched_planned(eamd_handle_t h, eamd_batched_plan_t plan,
+                                      const int64_t* offA, const float* A, const int64_t* offb,
+                                      float* b, int32_t* info);
+
+/* Per system PartialPivLU<MatrixXd>(A_i).solve(B_i) (PartialPivLU.h:128,
+ * 226-244), computed as an FP32 LU with FP64 iterative refinement.  B_i and
+ * X_i are n_i x nrhs column-major at offB[i]; iters[i] (nullable) receives
+ * the refinement steps and info[i] = 0 converged, 1 not converged.  Sizes
+ * up to 256. */
+int eamd_dgesv_ir_batched(eamd_handle_t h, int64_t count, const int64_t* offA, const int32_t* n,
+                          const double* A, const int64_t* offB, int64_t nrhs, const double* B,
+                          double* X, int32_t* info, int32_t* iters);
+
+/* ------------------------------------------------- host-buffer variants -- */
+/* Same operations on host (pageable) buffers: upload, compute, download.
+ * These back the Eigen plug-point specialisations, whose arguments are host
+ * pointers (PartialPivLU_LAPACKE.h:42-77, LLT_LAPACKE.h:42-93,
+ * GeneralMatrixMatrix_BLAS.h:49-116, TriangularSolverMatrix_BLAS.h:41-157). */
+int eamd_host_dgetrf(eamd_handle_t h, int64_t m, int64_t n, double* A, int64_t lda,
+                     int32_t* transp, int32_t* nb_transp, int64_t* first_zero_pivot);
+int eamd_host_dgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const double* LU,
+                     int64_t lda, const int32_t* transp, double* B, int64_t ldb);
+int eamd_host_dpotrf(eamd_handle_t h, char uplo, int64_t n, double* A, int64_t lda,
+                     int64_t* fail_col);
+int eamd_host_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const double* A,
+                     int64_t lda, double* B, int64_t ldb);
+int eamd_host_dgemm(eamd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                    double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                    double beta, double* C, int64_t ldc);
+int eamd_host_dtrsm(eamd_handle_t h, char side, char uplo, char transa, char diag, int64_t m,
+                    int64_t n, double alpha, const double* A, int64_t lda, double* B,
+                    int64_t ldb);
+int eamd_host_dgesv_ir(eamd_handle_t h, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                       const double* B, int64_t ldb, double* X, int64_t ldx,
+                       const eamd_ir_opts* opts, eamd_ir_report* report);
+
+/* ------------------------------------------------- device memory helpers -- */
+/* Thin wrappers so that C callers (and the adapter) need no HIP headers. */
+int eamd_malloc(void** ptr, int64_t bytes);
+int eamd_free(void* ptr);
+int eamd_memcpy_htod(eamd_handle_t h, void* dst, const void* src, int64_t bytes);
+int eamd_memcpy_dtoh(eamd_handle_t h, void* dst, const void* src, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EIGEN_AMD_H */

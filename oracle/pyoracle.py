@@ -1,0 +1,211 @@
+"""ctypes bindings for the CPU checker (test infrastructure only).
+
+Loads ``oracle/liboracle.so`` (the C restatement of Eigen's algorithms,
+``oracle.c``) and, when it was built, ``oracle/_ref/libeigen_ref.so`` (the
+reference compiled from its own headers, ``ref_eigen.cpp``).  Only tests,
+``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
+module; the product package never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = None
+_REF = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_d1 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f1 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+L = C.c_long
+
+
+def build():
+    """Compile the restatement (and the reference shim where headers exist)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def orc():
+    global _ORC
+    if _ORC is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        lib.orc_lu_blocked_d.argtypes = [L, L, _dp, L, _ip, C.POINTER(C.c_int), L, C.c_void_p]
+        lib.orc_lu_blocked_d.restype = L
+        lib.orc_lu_blocked_f.argtypes = [L, L, _fp, L, _ip, C.POINTER(C.c_int), L]
+        lib.orc_lu_blocked_f.restype = L
+        lib.orc_lu_unblocked_d.argtypes = [L, L, _dp, L, _ip, C.POINTER(C.c_int), C.c_void_p]
+        lib.orc_lu_unblocked_d.restype = L
+        lib.orc_transpositions_to_perm.argtypes = [L, _ip, _ip]
+        lib.orc_lu_solve_d.argtypes = [L, L, _dp, L, _ip, _dp, L]
+        lib.orc_lu_solve_f.argtypes = [L, L, _fp, L, _ip, _fp, L]
+        for s, p in (("d", _dp), ("f", _fp)):
+            getattr(lib, "orc_llt_unblocked_" + s).argtypes = [L, p, L]
+            getattr(lib, "orc_llt_unblocked_" + s).restype = L
+            getattr(lib, "orc_llt_blocked_" + s).argtypes = [L, p, L]
+            getattr(lib, "orc_llt_blocked_" + s).restype = L
+            getattr(lib, "orc_llt_solve_" + s).argtypes = [L, L, p, L, p, L]
+        lib.orc_gemm_d.argtypes = [C.c_char, C.c_char, L, L, L, C.c_double, _dp, L, _dp, L,
+                                   C.c_double, _dp, L]
+        lib.orc_trsm_d.argtypes = [C.c_char, C.c_char, C.c_char, C.c_char, L, L, C.c_double,
+                                   _dp, L, _dp, L]
+        lib.orc_syrk_d.argtypes = [C.c_char, C.c_char, L, L, C.c_double, _dp, L, C.c_double,
+                                   _dp, L]
+        lib.orc_backward_error_d.argtypes = [L, L, _dp, L, _dp, L, _dp, L]
+        lib.orc_backward_error_d.restype = C.c_double
+        lib.orc_batched_llt_solve_f.argtypes = [L, _i64p, _ip, _f1, _i64p, _f1, _ip]
+        lib.orc_batched_lu_solve_d.argtypes = [L, _i64p, _ip, _d1, _i64p, L, _d1, _ip]
+        _ORC = lib
+    return _ORC
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libeigen_ref.so"))
+
+
+def ref():
+    """The compiled reference (Eigen itself); None when it was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libeigen_ref.so")
+        if not os.path.exists(path):
+            return None
+        lib = C.CDLL(path)
+        lib.eref_version.restype = C.c_int
+        lib.eref_set_threads.argtypes = [C.c_int]
+        lib.eref_max_threads.restype = C.c_int
+        lib.eref_lu_blocked_d.argtypes = [L, L, _dp, L, _ip, C.POINTER(C.c_int)]
+        lib.eref_lu_blocked_d.restype = L
+        lib.eref_lu_blocked_f.argtypes = [L, L, _fp, L, _ip, C.POINTER(C.c_int)]
+        lib.eref_lu_blocked_f.restype = L
+        lib.eref_partialpivlu_d.argtypes = [L, _dp, L, L, _dp, L, _dp, _ip, _dp, L,
+                                            C.POINTER(C.c_double)]
+        lib.eref_llt_d.argtypes = [L, _dp, L, L, _dp, L, _dp, _dp, L]
+        lib.eref_llt_f.argtypes = [L, _fp, L, L, _fp, L, _fp, _fp, L]
+        lib.eref_llt_blocked_d.argtypes = [L, _dp, L]
+        lib.eref_llt_blocked_d.restype = L
+        lib.eref_gemm_d.argtypes = [L, L, L, _dp, L, _dp, L, _dp, L]
+        lib.eref_trsm_d.argtypes = [C.c_char, C.c_char, C.c_char, C.c_char, L, L, _dp, L, _dp, L]
+        lib.eref_syrk_lower_d.argtypes = [L, L, C.c_double, _dp, L, _dp, L]
+        lib.eref_batched_llt_solve_f.argtypes = [L, _i64p, _ip, _f1, _i64p, _f1, _ip]
+        lib.eref_batched_lu_solve_d.argtypes = [L, _i64p, _ip, _d1, _i64p, L, _d1, _ip]
+        _REF = lib
+    return _REF
+
+
+# ---------------------------------------------------------------- helpers --
+def lu(a, max_block=256, margins=False):
+    """Restated blocked_lu on a copy of ``a``; returns (lu, transp, nbt, fz[, margin])."""
+    a = np.array(a, dtype=np.float64, order="F", copy=True)
+    m, n = a.shape
+    t = np.zeros(min(m, n), dtype=np.int32)
+    nbt = C.c_int(0)
+    mg = np.zeros(min(m, n), dtype=np.float64) if margins else None
+    fz = orc().orc_lu_blocked_d(m, n, a, max(m, 1), t, C.byref(nbt), max_block,
+                                mg.ctypes.data if margins else None)
+    out = (a, t, nbt.value, int(fz))
+    return out + (mg,) if margins else out
+
+
+def lu_f(a, max_block=256):
+    a = np.array(a, dtype=np.float32, order="F", copy=True)
+    m, n = a.shape
+    t = np.zeros(min(m, n), dtype=np.int32)
+    nbt = C.c_int(0)
+    fz = orc().orc_lu_blocked_f(m, n, a, max(m, 1), t, C.byref(nbt), max_block)
+    return a, t, nbt.value, int(fz)
+
+
+def perm_from_transpositions(t):
+    p = np.zeros(len(t), dtype=np.int32)
+    orc().orc_transpositions_to_perm(len(t), np.ascontiguousarray(t, dtype=np.int32), p)
+    return p
+
+
+def lu_solve(lu_, t, b):
+    b = np.array(b, dtype=np.float64, order="F", copy=True)
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    n = lu_.shape[0]
+    orc().orc_lu_solve_d(n, b.shape[1], np.asfortranarray(lu_), n,
+                         np.ascontiguousarray(t, dtype=np.int32), b, n)
+    return b
+
+
+def llt(a, dtype=np.float64):
+    a = np.array(a, dtype=dtype, order="F", copy=True)
+    n = a.shape[0]
+    f = orc().orc_llt_blocked_d if dtype == np.float64 else orc().orc_llt_blocked_f
+    r = f(n, a, max(n, 1))
+    return a, int(r)
+
+
+def llt_solve(l, b):
+    dtype = l.dtype
+    b = np.array(b, dtype=dtype, order="F", copy=True)
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    n = l.shape[0]
+    f = orc().orc_llt_solve_d if dtype == np.float64 else orc().orc_llt_solve_f
+    f(n, b.shape[1], np.asfortranarray(l), n, b, n)
+    return b
+
+
+def gemm(a, b, c=None, alpha=1.0, beta=0.0, transa="N", transb="N"):
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64)
+    m = a.shape[0] if transa == "N" else a.shape[1]
+    k = a.shape[1] if transa == "N" else a.shape[0]
+    n = b.shape[1] if transb == "N" else b.shape[0]
+    c = np.zeros((m, n), order="F") if c is None else np.array(c, order="F", dtype=np.float64)
+    orc().orc_gemm_d(transa.encode(), transb.encode(), m, n, k, alpha, a, max(a.shape[0], 1),
+                     b, max(b.shape[0], 1), beta, c, max(m, 1))
+    return c
+
+
+def trsm(side, uplo, trans, diag, a, b, alpha=1.0):
+    b = np.array(b, dtype=np.float64, order="F", copy=True)
+    a = np.asfortranarray(a, dtype=np.float64)
+    m, n = b.shape
+    orc().orc_trsm_d(side.encode(), uplo.encode(), trans.encode(), diag.encode(), m, n, alpha,
+                     a, max(a.shape[0], 1), b, max(m, 1))
+    return b
+
+
+def syrk(uplo, trans, a, c, alpha=1.0, beta=1.0):
+    a = np.asfortranarray(a, dtype=np.float64)
+    c = np.array(c, dtype=np.float64, order="F", copy=True)
+    n = c.shape[0]
+    k = a.shape[1] if trans == "N" else a.shape[0]
+    orc().orc_syrk_d(uplo.encode(), trans.encode(), n, k, alpha, a, max(a.shape[0], 1), beta,
+                     c, max(n, 1))
+    return c
+
+
+def backward_error(a, x, b):
+    a = np.asfortranarray(a, dtype=np.float64)
+    x = np.asfortranarray(x.reshape(a.shape[0], -1), dtype=np.float64)
+    b = np.asfortranarray(b.reshape(a.shape[0], -1), dtype=np.float64)
+    n = a.shape[0]
+    return orc().orc_backward_error_d(n, x.shape[1], a, n, x, n, b, n)
+
+
+def batched_llt_solve_f(offA, n, A, offb, b):
+    b = np.array(b, dtype=np.float32, copy=True)
+    info = np.zeros(len(n), dtype=np.int32)
+    orc().orc_batched_llt_solve_f(len(n), offA, n, A, offb, b, info)
+    return b, info
+
+
+def batched_lu_solve_d(offA, n, A, offb, nrhs, B):
+    B = np.array(B, dtype=np.float64, copy=True)
+    info = np.zeros(len(n), dtype=np.int32)
+    orc().orc_batched_lu_solve_d(len(n), offA, n, A, offb, nrhs, B, info)
+    return B, info

@@ -1,0 +1,158 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+Runs in the build container only: it needs oracle/_ref/libeigen_ref.so,
+which oracle/Makefile compiles from the Eigen 3.3.90 headers (g++ -O3
+-mavx2 -mfma; the snapshot's AVX-512 path crashes).  The reference's own
+tests hold no golden vectors for this path (they are property tests), so
+these fixtures are Eigen's outputs on seeded inputs (tests/inputs.py):
+
+  lu.npz    PartialPivLU (blocked_lu, PartialPivLU.h:426-496) for a sweep of
+            sizes: transpositions, nb_transpositions, first zero pivot, LU
+            factors (n <= 64), solution of A x = b, determinant; plus the
+            n = 1024 FP64 instance of the plain drop-in configuration.
+  llt.npz   LLT<MatrixXd> (LLT.h:330-360): L (n <= 33), solution, info;
+            forced non-SPD cases with their failing column.
+  gemm.npz  C = A*B on the edge shapes of bench/perf_monitoring/
+            gemm_settings.txt reduced to <= 240 per side (full C for the
+            small ones; row sums, column sums and 256 sampled entries else).
+  batched.npz  a 1,000-system sample of the batched LLT (FP32) and batched LU
+            (FP64, 4 RHS, 48 systems) workloads with Eigen's per-system solutions.
+Inputs are not stored (they are regenerated from the seeds); their SHA-256
+is, so a diverging generator is detected.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import pyoracle as po  # noqa: E402
+import inputs  # noqa: E402
+
+LU_SIZES = [1, 2, 3, 4, 5, 7, 8, 15, 16, 17, 31, 32, 33, 64, 127, 128, 129, 255, 256, 257]
+LLT_SIZES = [1, 6, 9, 12, 15, 18, 24, 31, 32, 33, 127, 128, 129]
+GEMM_SHAPES = [(8, 8, 8), (9, 9, 9), (24, 24, 24), (239, 239, 239), (240, 240, 240),
+               (240, 24, 24), (24, 240, 24), (24, 24, 240), (24, 240, 240), (240, 24, 240),
+               (240, 240, 24), (240, 240, 64), (240, 23, 160), (23, 240, 160)]
+
+
+def lu_case(ref, a, b):
+    n = a.shape[0]
+    lu_ = np.asfortranarray(a.copy())
+    t = np.zeros(n, dtype=np.int32)
+    nbt = C.c_int(0)
+    fz = ref.eref_lu_blocked_d(n, n, lu_, n, t, C.byref(nbt))
+    x = np.zeros((n, 1), order="F")
+    lu2 = np.zeros((n, n), order="F")
+    perm = np.zeros(n, dtype=np.int32)
+    det = C.c_double(0)
+    ref.eref_partialpivlu_d(n, np.asfortranarray(a), n, 1, np.asfortranarray(b.reshape(n, 1)), n,
+                            lu2, perm, x, n, C.byref(det))
+    return lu_, t, nbt.value, fz, x[:, 0], perm, det.value
+
+
+def main():
+    ref = po.ref()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libeigen_ref.so not built (needs the reference headers)")
+    ref.eref_set_threads(1)
+    out = {"eigen_version": np.array(ref.eref_version())}
+    for n in LU_SIZES:
+        a = inputs.normal(1000 + n, n, n)
+        b = inputs.normal(2000 + n, n)
+        lu_, t, nbt, fz, x, perm, det = lu_case(ref, a, b)
+        out[f"n{n}_sha"] = np.array(inputs.sha256(a))
+        out[f"n{n}_transp"] = t
+        out[f"n{n}_perm"] = perm
+        out[f"n{n}_nbt"] = np.array(nbt)
+        out[f"n{n}_fz"] = np.array(fz)
+        out[f"n{n}_x"] = x
+        out[f"n{n}_det"] = np.array(det)
+        if n <= 64:
+            out[f"n{n}_lu"] = lu_
+        else:
+            out[f"n{n}_lu_fro"] = np.array(np.linalg.norm(lu_))
+    # the plain FP64 drop-in configuration: n = 1024, seeds 1 / 2
+    a = inputs.normal(1, 1024, 1024)
+    b = inputs.normal(2, 1024)
+    lu_, t, nbt, fz, x, perm, det = lu_case(ref, a, b)
+    out["c1_sha_a"] = np.array(inputs.sha256(a))
+    out["c1_sha_b"] = np.array(inputs.sha256(b))
+    out["c1_transp"] = t
+    out["c1_nbt"] = np.array(nbt)
+    out["c1_x"] = x
+    out["c1_berr"] = np.array(po.backward_error(a, x, b))
+    # special cases
+    z = inputs.normal(77, 40, 40)
+    z[:, 5] = 0.0  # exact zero column -> first zero pivot, factorization continues
+    lu_, t, nbt, fz, x, perm, det = lu_case(ref, z, inputs.normal(78, 40))
+    out["zero_transp"] = t
+    out["zero_fz"] = np.array(fz)
+    out["zero_lu"] = lu_
+    tie = np.asfortranarray(np.array([[1.0, 2.0, 0.0], [-1.0, 1.0, 3.0], [1.0, 0.0, 1.0]]))
+    lu_, t, nbt, fz, x, perm, det = lu_case(ref, tie, np.ones(3))
+    out["tie_transp"] = t
+    out["tie_lu"] = lu_
+    np.savez_compressed(os.path.join(HERE, "lu.npz"), **out)
+
+    out = {}
+    for n in LLT_SIZES:
+        a = inputs.spd(3000 + n, n)
+        b = inputs.normal(4000 + n, n)
+        L = np.zeros((n, n), order="F")
+        x = np.zeros((n, 1), order="F")
+        info = ref.eref_llt_d(n, a, n, 1, np.asfortranarray(b.reshape(n, 1)), n, L, x, n)
+        out[f"n{n}_sha"] = np.array(inputs.sha256(a))
+        out[f"n{n}_info"] = np.array(info)
+        out[f"n{n}_x"] = x[:, 0]
+        if n <= 33:
+            out[f"n{n}_L"] = np.tril(L)
+    for n, bad in [(10, 4), (40, 35), (200, 150)]:
+        a = inputs.spd(5000 + n, n)
+        a[bad, bad] = -a[bad, bad] - 1e3  # not SPD from column `bad` on
+        w = np.asfortranarray(a.copy())
+        out[f"bad{n}_fail"] = np.array(ref.eref_llt_blocked_d(n, w, n))
+        out[f"bad{n}_bad"] = np.array(bad)
+    np.savez_compressed(os.path.join(HERE, "llt.npz"), **out)
+
+    out = {}
+    for (m, n, k) in GEMM_SHAPES:
+        a = inputs.normal(6000 + m + 7 * n + 13 * k, m, k)
+        b = inputs.normal(7000 + m + 7 * n + 13 * k, k, n)
+        c = np.zeros((m, n), order="F")
+        ref.eref_gemm_d(m, n, k, a, m, b, k, c, m)
+        if m * n <= 24 * 240:
+            out[f"g{m}_{n}_{k}"] = c
+        else:  # row/column sums and a fixed sample pin orientation and values
+            out[f"g{m}_{n}_{k}_rows"] = c.sum(axis=1)
+            out[f"g{m}_{n}_{k}_cols"] = c.sum(axis=0)
+            idx = inputs.rng(99).integers(0, m * n, size=256)
+            out[f"g{m}_{n}_{k}_idx"] = idx
+            out[f"g{m}_{n}_{k}_val"] = c.ravel(order="F")[idx]
+    np.savez_compressed(os.path.join(HERE, "gemm.npz"), **out)
+
+    out = {}
+    n, offA, A, offb, b = inputs.packed_systems(11, 1000, [6, 9, 12, 15, 18, 24], np.float32, True)
+    x = b.copy()
+    info = np.zeros(1000, dtype=np.int32)
+    ref.eref_batched_llt_solve_f(1000, offA, n, A, offb, x, info)
+    out["llt_sha"] = np.array(inputs.sha256(A))
+    out["llt_x"] = x
+    out["llt_info"] = info
+    n, offA, A, offB, B = inputs.packed_systems(12, 48, (16, 256), np.float64, False, nrhs=4)
+    X = B.copy()
+    info = np.zeros(48, dtype=np.int32)
+    ref.eref_batched_lu_solve_d(48, offA, n, A, offB, 4, X, info)
+    out["lu_sha"] = np.array(inputs.sha256(A))
+    out["lu_x"] = X
+    np.savez_compressed(os.path.join(HERE, "batched.npz"), **out)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+"""CPU tests of the C ABI boundary: the library loads, exports every symbol
+include/eigen_amd.h declares, and rejects bad arguments with the
+reference's xerbla numbering before touching the GPU (lapack/lu.cpp:16-24,
+lapack/cholesky.cpp:17-24, blas/level3_impl.h:47-57)."""
+import ctypes as C
+
+import pytest
+
+import eigen_amd
+from eigen_amd import _lib
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = eigen_amd.lib()
+    names = _lib.declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes signatures cover the header
+    assert set(names) <= set(_lib._SIGS)
+
+
+def test_version():
+    assert eigen_amd.lib().eamd_version() == 10000
+
+
+NULL = None
+
+
+def test_dgetrf_argument_codes():
+    L = eigen_amd.lib()
+    nbt = C.c_int32()
+    fz = C.c_int64()
+    assert L.eamd_dgetrf(NULL, -1, 4, NULL, 4, NULL, C.byref(nbt), C.byref(fz)) == -1
+    assert L.eamd_dgetrf(NULL, 4, -1, NULL, 4, NULL, C.byref(nbt), C.byref(fz)) == -2
+    assert L.eamd_dgetrf(NULL, 4, 4, NULL, 3, NULL, C.byref(nbt), C.byref(fz)) == -4
+
+
+def test_dgetrs_argument_codes():
+    L = eigen_amd.lib()
+    assert L.eamd_dgetrs(NULL, b"X", 4, 1, NULL, 4, NULL, NULL, 4) == -1
+    assert L.eamd_dgetrs(NULL, b"N", -1, 1, NULL, 4, NULL, NULL, 4) == -2
+    assert L.eamd_dgetrs(NULL, b"N", 4, -1, NULL, 4, NULL, NULL, 4) == -3
+    assert L.eamd_dgetrs(NULL, b"N", 4, 1, NULL, 3, NULL, NULL, 4) == -5
+    assert L.eamd_dgetrs(NULL, b"N", 4, 1, NULL, 4, NULL, NULL, 3) == -8
+
+
+def test_dpotrf_dpotrs_argument_codes():
+    L = eigen_amd.lib()
+    fc = C.c_int64()
+    assert L.eamd_dpotrf(NULL, b"X", 4, NULL, 4, C.byref(fc)) == -1
+    assert L.eamd_dpotrf(NULL, b"L", -1, NULL, 4, C.byref(fc)) == -2
+    assert L.eamd_dpotrf(NULL, b"L", 4, NULL, 3, C.byref(fc)) == -4
+    assert L.eamd_dpotrs(NULL, b"U", 4, -2, NULL, 4, NULL, 4) == -3
+    assert L.eamd_dpotrs(NULL, b"U", 4, 1, NULL, 4, NULL, 2) == -7
+
+
+def test_dgemm_dtrsm_dsyrk_argument_codes():
+    L = eigen_amd.lib()
+    assert L.eamd_dgemm(NULL, b"X", b"N", 2, 2, 2, 1.0, NULL, 2, NULL, 2, 0.0, NULL, 2) == -1
+    assert L.eamd_dgemm(NULL, b"N", b"Q", 2, 2, 2, 1.0, NULL, 2, NULL, 2, 0.0, NULL, 2) == -2
+    assert L.eamd_dgemm(NULL, b"N", b"N", 4, 2, 3, 1.0, NULL, 3, NULL, 3, 0.0, NULL, 4) == -8
+    assert L.eamd_dgemm(NULL, b"N", b"N", 4, 2, 3, 1.0, NULL, 4, NULL, 2, 0.0, NULL, 4) == -10
+    assert L.eamd_dgemm(NULL, b"N", b"N", 4, 2, 3, 1.0, NULL, 4, NULL, 3, 0.0, NULL, 3) == -13
+    assert L.eamd_dtrsm(NULL, b"X", b"L", b"N", b"N", 2, 2, 1.0, NULL, 2, NULL, 2) == -1
+    assert L.eamd_dtrsm(NULL, b"L", b"X", b"N", b"N", 2, 2, 1.0, NULL, 2, NULL, 2) == -2
+    assert L.eamd_dtrsm(NULL, b"R", b"L", b"N", b"N", 5, 3, 1.0, NULL, 2, NULL, 5) == -9
+    assert L.eamd_dsyrk(NULL, b"L", b"N", 5, 3, 1.0, NULL, 4, 0.0, NULL, 5) == -7
+
+
+def test_null_handle_after_valid_arguments_is_an_error():
+    L = eigen_amd.lib()
+    fc = C.c_int64()
+    assert L.eamd_dpotrf(NULL, b"L", 4, NULL, 4, C.byref(fc)) == -1
+    assert b"handle" in L.eamd_last_error()
+
+
+def test_ir_defaults():
+    o = _lib.IrOpts()
+    eigen_amd.lib().eamd_ir_opts_default(C.byref(o))
+    assert o.tol == pytest.approx(1e-14)
+    assert o.factor_precision == 4 and o.gmres == 1

@@ -1,0 +1,240 @@
+"""GPU parity tests of the FP64 drop-in path (GEMM, PartialPivLU, LLT,
+triangular solves) against the oracle and the reference fixtures, through
+the C ABI.  Pattern of the reference's differential CPU-vs-GPU tests
+(test/cuda_common.h:65-80): same seeded input on both sides."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import eigen_amd as ea
+from oracle import pyoracle as po
+from tests import inputs
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(__file__), "golden")
+EPS = np.finfo(np.float64).eps
+
+
+def dev(a):
+    return ea.colmajor(torch.from_numpy(np.asfortranarray(a)))
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def first_pivot_difference(t_gpu, t_ref, margin):
+    """Index of the first differing transposition, None if none; a difference
+    at a near-tie (runner-up within 1e-12 relative) is excused."""
+    d = np.nonzero(np.asarray(t_gpu) != np.asarray(t_ref))[0]
+    if len(d) == 0:
+        return None
+    k = int(d[0])
+    assert margin[k] < 1e-12, f"pivot {k}: gpu {t_gpu[k]} vs ref {t_ref[k]}, margin {margin[k]}"
+    return k
+
+
+# ----------------------------------------------------------------- GEMM --
+@pytest.mark.parametrize("m,n,k", [(8, 8, 8), (9, 9, 9), (24, 24, 24), (239, 239, 239),
+                                   (240, 240, 240), (2400, 24, 24), (24, 2400, 24),
+                                   (24, 24, 2400), (24, 2400, 2400), (2400, 24, 2400),
+                                   (2400, 2400, 24), (2400, 2400, 64), (4800, 23, 160),
+                                   (23, 4800, 160), (1, 1, 1), (129, 130, 131)])
+def test_dgemm_shapes(gpu, m, n, k):
+    a = inputs.normal(6000 + m + 7 * n + 13 * k, m, k)
+    b = inputs.normal(7000 + m + 7 * n + 13 * k, k, n)
+    c = host(ea.gemm(dev(a), dev(b)))
+    ref = a @ b
+    bound = 2 * k * EPS * (np.abs(a) @ np.abs(b)) + 1e-300
+    assert np.all(np.abs(c - ref) <= bound)
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_dgemm_transposes_alpha_beta(gpu, ta, tb):
+    m, n, k = 300, 200, 150
+    a = inputs.normal(1, *(m, k) if ta == "N" else (k, m))
+    b = inputs.normal(2, *(k, n) if tb == "N" else (n, k))
+    c0 = inputs.normal(3, m, n)
+    out = host(ea.gemm(dev(a), dev(b), dev(c0), alpha=-1.5, beta=0.25, transa=ta, transb=tb))
+    opa = a if ta == "N" else a.T
+    opb = b if tb == "N" else b.T
+    np.testing.assert_allclose(out, -1.5 * opa @ opb + 0.25 * c0, atol=1e-12)
+
+
+def test_dgemm_reference_fixture(gpu):
+    g = np.load(os.path.join(G, "gemm.npz"), allow_pickle=False)
+    for (m, n, k) in [(8, 8, 8), (24, 240, 24), (240, 240, 64)]:
+        a = inputs.normal(6000 + m + 7 * n + 13 * k, m, k)
+        b = inputs.normal(7000 + m + 7 * n + 13 * k, k, n)
+        c = host(ea.gemm(dev(a), dev(b)))
+        key = f"g{m}_{n}_{k}"
+        if key in g:
+            np.testing.assert_allclose(c, g[key], atol=1e-13 * k)
+        else:
+            np.testing.assert_allclose(c.ravel(order="F")[g[key + "_idx"]], g[key + "_val"],
+                                       atol=1e-13 * k)
+
+
+def test_dsyrk_lower_and_upper(gpu):
+    a = inputs.normal(5, 300, 70)
+    c0 = inputs.normal(6, 300, 300)
+    for uplo in "LU":
+        c = dev(c0)
+        ea.rank_update(c, dev(a), alpha=-1.0, uplo=uplo)
+        out = host(c)
+        full = c0 - a @ a.T
+        tri = np.tril if uplo == "L" else np.triu
+        other = np.triu if uplo == "L" else np.tril
+        np.testing.assert_allclose(tri(out), tri(full), atol=1e-12)
+        np.testing.assert_array_equal(other(out, 1 if uplo == "L" else -1),
+                                      other(c0, 1 if uplo == "L" else -1))
+
+
+# ------------------------------------------------------------------- LU --
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 15, 16, 17, 31, 32, 33, 64, 127, 128, 129,
+                               255, 256, 257])
+def test_lu_sweep_against_reference_fixture(gpu, n):
+    g = np.load(os.path.join(G, "lu.npz"), allow_pickle=False)
+    a = inputs.normal(1000 + n, n, n)
+    assert inputs.sha256(a) == str(g[f"n{n}_sha"])
+    lu = ea.PartialPivLU(dev(a))
+    t = host(lu.transpositionsP())
+    _, _, _, _, margin = po.lu(a, margins=True)
+    k = first_pivot_difference(t, g[f"n{n}_transp"], margin)
+    if k is None:
+        assert lu.nb_transpositions() == int(g[f"n{n}_nbt"])
+        np.testing.assert_array_equal(host(lu.permutationP()), g[f"n{n}_perm"])
+        if n <= 64:
+            np.testing.assert_allclose(host(lu.matrixLU()), g[f"n{n}_lu"], atol=1e-12 * n)
+        assert lu.determinant() == pytest.approx(float(g[f"n{n}_det"]), rel=1e-9)
+    b = inputs.normal(2000 + n, n)
+    x = host(lu.solve(dev(b).reshape(-1)))
+    assert po.backward_error(a, x, b) <= 1e-14
+    rec = host(lu.reconstructedMatrix())
+    np.testing.assert_allclose(rec, a, atol=1e-12 * max(1, n))
+
+
+def test_lu_plain_fp64_config_n1024(gpu):
+    """The plain drop-in configuration: n = 1024 N(0,1), seeded; pivots and
+    backward error compared with Eigen's (fixture) and the oracle."""
+    g = np.load(os.path.join(G, "lu.npz"), allow_pickle=False)
+    a = inputs.normal(1, 1024, 1024)
+    b = inputs.normal(2, 1024)
+    lu = ea.PartialPivLU(dev(a))
+    t = host(lu.transpositionsP())
+    _, _, _, _, margin = po.lu(a, margins=True)
+    k = first_pivot_difference(t, g["c1_transp"], margin)
+    assert k is None, "no near-ties at this seed: pivots must be identical"
+    x = host(lu.solve(dev(b).reshape(-1)))
+    be = po.backward_error(a, x, b)
+    assert be <= 1e-14
+    # device-side backward error agrees with the long-double host one
+    np.testing.assert_allclose(x, g["c1_x"], rtol=1e-8, atol=1e-10)
+
+
+def test_lu_zero_pivot_and_tie_rules(gpu):
+    g = np.load(os.path.join(G, "lu.npz"), allow_pickle=False)
+    z = inputs.normal(77, 40, 40)
+    z[:, 5] = 0.0
+    lu = ea.PartialPivLU(dev(z))
+    assert lu.first_zero_pivot() == 5
+    np.testing.assert_array_equal(host(lu.transpositionsP()), g["zero_transp"])
+    np.testing.assert_allclose(host(lu.matrixLU()), g["zero_lu"], atol=1e-12)
+    tie = np.array([[1.0, 2.0, 0.0], [-1.0, 1.0, 3.0], [1.0, 0.0, 1.0]])
+    lu = ea.PartialPivLU(dev(tie))
+    np.testing.assert_array_equal(host(lu.transpositionsP()), g["tie_transp"])
+    np.testing.assert_allclose(host(lu.matrixLU()), g["tie_lu"])
+
+
+def test_lu_nan_rule(gpu):
+    lu = ea.PartialPivLU(dev(np.array([[1.0, 0.0], [np.nan, 1.0]])))
+    assert host(lu.transpositionsP())[0] == 0
+    lu = ea.PartialPivLU(dev(np.array([[np.nan, 0.0], [5.0, 1.0]])))
+    assert host(lu.transpositionsP())[0] == 0
+
+
+@pytest.mark.parametrize("n", [600, 2048])
+def test_lu_multi_workgroup_panel(gpu, n):
+    """Sizes whose panels span several workgroups (grid-wide pivot exchange)."""
+    a = inputs.normal(31 + n, n, n)
+    lu = ea.PartialPivLU(dev(a))
+    t = host(lu.transpositionsP())
+    _, t_ref, _, _, margin = po.lu(a, margins=True)
+    first_pivot_difference(t, t_ref, margin)
+    b = inputs.normal(32 + n, n, 3)
+    x = host(lu.solve(dev(b)))
+    assert po.backward_error(a, x, b) <= 1e-14
+    xt = host(lu.solve(dev(b), trans="T"))
+    assert po.backward_error(np.asfortranarray(a.T), xt, b) <= 1e-14
+
+
+def test_lu_host_buffer_entry_point(gpu):
+    import ctypes as C
+    n = 100
+    a = inputs.normal(4, n, n)
+    lu_h = np.asfortranarray(a.copy())
+    t = np.zeros(n, dtype=np.int32)
+    nbt, fz = C.c_int32(), C.c_int64()
+    st = ea.lib().eamd_host_dgetrf(ea.handle(), n, n, lu_h.ctypes.data, n, t.ctypes.data,
+                                   C.byref(nbt), C.byref(fz))
+    assert st == 0
+    lu_o, t_o, nbt_o, fz_o, margin = po.lu(a, margins=True)
+    first_pivot_difference(t, t_o, margin)
+    np.testing.assert_allclose(lu_h, lu_o, atol=1e-11)
+
+
+# ------------------------------------------------------------------ LLT --
+@pytest.mark.parametrize("n", [1, 6, 9, 24, 31, 32, 33, 127, 128, 129, 300, 1000])
+def test_llt_against_oracle(gpu, n):
+    a = inputs.spd(3000 + n, n)
+    llt = ea.LLT(dev(a))
+    assert llt.info() == ea.Success
+    L = np.tril(host(llt.matrixLLT()))
+    Lo, f = po.llt(a)
+    np.testing.assert_allclose(L, np.tril(Lo), rtol=1e-10, atol=1e-10 * np.sqrt(n))
+    # strictly upper part untouched, as the reference leaves it
+    np.testing.assert_array_equal(np.triu(host(llt.matrixLLT()), 1), np.triu(a, 1))
+    b = inputs.normal(4000 + n, n, 2)
+    x = host(llt.solve(dev(b)))
+    assert po.backward_error(a, x, b) <= 1e-14
+
+
+def test_llt_upper(gpu):
+    n = 200
+    a = inputs.spd(9, n)
+    llt = ea.LLT(dev(a), uplo="U")
+    assert llt.info() == ea.Success
+    U = np.triu(host(llt.matrixLLT()))
+    np.testing.assert_allclose(U.T @ U, a, atol=1e-10 * n)
+    b = inputs.normal(10, n)
+    x = host(llt.solve(dev(b).reshape(-1)))
+    assert po.backward_error(a, x, b) <= 1e-14
+
+
+@pytest.mark.parametrize("n", [10, 40, 200])
+def test_llt_failing_column(gpu, n):
+    g = np.load(os.path.join(G, "llt.npz"), allow_pickle=False)
+    a = inputs.spd(5000 + n, n)
+    bad = int(g[f"bad{n}_bad"])
+    a[bad, bad] = -a[bad, bad] - 1e3
+    llt = ea.LLT(dev(a))
+    assert llt.info() == ea.NumericalIssue
+    assert llt.fail_column() == int(g[f"bad{n}_fail"])
+
+
+# ----------------------------------------------------------------- TRSM --
+@pytest.mark.parametrize("side,uplo,trans,diag",
+                         list(itertools.product("LR", "LU", "NT", "NU")))
+def test_trsm_all_variants(gpu, side, uplo, trans, diag):
+    m, n = 150, 70
+    na = m if side == "L" else n
+    t = inputs.normal(11, na, na) + 4 * np.sqrt(na) * np.eye(na)
+    t = np.asfortranarray(np.tril(t) if uplo == "L" else np.triu(t))
+    b = inputs.normal(12, m, n)
+    out = dev(b)
+    ea.triangular_solve(dev(t), out, side=side, uplo=uplo, trans=trans, diag=diag, alpha=2.0)
+    ref = po.trsm(side, uplo, trans, diag, t, b, alpha=2.0)
+    np.testing.assert_allclose(host(out), ref, rtol=1e-10, atol=1e-11)

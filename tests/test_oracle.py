@@ -1,0 +1,291 @@
+"""CPU tests of the checker itself (oracle/oracle.c), pinned against
+(1) the golden fixtures written from the reference (tests/golden/*.npz, made
+by compiled Eigen), (2) closed forms and invariants, (3) brute force on tiny
+inputs.  Each check is chosen so that a plausible mistake in the restatement
+(dropped term, wrong sign or index, transposed operand) fails it."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import inputs
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def glu():
+    return np.load(os.path.join(G, "lu.npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def gllt():
+    return np.load(os.path.join(G, "llt.npz"), allow_pickle=False)
+
+
+# ------------------------------------------------------------------- LU --
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 15, 16, 17, 31, 32, 33, 64, 127, 128, 129,
+                               255, 256, 257])
+def test_lu_matches_reference_fixture(glu, n):
+    a = inputs.normal(1000 + n, n, n)
+    assert inputs.sha256(a) == str(glu[f"n{n}_sha"]), "input generator diverged"
+    lu, t, nbt, fz, margin = po.lu(a, margins=True)
+    ref_t = glu[f"n{n}_transp"]
+    diff = np.nonzero(t != ref_t)[0]
+    # identical pivots except at a near-tie (none expected at these seeds)
+    if len(diff):
+        assert margin[diff[0]] < 1e-12, f"pivot differs at column {diff[0]}"
+    else:
+        assert nbt == int(glu[f"n{n}_nbt"])
+        assert fz == int(glu[f"n{n}_fz"])
+    if n <= 64:
+        np.testing.assert_allclose(lu, glu[f"n{n}_lu"], rtol=0, atol=1e-12 * max(1, n))
+    else:
+        assert abs(np.linalg.norm(lu) - float(glu[f"n{n}_lu_fro"])) <= 1e-10 * np.linalg.norm(lu)
+    # permutation indices as PartialPivLU::permutationP() (PermutationMatrix.h:82-88)
+    np.testing.assert_array_equal(po.perm_from_transpositions(t), glu[f"n{n}_perm"])
+    b = inputs.normal(2000 + n, n)
+    x = po.lu_solve(lu, t, b)[:, 0]
+    np.testing.assert_allclose(x, glu[f"n{n}_x"], rtol=1e-9, atol=1e-9)
+    # determinant = det_p * prod(diag U)
+    det = (-1.0) ** nbt * np.prod(np.diag(lu))
+    assert det == pytest.approx(float(glu[f"n{n}_det"]), rel=1e-10)
+
+
+def test_lu_c1_instance_matches_reference(glu):
+    a = inputs.normal(1, 1024, 1024)
+    b = inputs.normal(2, 1024)
+    assert inputs.sha256(a) == str(glu["c1_sha_a"])
+    assert inputs.sha256(b) == str(glu["c1_sha_b"])
+    lu, t, nbt, fz, margin = po.lu(a, margins=True)
+    diff = np.nonzero(t != glu["c1_transp"])[0]
+    assert len(diff) == 0 or margin[diff[0]] < 1e-12
+    x = po.lu_solve(lu, t, b)[:, 0]
+    be = po.backward_error(a, x, b)
+    assert be < 1e-15
+    assert float(glu["c1_berr"]) < 1e-15
+    np.testing.assert_allclose(x, glu["c1_x"], rtol=1e-8, atol=1e-10)
+
+
+def test_lu_reconstruction_pa_equals_lu():
+    for n in (3, 20, 70, 200):
+        a = inputs.normal(5 + n, n, n)
+        lu, t, nbt, fz = po.lu(a)
+        L = np.tril(lu, -1) + np.eye(n)
+        U = np.triu(lu)
+        perm = po.perm_from_transpositions(t)
+        # (P A)[perm[i]] = A[i]
+        pa = np.empty_like(a)
+        pa[perm] = a
+        np.testing.assert_allclose(L @ U, pa, atol=1e-12 * n)
+        assert np.all(np.abs(L) <= 1.0 + 1e-15)  # partial pivoting bound
+
+
+def test_lu_upper_triangular_input_has_identity_pivots():
+    a = np.asfortranarray(np.triu(inputs.normal(9, 50, 50)) + 10 * np.eye(50))
+    # make the diagonal dominant in its column so no interchange happens
+    lu, t, nbt, fz = po.lu(a)
+    np.testing.assert_array_equal(t, np.arange(50))
+    assert nbt == 0
+    np.testing.assert_allclose(lu, a)
+
+
+def test_lu_permutation_matrix_input():
+    n = 33
+    p = inputs.rng(4).permutation(n)
+    a = np.asfortranarray(np.eye(n)[p])
+    lu, t, nbt, fz = po.lu(a)
+    np.testing.assert_allclose(lu, np.eye(n))
+    perm = po.perm_from_transpositions(t)
+    pa = np.empty_like(a)
+    pa[perm] = a
+    np.testing.assert_allclose(pa, np.eye(n))
+
+
+def test_lu_zero_column_reports_first_zero_pivot(glu):
+    z = inputs.normal(77, 40, 40)
+    z[:, 5] = 0.0
+    lu, t, nbt, fz = po.lu(z)
+    assert fz == int(glu["zero_fz"]) == 5
+    np.testing.assert_array_equal(t, glu["zero_transp"])
+    np.testing.assert_allclose(lu, glu["zero_lu"], atol=1e-12)
+
+
+def test_lu_exact_tie_takes_lowest_row(glu):
+    tie = np.asfortranarray(np.array([[1.0, 2.0, 0.0], [-1.0, 1.0, 3.0], [1.0, 0.0, 1.0]]))
+    lu, t, nbt, fz = po.lu(tie)
+    np.testing.assert_array_equal(t, glu["tie_transp"])
+    assert t[0] == 0  # |1| == |-1| == |1|: first index wins (Visitor.h:179)
+    np.testing.assert_allclose(lu, glu["tie_lu"])
+
+
+def test_lu_nan_never_wins_below_the_diagonal():
+    a = np.asfortranarray(np.array([[1.0, 0.0], [np.nan, 1.0]]))
+    lu, t, nbt, fz = po.lu(a)
+    assert t[0] == 0
+    a = np.asfortranarray(np.array([[np.nan, 0.0], [5.0, 1.0]]))
+    lu, t, nbt, fz = po.lu(a)
+    assert t[0] == 0  # NaN in the first position is the scan's initial max
+
+
+def test_transpositions_to_permutation_brute_force():
+    for n in range(1, 6):
+        for t in itertools.product(*[range(k, n) for k in range(n)]):
+            t = np.array(t, dtype=np.int32)
+            perm = po.perm_from_transpositions(t)
+            # apply the transpositions in order to rows of the identity
+            x = np.arange(n)
+            for k in range(n):
+                x[[k, t[k]]] = x[[t[k], k]]
+            # P*b with b = arange: (P b)[perm[i]] = b[i]
+            pb = np.empty(n, dtype=int)
+            pb[perm] = np.arange(n)
+            np.testing.assert_array_equal(pb, x)
+
+
+# ------------------------------------------------------------------ LLT --
+@pytest.mark.parametrize("n", [1, 6, 9, 12, 15, 18, 24, 31, 32, 33, 127, 128, 129])
+def test_llt_matches_reference_fixture(gllt, n):
+    a = inputs.spd(3000 + n, n)
+    assert inputs.sha256(a) == str(gllt[f"n{n}_sha"])
+    L, fail = po.llt(a)
+    assert fail == -1 and int(gllt[f"n{n}_info"]) == 0
+    if n <= 33:
+        np.testing.assert_allclose(np.tril(L), gllt[f"n{n}_L"], rtol=1e-13, atol=1e-13)
+    b = inputs.normal(4000 + n, n)
+    x = po.llt_solve(L, b)[:, 0]
+    np.testing.assert_allclose(x, gllt[f"n{n}_x"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(np.tril(L) @ np.tril(L).T, a, atol=1e-12 * n * np.abs(a).max())
+
+
+@pytest.mark.parametrize("n", [10, 40, 200])
+def test_llt_failing_column_matches_reference(gllt, n):
+    a = inputs.spd(5000 + n, n)
+    bad = int(gllt[f"bad{n}_bad"])
+    a[bad, bad] = -a[bad, bad] - 1e3
+    L, fail = po.llt(a)
+    assert fail == int(gllt[f"bad{n}_fail"]) == bad
+
+
+def test_llt_diagonal_closed_form():
+    d = np.array([4.0, 9.0, 0.25, 16.0])
+    L, fail = po.llt(np.diag(d))
+    assert fail == -1
+    np.testing.assert_allclose(np.diag(L), np.sqrt(d))
+    L, fail = po.llt(np.diag([1.0, 2.0, -3.0, 4.0]))
+    assert fail == 2
+
+
+def test_llt_float_matches_double_within_single_precision():
+    a = inputs.spd(8, 24)
+    Ld, _ = po.llt(a)
+    Lf, _ = po.llt(a.astype(np.float32), dtype=np.float32)
+    np.testing.assert_allclose(np.tril(Lf), np.tril(Ld), rtol=2e-5, atol=2e-5)
+
+
+# --------------------------------------------------------------- level 3 --
+def test_gemm_matches_reference_fixture():
+    g = np.load(os.path.join(G, "gemm.npz"), allow_pickle=False)
+    shapes = [(8, 8, 8), (9, 9, 9), (24, 24, 24), (239, 239, 239), (240, 240, 240),
+              (240, 24, 24), (24, 240, 24), (24, 24, 240), (24, 240, 240), (240, 24, 240),
+              (240, 240, 24), (240, 240, 64), (240, 23, 160), (23, 240, 160)]
+    for (m, n, k) in shapes:
+        a = inputs.normal(6000 + m + 7 * n + 13 * k, m, k)
+        b = inputs.normal(7000 + m + 7 * n + 13 * k, k, n)
+        c = po.gemm(a, b)
+        key = f"g{m}_{n}_{k}"
+        tol = 1e-13 * k
+        if key in g:
+            np.testing.assert_allclose(c, g[key], atol=tol)
+        else:
+            np.testing.assert_allclose(c.sum(axis=1), g[key + "_rows"], atol=tol * n)
+            np.testing.assert_allclose(c.sum(axis=0), g[key + "_cols"], atol=tol * m)
+            np.testing.assert_allclose(c.ravel(order="F")[g[key + "_idx"]], g[key + "_val"],
+                                       atol=tol)
+
+
+def test_gemm_closed_forms_and_transposes():
+    a = inputs.normal(1, 7, 5)
+    b = inputs.normal(2, 5, 6)
+    np.testing.assert_allclose(po.gemm(np.eye(7), a), a)
+    np.testing.assert_allclose(po.gemm(a, b), a @ b, atol=1e-14)
+    np.testing.assert_allclose(po.gemm(a.T, b, transa="T"), a @ b, atol=1e-14)
+    np.testing.assert_allclose(po.gemm(a, b.T, transb="T"), a @ b, atol=1e-14)
+    u = inputs.normal(3, 7)
+    v = inputs.normal(4, 6)
+    np.testing.assert_allclose(po.gemm(u.reshape(7, 1), v.reshape(1, 6)), np.outer(u, v))
+    c0 = inputs.normal(5, 7, 6)
+    np.testing.assert_allclose(po.gemm(a, b, c0, alpha=2.0, beta=-0.5), 2 * a @ b - 0.5 * c0,
+                               atol=1e-13)
+
+
+@pytest.mark.parametrize("side,uplo,trans,diag",
+                         list(itertools.product("LR", "LU", "NT", "NU")))
+def test_trsm_round_trip(side, uplo, trans, diag):
+    m, n = 9, 7
+    na = m if side == "L" else n
+    t = inputs.normal(11, na, na) + 4 * np.eye(na)
+    t = np.tril(t) if uplo == "L" else np.triu(t)
+    top = t.copy()
+    if diag == "U":
+        np.fill_diagonal(top, 1.0)
+    op = top.T if trans == "T" else top
+    b = inputs.normal(12, m, n)
+    x = po.trsm(side, uplo, trans, diag, t, b, alpha=1.5)
+    lhs = op @ x if side == "L" else x @ op
+    np.testing.assert_allclose(lhs, 1.5 * b, atol=1e-12)
+    r = po.ref()
+    if r is not None:  # the reference itself on the same call
+        xr = np.asfortranarray(1.5 * b)
+        r.eref_trsm_d(side.encode(), uplo.encode(), trans.encode(), diag.encode(), m, n,
+                      np.asfortranarray(t), na, xr, m)
+        np.testing.assert_allclose(x, xr, atol=1e-12)
+
+
+def test_syrk_lower_equals_brute_force():
+    a = inputs.normal(3, 10, 4)
+    c = inputs.normal(4, 10, 10)
+    out = po.syrk("L", "N", a, c, alpha=-1.0, beta=1.0)
+    full = c - a @ a.T
+    np.testing.assert_allclose(np.tril(out), np.tril(full), atol=1e-13)
+    np.testing.assert_array_equal(np.triu(out, 1), np.triu(c, 1))
+
+
+def test_backward_error_of_exact_solution_is_zero():
+    a = np.asfortranarray(np.diag([2.0, 4.0, 8.0]))
+    x = np.array([1.0, 1.0, 1.0])
+    b = a @ x
+    assert po.backward_error(a, x, b) == 0.0
+    assert po.backward_error(a, x + np.array([1.0, 0, 0]), b) == pytest.approx(2.0 / (8 * 2 + 8))
+
+
+# -------------------------------------------------------------- batched --
+def test_batched_llt_sample_matches_reference():
+    g = np.load(os.path.join(G, "batched.npz"), allow_pickle=False)
+    n, offA, A, offb, b = inputs.packed_systems(11, 1000, [6, 9, 12, 15, 18, 24], np.float32,
+                                                True)
+    assert inputs.sha256(A) == str(g["llt_sha"])
+    x, info = po.batched_llt_solve_f(offA, n, A, offb, b)
+    assert np.all(info == -1) and np.all(g["llt_info"] == -1)
+    np.testing.assert_allclose(x, g["llt_x"], rtol=1e-4, atol=1e-5)
+
+
+def test_batched_lu_sample_matches_reference():
+    g = np.load(os.path.join(G, "batched.npz"), allow_pickle=False)
+    n, offA, A, offB, B = inputs.packed_systems(12, 48, (16, 256), np.float64, False, nrhs=4)
+    assert inputs.sha256(A) == str(g["lu_sha"])
+    X, info = po.batched_lu_solve_d(offA, n, A, offB, 4, B)
+    np.testing.assert_allclose(X, g["lu_x"], rtol=1e-8, atol=1e-9)
+
+
+def test_batched_equals_single_system_routines():
+    n, offA, A, offb, b = inputs.packed_systems(13, 20, [6, 9, 24], np.float32, True)
+    x, info = po.batched_llt_solve_f(offA, n, A, offb, b)
+    for i in range(20):
+        k = int(n[i])
+        a = A[offA[i]:offA[i] + k * k].reshape(k, k, order="F")
+        L, f = po.llt(a, dtype=np.float32)
+        xi = po.llt_solve(L, b[offb[i]:offb[i] + k])[:, 0]
+        np.testing.assert_array_equal(xi, x[offb[i]:offb[i] + k])
