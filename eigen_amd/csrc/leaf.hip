@@ -1,0 +1,433 @@
+// leaf.hip -- the partial-pivoting column loop of the LU panel.
+//
+// Restates unblocked_lu (Eigen/src/LU/PartialPivLU.h:365-409) for a leaf of
+// at most 32 columns of a tall row-major sub-panel: per column, the first
+// maximum of |a| (strict '>' scan in row order, Visitor.h:179; a NaN wins only
+// from the diagonal position), the row interchange, the division of the
+// column by the pivot and the rank-1 update of the leaf.
+//
+// MI355X mapping.  Every thread keeps one row of the leaf (32 values) in
+// registers, so the rank-1 update is register FMAs; a workgroup holds
+// BLOCK rows (512 in FP64, 1024 in FP32) and at most 32 workgroups cover a
+// 16384-row panel.  With several workgroups, each column costs ONE memory
+// round trip: every workgroup publishes its candidate (value, row) together
+// with the whole candidate row, and the owner of row j publishes row j, all
+// as 8-byte words carrying 32 bits of payload and the column's epoch (an
+// aligned 8-byte store is single-copy atomic, so a word with the right epoch
+// has the right payload: no fences).  Every workgroup then reads all
+// candidates and row j in one parallel poll and resolves the winner locally.
+#include "common.h"
+#include "kernels.h"
+
+namespace eamd {
+
+namespace {
+
+constexpr int LW = 32;       // leaf width
+constexpr int kMaxG = 32;    // workgroups in the exchange
+constexpr int kLeafSpin = 1 << 22;
+typedef unsigned long long u64;
+
+template <typename T>
+struct Words {
+  static constexpr int n = sizeof(T) / 4;  // 32-bit payload words per value
+};
+
+template <typename T>
+struct LeafBlock {
+  static constexpr int v = sizeof(T) == 8 ? 512 : 1024;  // rows per workgroup
+};
+
+__device__ __forceinline__ void put_word(u64* p, unsigned payload, unsigned epoch) {
+  __hip_atomic_store(p, ((u64)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ void put_value(u64* p, T v, unsigned epoch) {
+  if (sizeof(T) == 8) {
+    u64 bits = (u64)__double_as_longlong((double)v);
+    put_word(p, (unsigned)bits, epoch);
+    put_word(p + 1, (unsigned)(bits >> 32), epoch);
+  } else {
+    put_word(p, __float_as_uint((float)v), epoch);
+  }
+}
+
+// Bit-mask selection (a ? x : y with a mask of all ones or zeros).  Written
+// on the integer bits so the compiler does not fold a chain of them back
+// into a runtime-indexed access of the register array (which it would place
+// in scratch memory).
+__device__ __forceinline__ float mask_sel(unsigned m, float x, float y) {
+  return __uint_as_float((__float_as_uint(x) & m) | (__float_as_uint(y) & ~m));
+}
+__device__ __forceinline__ double mask_sel(unsigned m, double x, double y) {
+  const unsigned long long mm = ((unsigned long long)m << 32) | m;
+  return __longlong_as_double((long long)(((unsigned long long)__double_as_longlong(x) & mm) |
+                                          ((unsigned long long)__double_as_longlong(y) & ~mm)));
+}
+
+template <typename T>
+__device__ __forceinline__ T get_value(const unsigned* w) {
+  if (sizeof(T) == 8) return (T)__longlong_as_double((long long)(((u64)w[1] << 32) | w[0]));
+  return (T)__uint_as_float(w[0]);
+}
+
+__device__ __forceinline__ bool beats(double s, int i, double bs, int bi) {
+  return s > bs || (s == bs && i < bi);
+}
+
+__device__ __forceinline__ double score_of(double v, bool first) {
+  double s = fabs(v);
+  if (s != s) s = first ? INFINITY : -1.0;
+  return s;
+}
+
+// Slot (words): [0..1] candidate value (FP64 bits), [2] row or ~0u (no
+// candidate), [3] unused, [4, 4 + LW*Words) candidate row.  The slot after
+// the last workgroup's holds row j (from word 4).
+template <typename T>
+constexpr int slot_words() {
+  return 4 + LW * Words<T>::n;
+}
+
+}  // namespace
+
+// Outputs: piv[c] leaf-relative transposition of leaf column c; leaf
+// permutation lists (rows touched, leaf-relative) lperm_n/lpos/lsrc; the
+// panel-level lists pperm_n/ppos/psrc (panel-relative rows, composed across
+// the leaves of a panel: on entry they hold the permutation of the previous
+// leaves; `row0` is the leaf's first row inside the panel, `pnb` the panel
+// width); zero_col the first exactly-zero pivot column of the leaf or -1.
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
+    T* __restrict__ P, int64_t ldp, int m, int w, int G, int* __restrict__ piv,
+    int* __restrict__ lperm_n, int* __restrict__ lpos, int* __restrict__ lsrc,
+    int* __restrict__ pperm_n, int* __restrict__ ppos, int* __restrict__ psrc, int row0,
+    int pnb, int* __restrict__ zero_col, u64* __restrict__ xbuf, int* __restrict__ err,
+    unsigned epoch_base) {
+  constexpr int NW = Words<T>::n;
+  constexpr int SW = slot_words<T>();
+  constexpr int NWAVE = BLOCK / 64;
+  constexpr int STAGE = (kMaxG + 1) * SW;
+  constexpr int KMAX = (STAGE + BLOCK - 1) / BLOCK;  // words polled per thread
+  __shared__ double red_s[NWAVE];
+  __shared__ int red_i[NWAVE];
+  __shared__ double red_v[NWAVE];
+  __shared__ T urow[2][LW];   // pivot row received from the exchange
+  __shared__ T rowj[2][LW];   // old row j received from the exchange
+  __shared__ T cand[2][LW];   // this workgroup's candidate row
+  __shared__ T rowjL[2][LW];  // row j, when this workgroup owns it
+  __shared__ unsigned stage[STAGE];
+  __shared__ int s_abort;
+  // bookkeeping (workgroup 0): leaf-level and panel-level slot tables
+  __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
+  __shared__ int pPos[512], pCont[512], s_pnS;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = blockIdx.x;
+  const int i = g * BLOCK + tid;  // my leaf-relative row
+  const bool valid = i < m;
+
+  T v[LW];
+  {
+    const T* src = P + (int64_t)i * ldp;
+#pragma unroll
+    for (int c = 0; c < LW; ++c) v[c] = (valid && c < w) ? src[c] : T(0);
+  }
+  if (g == 0) {
+    // leaf rows 0..w-1 are slots 0..w-1; rows swapped in from below are appended
+    for (int s = tid; s < w; s += BLOCK) { lPos[s] = s; lCont[s] = s; }
+    if (tid == 0) { s_lnS = w; s_pnS = *pperm_n; }
+    __syncthreads();
+    for (int s = tid; s < s_pnS; s += BLOCK) { pPos[s] = ppos[s]; pCont[s] = psrc[s]; }
+  }
+  if (tid == 0) s_abort = 0;
+  __syncthreads();
+
+  int first_zero = -1;
+  // A runtime column loop over a register row: every access to v[] uses a
+  // compile-time index, with the column selected by branch-free selects and
+  // masked FMAs (a runtime index into a register array would go to scratch
+  // memory; a fully unrolled loop overflows the instruction cache).
+  auto column = [&](int j) __attribute__((always_inline)) -> bool {
+    const int par = j & 1;
+    const unsigned epoch = epoch_base + (unsigned)j + 1u;
+    T vj = v[0];
+#pragma unroll
+    for (int c = 1; c < LW; ++c) vj = mask_sel(0u - (unsigned)(c == j), v[c], vj);
+    // ---- 1. my candidate, then the workgroup's
+    double best = -2.0, bval = 0.0;
+    int bi = 0x7fffffff;
+    if (valid && i >= j) {
+      best = score_of((double)vj, i == j);
+      bi = i;
+      bval = (double)vj;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double os = __shfl_xor(best, off);
+      int oi = __shfl_xor(bi, off);
+      double ov = __shfl_xor(bval, off);
+      if (beats(os, oi, best, bi)) { best = os; bi = oi; bval = ov; }
+    }
+    if (lane == 0) { red_s[wave] = best; red_i[wave] = bi; red_v[wave] = bval; }
+    __syncthreads();
+    best = red_s[0];
+    bi = red_i[0];
+    bval = red_v[0];
+#pragma unroll
+    for (int q = 1; q < NWAVE; ++q)
+      if (beats(red_s[q], red_i[q], best, bi)) { best = red_s[q]; bi = red_i[q]; bval = red_v[q]; }
+    // ---- 2. candidate row and row j into LDS (the rows stay in registers)
+    const bool none = bi == 0x7fffffff;
+    if (i == bi) {
+#pragma unroll
+      for (int c = 0; c < LW; ++c) cand[par][c] = v[c];
+    }
+    if (i == j) {
+#pragma unroll
+      for (int c = 0; c < LW; ++c) rowjL[par][c] = v[c];
+    }
+    __syncthreads();
+    const T* up = cand[par];   // pivot row
+    const T* rp = rowjL[par];  // old row j
+    int p = bi;
+    if (G > 1) {
+      // ---- 3. publish: header, candidate row and (owner) row j, one word per lane
+      u64* myslot = xbuf + ((size_t)par * (G + 1) + g) * SW;
+      u64* rj = xbuf + ((size_t)par * (G + 1) + G) * SW;
+      if (tid == 0) {
+        u64 bits = (u64)__double_as_longlong(bval);
+        put_word(myslot, (unsigned)bits, epoch);
+        put_word(myslot + 1, (unsigned)(bits >> 32), epoch);
+        put_word(myslot + 2, none ? 0xffffffffu : (unsigned)bi, epoch);
+        put_word(myslot + 3, 0u, epoch);
+      } else if (tid >= 64 && tid < 64 + LW * NW) {
+        const int q = tid - 64, c = q / NW, half = q - c * NW;
+        const T val = none ? T(0) : cand[par][c];
+        unsigned pl;
+        if (sizeof(T) == 8) {
+          u64 bits = (u64)__double_as_longlong((double)val);
+          pl = half ? (unsigned)(bits >> 32) : (unsigned)bits;
+        } else {
+          pl = __float_as_uint((float)val);
+        }
+        put_word(myslot + 4 + q, pl, epoch);
+      } else if (j >= g * BLOCK && j < (g + 1) * BLOCK && tid >= 128 && tid < 128 + 4 + LW * NW) {
+        const int q = tid - 128;
+        unsigned pl = 0;
+        if (q >= 4) {
+          const int c = (q - 4) / NW, half = (q - 4) - c * NW;
+          const T val = rowjL[par][c];
+          if (sizeof(T) == 8) {
+            u64 bits = (u64)__double_as_longlong((double)val);
+            pl = half ? (unsigned)(bits >> 32) : (unsigned)bits;
+          } else {
+            pl = __float_as_uint((float)val);
+          }
+        }
+        put_word(rj + q, pl, epoch);
+      }
+      // ---- 4. one parallel poll of every slot and of row j: each thread
+      // issues the loads of its words together, then re-polls the missing ones
+      const int total = (G + 1) * SW;
+      const u64* base = xbuf + (size_t)par * (G + 1) * SW;
+      bool done[KMAX];
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) done[k] = tid + k * BLOCK >= total;
+      int bad = 0;
+      for (int spins = 0;; ++spins) {
+        u64 wv[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+          if (!done[k])
+            wv[k] = __hip_atomic_load(const_cast<u64*>(base + tid + k * BLOCK), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+          if (!done[k]) {
+            if ((unsigned)(wv[k] >> 32) == epoch) {
+              stage[tid + k * BLOCK] = (unsigned)wv[k];
+              done[k] = true;
+            } else {
+              all = false;
+            }
+          }
+        if (all) break;
+        if ((spins & 255) == 255 &&
+            (spins > kLeafSpin ||
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          bad = 1;
+          break;
+        }
+      }
+      if (bad) s_abort = 1;
+      __syncthreads();
+      if (s_abort) {
+        if (tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+      // ---- resolve the winner (every thread, G headers from LDS)
+      double wb = -2.0;
+      int wi = 0x7fffffff, wslot = 0;
+      for (int s2 = 0; s2 < G; ++s2) {
+        const unsigned* hh = &stage[s2 * SW];
+        const unsigned idx = hh[2];
+        if (idx == 0xffffffffu) continue;
+        const double val = get_value<double>(hh);
+        const double sc = score_of(val, (int)idx == j);
+        if (beats(sc, (int)idx, wb, wi)) { wb = sc; wi = (int)idx; wslot = s2; }
+      }
+      p = wi;
+      if (tid < LW) {
+        urow[par][tid] = get_value<T>(&stage[wslot * SW + 4 + tid * NW]);
+        rowj[par][tid] = get_value<T>(&stage[G * SW + 4 + tid * NW]);
+      }
+      __syncthreads();
+      up = urow[par];
+      rp = rowj[par];
+    }
+    // ---- 5. interchange in registers
+    if (p != j) {
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < LW; ++c) v[c] = up[c];
+      } else if (i == p) {
+#pragma unroll
+        for (int c = 0; c < LW; ++c) v[c] = rp[c];
+      }
+    }
+    // ---- bookkeeping (workgroup 0, wave 0)
+    if (g == 0 && wave == 0) {
+      if (lane == 0) piv[j] = p;
+      if (p != j) {
+        // leaf-level table: rows 0..w-1 are slots 0..w-1
+        int slotb = p;
+        if (p >= w) {
+          const int nS = s_lnS;
+          int found = -1;
+          for (int s2 = w + lane; s2 < nS; s2 += 64)
+            if (lPos[s2] == p) found = s2;
+          unsigned long long bal = __ballot(found >= 0);
+          slotb = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
+          if (lane == 0 && !bal) { lPos[nS] = p; lCont[nS] = nS; s_lnS = nS + 1; }
+        }
+        // panel-level table: rows 0..pnb-1 are slots 0..pnb-1
+        const int pa = row0 + j, pb = row0 + p;
+        int pslot = pb;
+        if (pb >= pnb) {
+          const int nS = s_pnS;
+          int found = -1;
+          for (int s2 = pnb + lane; s2 < nS; s2 += 64)
+            if (pPos[s2] == pb) found = s2;
+          unsigned long long bal = __ballot(found >= 0);
+          pslot = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
+          if (lane == 0 && !bal) { pPos[nS] = pb; pCont[nS] = nS; s_pnS = nS + 1; }
+        }
+        if (lane == 0) {
+          int t = lCont[j];
+          lCont[j] = lCont[slotb];
+          lCont[slotb] = t;
+          t = pCont[pa];
+          pCont[pa] = pCont[pslot];
+          pCont[pslot] = t;
+        }
+      }
+    }
+    // ---- 6. scale and rank-1 update, branch-free over the register row
+    const T pv = up[j];
+    if (pv != T(0)) {
+      const bool act = valid && i > j;
+      T cur = v[0];
+#pragma unroll
+      for (int c = 1; c < LW; ++c) cur = mask_sel(0u - (unsigned)(c == j), v[c], cur);
+      const T l = act ? cur / pv : T(0);
+      const unsigned am = 0u - (unsigned)act;
+#pragma unroll
+      for (int c = 0; c < LW; ++c) {
+        const T uc = (c > j) ? up[c] : T(0);
+        v[c] = mask_sel(am & (0u - (unsigned)(c == j)), l, v[c] - l * uc);
+      }
+    } else if (first_zero < 0) {
+      first_zero = j;
+    }
+    return true;
+  };
+#pragma unroll 1
+  for (int j = 0; j < w; ++j)
+    if (!column(j)) break;
+  if (s_abort) return;
+  __syncthreads();
+  if (valid) {
+    T* dst = P + (int64_t)i * ldp;
+#pragma unroll
+    for (int c = 0; c < LW; ++c)
+      if (c < w) dst[c] = v[c];
+  }
+  if (g == 0) {
+    const int ln = s_lnS, pn = s_pnS;
+    for (int s = tid; s < ln; s += BLOCK) { lpos[s] = lPos[s]; lsrc[s] = lCont[s]; }
+    for (int s = tid; s < pn; s += BLOCK) { ppos[s] = pPos[s]; psrc[s] = pCont[s]; }
+    if (tid == 0) {
+      *lperm_n = ln;
+      *pperm_n = pn;
+      *zero_col = first_zero;
+    }
+  }
+}
+
+namespace {
+__global__ void panel_perm_init_kernel(int* pn, int* ppos, int* psrc, int nb) {
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) { ppos[s] = s; psrc[s] = s; }
+  if (threadIdx.x == 0) *pn = nb;
+}
+}  // namespace
+
+int panel_perm_init(hipStream_t s, int* pn, int* ppos, int* psrc, int nb) {
+  hipLaunchKernelGGL(panel_perm_init_kernel, dim3(1), dim3(256), 0, s, pn, ppos, psrc, nb);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+// Launches one leaf over rows [0, m) of the sub-panel at P (row-major, ldp),
+// columns [0, w), w <= 32.
+template <typename T>
+int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n, int* lpos,
+            int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col) {
+  constexpr int BLOCK = LeafBlock<T>::v;
+  if (w > LW || pnb > 256) {
+    set_error("leaf width %d / panel width %d unsupported", w, pnb);
+    return kErrUnsupported;
+  }
+  const int G = ceil_div(m, BLOCK);
+  if (G > kMaxG) {
+    set_error("panel of %d rows exceeds the leaf exchange (%d workgroups of %d rows)", m, kMaxG,
+              BLOCK);
+    return kErrUnsupported;
+  }
+  const int SW = slot_words<T>();
+  size_t need = 256 + sizeof(u64) * 2 * (size_t)(G + 1) * SW;
+  void* base = nullptr;
+  bool fresh = h->ws.bytes[kWsPanelSync] < need;
+  EAMD_CHECK(h->ws.get(kWsPanelSync, need, &base));
+  if (fresh || h->panel_epoch > (1u << 31)) {
+    EAMD_HIP(hipMemsetAsync(base, 0, h->ws.bytes[kWsPanelSync], h->stream));
+    h->panel_epoch = 0;
+  }
+  unsigned epoch_base = h->panel_epoch;
+  h->panel_epoch += (unsigned)w + 1u;
+  char* b = static_cast<char*>(base);
+  hipLaunchKernelGGL((leaf_lu_kernel<T, BLOCK>), dim3(G), dim3(BLOCK), 0, h->stream, P, ldp, m, w,
+                     G, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb, zero_col,
+                     reinterpret_cast<u64*>(b + 256), reinterpret_cast<int*>(b), epoch_base);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+template int leaf_lu<double>(eamd_handle*, double*, int64_t, int, int, int*, int*, int*, int*,
+                             int*, int*, int*, int, int, int*);
+template int leaf_lu<float>(eamd_handle*, float*, int64_t, int, int, int*, int*, int*, int*,
+                            int*, int*, int*, int, int, int*);
+
+}  // namespace eamd

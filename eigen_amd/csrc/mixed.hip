@@ -377,8 +377,12 @@ int sgemm_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, fl
   hipStream_t s = h->stream;
   const int M = ncols, N = mrows;
   const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
-  if (precision == EAMD_PREC_FP32 || precision == EAMD_PREC_FP64) {
-    PhaseScope ps(h, kPhUpdate, 4.0 * (2.0 * M * N + (double)k * (M + N)), 2.0 * M * N * k);
+  if (precision == EAMD_PREC_FP32 || precision == EAMD_PREC_FP64 || precision < 0) {
+    // precision < 0: an update inside the panel (FP32, accounted to the panel)
+    const bool in_panel = precision < 0;
+    PhaseScope ps(h, in_panel ? kPhPanel : kPhUpdate,
+                  in_panel ? 0.0 : 4.0 * (2.0 * M * N + (double)k * (M + N)),
+                  in_panel ? 0.0 : 2.0 * M * N * k);
     hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, s, M, N, k, A12, A21, ld,
                        A22);
     EAMD_HIP(hipGetLastError());
