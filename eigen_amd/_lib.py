@@ -69,6 +69,8 @@ _SIGS = {
     "eamd_batched_plan_lanes": ([vp, C.c_int32], c_int),
     "eamd_spotrf_solve_batched_planned": ([vp, vp, vp, vp, vp, vp, vp], c_int),
     "eamd_dgesv_ir_batched": ([vp, c_int64, vp, vp, vp, vp, c_int64, vp, vp, vp, vp], c_int),
+    "eamd_batched_lu_plan_create": ([vp, c_int64, vp, c_int64, C.POINTER(vp)], c_int),
+    "eamd_dgesv_ir_batched_planned": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], c_int),
     "eamd_host_dgetrf": ([vp, c_int64, c_int64, vp, c_int64, vp, C.POINTER(C.c_int32),
                           C.POINTER(c_int64)], c_int),
     "eamd_host_dgetrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, vp, c_int64], c_int),

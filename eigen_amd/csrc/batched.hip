@@ -29,6 +29,7 @@
 namespace eamd {
 
 constexpr int kMaxBatchN = 32;
+constexpr int kMaxPlanN = 256;
 
 namespace {
 
@@ -152,29 +153,29 @@ __global__ __launch_bounds__(256) void llt_batch_kernel(int count, const int* __
 }
 
 // ---- plan kernels
-__global__ void size_histogram_kernel(int count, const int32_t* __restrict__ n,
-                                      int* __restrict__ hist, int* __restrict__ bad) {
-  __shared__ int h[kMaxBatchN + 2];
-  for (int i = threadIdx.x; i < kMaxBatchN + 2; i += blockDim.x) h[i] = 0;
+// hist[v] = systems of size v (1..max_n); hist[kMaxPlanN + 1] = sizes outside
+__global__ void size_histogram_kernel(int count, int max_n, const int32_t* __restrict__ n,
+                                      int* __restrict__ hist) {
+  __shared__ int h[kMaxPlanN + 2];
+  for (int i = threadIdx.x; i < kMaxPlanN + 2; i += blockDim.x) h[i] = 0;
   __syncthreads();
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (long)gridDim.x * blockDim.x) {
     int v = n[i];
-    if (v < 1 || v > kMaxBatchN) atomicAdd(&h[kMaxBatchN + 1], 1);
+    if (v < 1 || v > max_n) atomicAdd(&h[kMaxPlanN + 1], 1);
     else atomicAdd(&h[v], 1);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kMaxBatchN + 2; i += blockDim.x)
+  for (int i = threadIdx.x; i < kMaxPlanN + 2; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
-  (void)bad;
 }
 
-__global__ void size_scatter_kernel(int count, const int32_t* __restrict__ n,
+__global__ void size_scatter_kernel(int count, int max_n, const int32_t* __restrict__ n,
                                     int* __restrict__ cursor, int* __restrict__ order) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (long)gridDim.x * blockDim.x) {
     int v = n[i];
-    if (v >= 1 && v <= kMaxBatchN) order[atomicAdd(&cursor[v], 1)] = (int)i;
+    if (v >= 1 && v <= max_n) order[atomicAdd(&cursor[v], 1)] = (int)i;
   }
 }
 
@@ -294,15 +295,19 @@ int autotune_size(eamd_handle* h, int N) {
 
 namespace eamd {
 
-int batched_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
-                        eamd_batched_plan** out) {
+// Size sort shared by the batched solvers: one histogram of n[] (one host
+// synchronisation), descending-size buckets, a scatter of the system
+// indices into `order`.
+int batched_plan_sort(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
+                      eamd_batched_plan** out) {
   *out = nullptr;
   if (count > INT32_MAX) return kErrUnsupported;
+  if (max_n < 1 || max_n > kMaxPlanN) return kErrUnsupported;
   auto* p = new eamd_batched_plan();
   p->count = count;
   p->max_n = max_n;
   int* hist = nullptr;
-  hipError_t e = hipMalloc(&hist, sizeof(int) * 2 * (kMaxBatchN + 2));
+  hipError_t e = hipMalloc(&hist, sizeof(int) * 2 * (kMaxPlanN + 2));
   if (e != hipSuccess) {
     delete p;
     return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
@@ -313,43 +318,43 @@ int batched_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int max
     delete p;
     return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
   }
-  int hh[kMaxBatchN + 2] = {0};
+  int hh[kMaxPlanN + 2] = {0};
   int st = kOk;
   do {
-    if ((e = hipMemsetAsync(hist, 0, sizeof(int) * 2 * (kMaxBatchN + 2), h->stream))) break;
+    if ((e = hipMemsetAsync(hist, 0, sizeof(int) * 2 * (kMaxPlanN + 2), h->stream))) break;
     int grid = (int)std::min<int64_t>(1024, (count + 255) / 256);
     hipLaunchKernelGGL(size_histogram_kernel, dim3(std::max(1, grid)), dim3(256), 0, h->stream,
-                       (int)count, n, hist, nullptr);
+                       (int)count, max_n, n, hist);
     if ((e = hipMemcpyAsync(hh, hist, sizeof(hh), hipMemcpyDeviceToHost, h->stream))) break;
     if ((e = hipStreamSynchronize(h->stream))) break;
   } while (0);
   if (e != hipSuccess) {
     st = hip_fail(e, "batched plan", __FILE__, __LINE__);
-  } else if (hh[kMaxBatchN + 1] > 0) {
-    set_error("batched solve: %d systems have a size outside [1, %d]", hh[kMaxBatchN + 1], max_n);
+  } else if (hh[kMaxPlanN + 1] > 0) {
+    set_error("batched solve: %d systems have a size outside [1, %d]", hh[kMaxPlanN + 1], max_n);
     st = -3;
   }
   if (st == kOk) {
     int acc = 0;
     // descending size: the largest (slowest) systems are scheduled first
-    int cur[kMaxBatchN + 2] = {0};
-    for (int v = kMaxBatchN; v >= 1; --v) {
+    int cur[kMaxPlanN + 2] = {0};
+    for (int v = kMaxPlanN; v >= 1; --v) {
       p->bucket_start[v] = acc;
       p->bucket_count[v] = hh[v];
       cur[v] = acc;
       acc += hh[v];
     }
-    int* cursor = hist + (kMaxBatchN + 2);
+    p->bucket_start[0] = acc;
+    p->bucket_count[0] = 0;
+    int* cursor = hist + (kMaxPlanN + 2);
     e = hipMemcpyAsync(cursor, cur, sizeof(cur), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
       int grid = (int)std::min<int64_t>(1024, (count + 255) / 256);
       hipLaunchKernelGGL(size_scatter_kernel, dim3(std::max(1, grid)), dim3(256), 0, h->stream,
-                         (int)count, n, cursor, p->order);
+                         (int)count, max_n, n, cursor, p->order);
       e = hipStreamSynchronize(h->stream);
     }
     if (e != hipSuccess) st = hip_fail(e, "batched plan scatter", __FILE__, __LINE__);
-    for (int v = 1; v <= kMaxBatchN && st == kOk; ++v)
-      p->lanes[v] = p->bucket_count[v] ? autotune_size(h, v) : default_lanes(v);
   }
   (void)hipFree(hist);
   if (st != kOk) {
@@ -361,9 +366,20 @@ int batched_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int max
   return kOk;
 }
 
+int batched_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
+                        eamd_batched_plan** out) {
+  EAMD_CHECK(batched_plan_sort(h, count, n, max_n, out));
+  eamd_batched_plan* p = *out;
+  for (int v = 1; v <= kMaxBatchN; ++v)
+    p->lanes[v] = p->bucket_count[v] ? autotune_size(h, v) : default_lanes(v);
+  return kOk;
+}
+
 void batched_plan_destroy(eamd_batched_plan* p) {
   if (!p) return;
   if (p->order) (void)hipFree(p->order);
+  if (p->wbase) (void)hipFree(p->wbase);
+  if (p->bstart) (void)hipFree(p->bstart);
   delete p;
 }
 
@@ -394,7 +410,7 @@ int spotrf_solve_batched_impl(eamd_handle* h, int64_t count, const int64_t* offA
 }
 
 int batched_lanes(const eamd_batched_plan* p, int n) {
-  return (n >= 1 && n <= kMaxBatchN) ? p->lanes[n] : 0;
+  return (n >= 1 && n <= p->max_n && n <= kMaxPlanN) ? p->lanes[n] : 0;
 }
 
 }  // namespace eamd

@@ -1,11 +1,710 @@
-// batched_lu.hip -- placeholder until the batched LU kernels land.
+// batched_lu.hip -- many small general systems A_i X_i = B_i, each solved as
+// PartialPivLU<MatrixXd>(A_i).solve(B_i) (Eigen/src/LU/PartialPivLU.h:128,
+// 226-244, 426-496), computed as an FP32 LU on matrix cores with FP64
+// iterative refinement; systems the FP32 factor cannot refine to the
+// tolerance are refactored in FP64 by a second launch.
+//
+// MI355X mapping.  One workgroup per system (n <= 256), systems taken in
+// descending size from the plan's order so the largest start first.  The
+// workgroup converts its FP64 column-major A_i into a row-major working copy
+// W_i (row interchanges are then contiguous), and runs a right-looking
+// blocked LU with 16-column panels: the panel is factored in LDS (pivot
+// search by DPP wave reductions), the interchanges are applied to the rest
+// of the rows, U12 is solved against the unit lower L11, and the trailing
+// update A22 -= L21 U12 runs as 16x16x4 MFMA tiles (FP32, or FP64 for the
+// refactor pass).  The solves and the refinement steps (residual B - A X in
+// FP64 from the original A) stay in the same workgroup.  The workgroup size
+// (64, 128 or 256 threads) is chosen per 16-wide size bucket by timing on
+// synthetic systems, once per process.
+#include <algorithm>
+#include <mutex>
+
 #include "common.h"
 #include "kernels.h"
+
 namespace eamd {
+
+namespace {
+
+constexpr int kMaxLuN = 256;
+constexpr int PB = 16;       // panel width = the MFMA k-depth used per trailing update
+constexpr int kMaxRhs = 8;
+constexpr int DB = 64;       // diagonal block of the triangular solves (one wave)
+constexpr int kMaxIter = 10;
+constexpr double kTol = 1e-14;
+
+__device__ __forceinline__ v4f mfma16x4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4d mfma16x4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<float> {
+  typedef v4f type;
+};
+template <>
+struct Acc4<double> {
+  typedef v4d type;
+};
+// row of accumulator register r in lane l (column is l % 16)
+template <typename T>
+__device__ __forceinline__ int acc_row(int l, int r) {
+  return sizeof(T) == 4 ? 4 * (l >> 4) + r : (l >> 4) + 4 * r;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// wave reductions (inputs never NaN): quad xor-1, xor-2, half-row and row
+// mirrors, then the four 16-lane rows read as uniform values
+__device__ __forceinline__ double wave_max(double s) {
+  s = fmax(s, dpp_d<0xB1>(s));
+  s = fmax(s, dpp_d<0x4E>(s));
+  s = fmax(s, dpp_d<0x141>(s));
+  s = fmax(s, dpp_d<0x140>(s));
+  return fmax(fmax(readlane_d(s, 0), readlane_d(s, 16)), fmax(readlane_d(s, 32), readlane_d(s, 48)));
+}
+__device__ __forceinline__ double wave_sum(double s) {
+  s += dpp_d<0xB1>(s);
+  s += dpp_d<0x4E>(s);
+  s += dpp_d<0x141>(s);
+  s += dpp_d<0x140>(s);
+  return (readlane_d(s, 0) + readlane_d(s, 16)) + (readlane_d(s, 32) + readlane_d(s, 48));
+}
+__device__ __forceinline__ int wave_min_i(int x) {
+  x = min(x, __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_mov_dpp(x, 0x141, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_mov_dpp(x, 0x140, 0xf, 0xf, false));
+  return min(min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+             min(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+}
+
+// Eigen's pivot score: |a|; the strict '>' scan makes a NaN win only from
+// the first position (UnaryFunctors.h:63-66, Visitor.h:179)
+__device__ __forceinline__ double score_of(double v, bool first) {
+  double s = fabs(v);
+  if (s != s) s = first ? INFINITY : -1.0;
+  return s;
+}
+
+template <typename TF, int NMAX>
+struct LuSmem {
+  // factorization: the panel and U12
+  static constexpr int kPanel = NMAX * (PB + 1);
+  static constexpr int kU = PB * (NMAX + 1);
+  // solves: the diagonal triangle, the right-hand sides and the residual
+  static constexpr int kDiag = DB * (DB + 1);
+  static constexpr int kVec = NMAX * kMaxRhs;
+  static constexpr int kWords = (kPanel + kU) > (kDiag + 2 * kVec) ? (kPanel + kU)
+                                                                   : (kDiag + 2 * kVec);
+};
+
+template <int NW>
+__device__ __forceinline__ double block_max(double v, double* red, int lane, int wave) {
+  const double w = wave_max(v);
+  if (lane == 0) red[wave] = w;
+  __syncthreads();
+  double m = red[0];
+#pragma unroll
+  for (int q = 1; q < NW; ++q) m = fmax(m, red[q]);
+  __syncthreads();
+  return m;
+}
+
+// Forward (unit lower, FORWARD=true) or backward (upper) substitution of the
+// nrhs columns y[r * kMaxRhs + q] against the factors in the row-major W.
+// Rows are processed in blocks of DB: the off-block part as row-times-vector
+// products (a wave per row, lanes over the row: coalesced), the diagonal
+// triangle by wave 0 with lanes over rows, staged through LDS.
+template <typename TF, int NT, bool FORWARD>
+__device__ void tri_solve(const TF* __restrict__ W, int n, int nrhs, TF* y, TF* diag, int tid) {
+  constexpr int NW = NT / 64;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + DB - 1) / DB;
+  for (int bb = 0; bb < nblk; ++bb) {
+    const int blk = FORWARD ? bb : nblk - 1 - bb;
+    const int b0 = blk * DB, bn = min(DB, n - b0);
+    // off-block part: columns [0, b0) (forward) or [b0 + bn, n) (backward)
+    const int t0 = FORWARD ? 0 : b0 + bn, t1 = FORWARD ? b0 : n;
+    for (int rr = wave; rr < bn; rr += NW) {
+      const int r = b0 + rr;
+      double acc[kMaxRhs];
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
+      for (int t = t0 + lane; t < t1; t += 64) {
+        const double a = (double)W[(int64_t)r * n + t];
+#pragma unroll
+        for (int q = 0; q < kMaxRhs; ++q)
+          if (q < nrhs) acc[q] += a * (double)y[t * kMaxRhs + q];
+      }
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q)
+        if (q < nrhs) {
+          const double s = wave_sum(acc[q]);
+          if (lane == 0) y[r * kMaxRhs + q] -= (TF)s;
+        }
+    }
+    // stage the diagonal triangle: diag[r][t], r,t < bn
+    for (int e = tid; e < bn * bn; e += NT) {
+      const int r = e / bn, t = e - r * bn;
+      diag[r * (DB + 1) + t] = W[(int64_t)(b0 + r) * n + b0 + t];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      TF acc[kMaxRhs];
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q) acc[q] = (lane < bn && q < nrhs) ? y[(b0 + lane) * kMaxRhs + q] : TF(0);
+      if (FORWARD) {
+        for (int t = 0; t < bn; ++t) {
+          const TF l = (lane > t && lane < bn) ? diag[lane * (DB + 1) + t] : TF(0);
+#pragma unroll
+          for (int q = 0; q < kMaxRhs; ++q)
+            if (q < nrhs) {
+              const TF yt = sizeof(TF) == 8 ? (TF)readlane_d((double)acc[q], t)
+                                            : __int_as_float(__builtin_amdgcn_readlane(
+                                                  __float_as_int((float)acc[q]), t));
+              acc[q] -= l * yt;
+            }
+        }
+      } else {
+        for (int t = bn - 1; t >= 0; --t) {
+          const TF d = diag[t * (DB + 1) + t];
+          if (lane == t) {
+#pragma unroll
+            for (int q = 0; q < kMaxRhs; ++q) acc[q] = acc[q] / d;
+          }
+          const TF u = (lane < t) ? diag[lane * (DB + 1) + t] : TF(0);
+#pragma unroll
+          for (int q = 0; q < kMaxRhs; ++q)
+            if (q < nrhs) {
+              const TF yt = sizeof(TF) == 8 ? (TF)readlane_d((double)acc[q], t)
+                                            : __int_as_float(__builtin_amdgcn_readlane(
+                                                  __float_as_int((float)acc[q]), t));
+              acc[q] -= u * yt;
+            }
+        }
+      }
+      if (lane < bn) {
+#pragma unroll
+        for (int q = 0; q < kMaxRhs; ++q)
+          if (q < nrhs) y[(b0 + lane) * kMaxRhs + q] = acc[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per system.  Pass 1 (TF = float): systems list[0..count);
+// systems that do not reach the tolerance are appended to the refactor
+// queue.  Pass 2 (TF = double): the systems of the queue (its length read
+// on the device).  info: 0 refined from the FP32 factor, 1 after the FP64
+// refactor, 2 not converged.  iters: refinement steps of the final pass.
+//
+// Systems are addressed by their position in the plan's size-sorted order:
+// launch positions pos0 .. pos0+count-1, or the positions held in `list`
+// (length *list_count).  The working copy of the system at position pos
+// starts at wbase[n] + (pos - bstart[n]) n^2 (the plan lays the copies out
+// by size).
+template <typename TF, int NT, int NMAX>
+__global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
+    int count, int pos0, const int* __restrict__ list, const int* __restrict__ list_count,
+    const int* __restrict__ order, const int64_t* __restrict__ wbase,
+    const int* __restrict__ bstart, const int64_t* __restrict__ offA,
+    const int32_t* __restrict__ nsz, const double* __restrict__ A,
+    const int64_t* __restrict__ offB, int nrhs, const double* __restrict__ B,
+    double* __restrict__ X, void* __restrict__ Wbase, int32_t* __restrict__ info,
+    int32_t* __restrict__ iters, int* __restrict__ queue, int* __restrict__ queue_n) {
+  constexpr int NW = NT / 64;
+  typedef typename Acc4<TF>::type AccT;
+  __shared__ TF smem[LuSmem<TF, NMAX>::kWords];
+  __shared__ int piv[NMAX];
+  __shared__ int perm[NMAX + 1];
+  __shared__ double red_d[NW];
+  __shared__ int red_i[NW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int total = list_count ? *list_count : count;
+  constexpr bool kRefactor = sizeof(TF) == 8;
+
+  for (int qi = blockIdx.x; qi < total; qi += gridDim.x) {
+    const int pos = list ? list[qi] : pos0 + qi;
+    const int s = order[pos];
+    const int n = nsz[s];
+    const double* __restrict__ As = A + offA[s];
+    const double* __restrict__ Bs = B + offB[s];
+    double* __restrict__ Xs = X + offB[s];
+    TF* __restrict__ W =
+        static_cast<TF*>(Wbase) + wbase[n] + (int64_t)(pos - bstart[n]) * n * n;
+
+    // ---- 1. W = A in row-major, TF (32x32 tiles through LDS)
+    {
+      TF* tile = smem;  // 32 x 33
+      const int nt = (n + 31) / 32;
+      for (int tt = 0; tt < nt * nt; ++tt) {
+        const int tr = tt / nt, tc = tt - tr * nt;
+        for (int e = tid; e < 1024; e += NT) {
+          const int i = e & 31, j = e >> 5, r = tr * 32 + i, c = tc * 32 + j;
+          tile[j * 33 + i] = (r < n && c < n) ? (TF)As[r + (int64_t)c * n] : TF(0);
+        }
+        __syncthreads();
+        for (int e = tid; e < 1024; e += NT) {
+          const int j = e & 31, i = e >> 5, r = tr * 32 + i, c = tc * 32 + j;
+          if (r < n && c < n) W[(int64_t)r * n + c] = tile[j * 33 + i];
+        }
+        __syncthreads();
+      }
+    }
+    for (int r = tid; r <= n; r += NT) perm[r] = r;
+
+    // ---- 2. blocked LU with partial pivoting
+    TF* Pn = smem;                          // panel: [row][PB + 1]
+    TF* Us = smem + LuSmem<TF, NMAX>::kPanel;     // U12: [PB][NMAX + 1]
+    for (int k0 = 0; k0 < n; k0 += PB) {
+      const int pb = min(PB, n - k0), R = n - k0;
+      for (int e = tid; e < R * PB; e += NT) {
+        const int r = e / PB, c = e - r * PB;
+        Pn[r * (PB + 1) + c] = c < pb ? W[(int64_t)(k0 + r) * n + k0 + c] : TF(0);
+      }
+      __syncthreads();
+      for (int c = 0; c < pb; ++c) {
+        double sc = -1.0;
+        int ri = 0x7fffffff;
+        for (int r = c + tid; r < R; r += NT) {
+          const double v = score_of((double)Pn[r * (PB + 1) + c], r == c);
+          if (v > sc) { sc = v; ri = r; }
+        }
+        const double M = block_max<NW>(sc, red_d, lane, wave);
+        const int wmin = wave_min_i(sc == M ? ri : 0x7fffffff);
+        if (lane == 0) red_i[wave] = wmin;
+        __syncthreads();
+        int p = red_i[0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) p = min(p, red_i[q]);
+        const TF pv = Pn[p * (PB + 1) + c];
+        if (tid == 0) piv[k0 + c] = k0 + p;
+        __syncthreads();
+        if (p != c && tid < PB) {
+          const TF t = Pn[c * (PB + 1) + tid];
+          Pn[c * (PB + 1) + tid] = Pn[p * (PB + 1) + tid];
+          Pn[p * (PB + 1) + tid] = t;
+        }
+        __syncthreads();
+        if (pv != TF(0)) {
+          for (int r = c + 1 + tid; r < R; r += NT) {
+            const TF l = Pn[r * (PB + 1) + c] / pv;
+            Pn[r * (PB + 1) + c] = l;
+            for (int cc = c + 1; cc < pb; ++cc)
+              Pn[r * (PB + 1) + cc] -= l * Pn[c * (PB + 1) + cc];
+          }
+        }
+        __syncthreads();
+      }
+      // panel back to W
+      for (int e = tid; e < R * pb; e += NT) {
+        const int r = e / pb, c = e - r * pb;
+        W[(int64_t)(k0 + r) * n + k0 + c] = Pn[r * (PB + 1) + c];
+      }
+      // the panel's interchanges on the other columns; column n is the
+      // permutation vector
+      for (int col = tid; col <= n; col += NT) {
+        if (col >= k0 && col < k0 + pb) continue;
+        for (int c = 0; c < pb; ++c) {
+          const int a = k0 + c, b = piv[a];
+          if (b == a) continue;
+          if (col == n) {
+            const int t = perm[a];
+            perm[a] = perm[b];
+            perm[b] = t;
+          } else {
+            const TF t = W[(int64_t)a * n + col];
+            W[(int64_t)a * n + col] = W[(int64_t)b * n + col];
+            W[(int64_t)b * n + col] = t;
+          }
+        }
+      }
+      __syncthreads();
+      const int T0 = k0 + pb, nr = n - T0;
+      if (nr <= 0) continue;
+      // U12 = L11^-1 A12
+      for (int col = T0 + tid; col < n; col += NT) {
+        TF u[PB];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) u[t] = W[(int64_t)(k0 + t) * n + col];
+#pragma unroll
+        for (int t = 1; t < PB; ++t)
+#pragma unroll
+          for (int q = 0; q < t; ++q) u[t] -= Pn[t * (PB + 1) + q] * u[q];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) {
+          W[(int64_t)(k0 + t) * n + col] = u[t];
+          Us[t * (NMAX + 1) + col - T0] = u[t];
+        }
+      }
+      __syncthreads();
+      // A22 -= L21 U12 on 16x16 tiles, one wave per tile
+      const int l16 = lane & 15, lk = lane >> 4;
+      const int tn = (nr + 15) / 16;
+      for (int tile = wave; tile < tn * tn; tile += NW) {
+        const int ti = tile / tn, tj = tile - ti * tn;
+        const int r0 = T0 + ti * 16, c0 = T0 + tj * 16;
+        const int col = c0 + l16;
+        AccT acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + acc_row<TF>(lane, r);
+          acc[r] = (row < n && col < n) ? W[(int64_t)row * n + col] : TF(0);
+        }
+        const int arow = r0 + l16;
+#pragma unroll
+        for (int kk = 0; kk < PB / 4; ++kk) {
+          const int k = kk * 4 + lk;
+          const TF a = arow < n ? -Pn[(arow - k0) * (PB + 1) + k] : TF(0);
+          const TF b = col < n ? Us[k * (NMAX + 1) + col - T0] : TF(0);
+          acc = mfma16x4(a, b, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + acc_row<TF>(lane, r);
+          if (row < n && col < n) W[(int64_t)row * n + col] = acc[r];
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- 3. solve and refine
+    TF* diag = smem;
+    TF* y = smem + LuSmem<TF, NMAX>::kDiag;
+    TF* rv = y + LuSmem<TF, NMAX>::kVec;
+    for (int e = tid; e < n * nrhs; e += NT) {
+      const int r = e / nrhs, q = e - r * nrhs;
+      y[r * kMaxRhs + q] = (TF)Bs[perm[r] + (int64_t)q * n];
+    }
+    __syncthreads();
+    tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
+    tri_solve<TF, NT, false>(W, n, nrhs, y, diag, tid);
+    for (int e = tid; e < n * nrhs; e += NT) {
+      const int q = e / n, r = e - q * n;
+      Xs[r + (int64_t)q * n] = (double)y[r * kMaxRhs + q];
+    }
+    __syncthreads();
+
+    // ||A||_inf and ||B_q||_inf
+    double arow = 0.0;
+    for (int r = tid; r < n; r += NT) {
+      double sacc = 0.0;
+      for (int t = 0; t < n; ++t) sacc += fabs(As[r + (int64_t)t * n]);
+      arow = fmax(arow, sacc);
+    }
+    const double anorm = block_max<NW>(arow, red_d, lane, wave);
+    double bnorm[kMaxRhs];
+#pragma unroll
+    for (int q = 0; q < kMaxRhs; ++q) {
+      double v = 0.0;
+      if (q < nrhs)
+        for (int r = tid; r < n; r += NT) v = fmax(v, fabs(Bs[r + (int64_t)q * n]));
+      bnorm[q] = q < nrhs ? block_max<NW>(v, red_d, lane, wave) : 0.0;
+    }
+
+    int it = 0, converged = 0;
+    double prev = INFINITY;
+    for (;; ++it) {
+      // R = B - A X in FP64 from the original A
+      double rmax[kMaxRhs], xmax[kMaxRhs];
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q) { rmax[q] = 0.0; xmax[q] = 0.0; }
+      for (int r = tid; r < n; r += NT) {
+        double acc[kMaxRhs];
+#pragma unroll
+        for (int q = 0; q < kMaxRhs; ++q) acc[q] = q < nrhs ? Bs[r + (int64_t)q * n] : 0.0;
+        for (int t = 0; t < n; ++t) {
+          const double a = As[r + (int64_t)t * n];
+#pragma unroll
+          for (int q = 0; q < kMaxRhs; ++q)
+            if (q < nrhs) acc[q] -= a * Xs[t + (int64_t)q * n];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxRhs; ++q)
+          if (q < nrhs) {
+            rv[r * kMaxRhs + q] = (TF)acc[q];
+            rmax[q] = fmax(rmax[q], fabs(acc[q]));
+            xmax[q] = fmax(xmax[q], fabs(Xs[r + (int64_t)q * n]));
+          }
+      }
+      double worst = 0.0;
+      bool bad = false;
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q)
+        if (q < nrhs) {
+          const double rn = block_max<NW>(rmax[q], red_d, lane, wave);
+          const double xn = block_max<NW>(xmax[q], red_d, lane, wave);
+          const double den = anorm * xn + bnorm[q];
+          const double be = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
+          if (!(be <= kTol)) bad = true;
+          worst = (be == be) ? fmax(worst, be) : INFINITY;
+        }
+      if (!bad) { converged = 1; break; }
+      // stop on the iteration limit, or when a step gains less than 2x
+      if (it >= kMaxIter || !(worst < 0.5 * prev)) break;
+      prev = worst;
+      // correction: dX = U^-1 L^-1 P R, X += dX
+      for (int e = tid; e < n * nrhs; e += NT) {
+        const int r = e / nrhs, q = e - r * nrhs;
+        y[r * kMaxRhs + q] = rv[perm[r] * kMaxRhs + q];
+      }
+      __syncthreads();
+      tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
+      tri_solve<TF, NT, false>(W, n, nrhs, y, diag, tid);
+      for (int e = tid; e < n * nrhs; e += NT) {
+        const int q = e / n, r = e - q * n;
+        Xs[r + (int64_t)q * n] += (double)y[r * kMaxRhs + q];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      if (converged) {
+        info[s] = kRefactor ? 1 : 0;
+        if (iters) iters[s] = it;
+      } else if (kRefactor || !queue) {
+        info[s] = 2;
+        if (iters) iters[s] = it;
+      } else {
+        queue[atomicAdd(queue_n, 1)] = pos;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void zero_int_kernel(int* p) { *p = 0; }
+
+typedef void (*LuKernel)(int, int, const int*, const int*, const int*, const int64_t*,
+                         const int*, const int64_t*, const int32_t*, const double*,
+                         const int64_t*, int, const double*, double*, void*, int32_t*, int32_t*,
+                         int*, int*);
+
+// LDS is sized for the largest system of a launch: 64, 128 or 256
+int nmax_for(int n) { return n <= 64 ? 64 : (n <= 128 ? 128 : 256); }
+
+template <int NMAX>
+LuKernel lu_kernel_nt(int nt) {
+  if (nt == 64) return lu_ir_batch_kernel<float, 64, NMAX>;
+  if (nt == 128) return lu_ir_batch_kernel<float, 128, NMAX>;
+  return lu_ir_batch_kernel<float, 256, NMAX>;
+}
+
+LuKernel lu_kernel_for(int nt, int nmax) {
+  if (nmax == 64) return lu_kernel_nt<64>(nt);
+  if (nmax == 128) return lu_kernel_nt<128>(nt);
+  return lu_kernel_nt<256>(nt);
+}
+
+int lu_default_threads(int n) { return n <= 48 ? 64 : (n <= 128 ? 128 : 256); }
+
+int lu_grid(eamd_handle* h, int cnt) {
+  return std::max(1, std::min(cnt, h->num_cus * 8));
+}
+
+// synthetic systems for the autotuner: A = n I + a small non-symmetric
+// pattern (well conditioned: refinement converges from the FP32 factor)
+__global__ void lu_tune_fill_kernel(int N, int count, int nrhs, double* A, int64_t* offA,
+                                    double* B, int64_t* offB, int* order, int32_t* nn) {
+  const long i = (long)blockIdx.x;
+  if (i >= count) return;
+  if (threadIdx.x == 0) {
+    offA[i] = i * N * N;
+    offB[i] = i * N * nrhs;
+    order[i] = (int)i;
+    nn[i] = N;
+  }
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    const int r = e % N, c = e / N;
+    A[i * N * N + e] = (r == c) ? (double)N : 0.5 / (1.0 + (double)((3 * r + c) % 11));
+  }
+  for (int e = threadIdx.x; e < N * nrhs; e += blockDim.x) B[i * N * nrhs + e] = 1.0 + 0.01 * e;
+}
+
+std::mutex g_lu_tune_mu;
+int g_lu_tune[kMaxRhs + 1][kMaxLuN / 16 + 1] = {{0}};
+
+int lu_autotune(eamd_handle* h, int n, int nrhs) {
+  const int bkt = (n - 1) / 16;
+  {
+    std::lock_guard<std::mutex> lk(g_lu_tune_mu);
+    if (g_lu_tune[nrhs][bkt]) return g_lu_tune[nrhs][bkt];
+  }
+  int best = lu_default_threads(n);
+  const char* env = getenv("EAMD_BATCH_AUTOTUNE");
+  if (!(env && env[0] == '0')) {
+    const int N = std::min(kMaxLuN, bkt * 16 + 8);
+    const int cnt = std::max(256, std::min(4096, (1 << 24) / (N * N)));
+    const size_t nA = (size_t)cnt * N * N, nB = (size_t)cnt * N * nrhs;
+    size_t need = sizeof(double) * (2 * nA + 2 * nB) + sizeof(int64_t) * (2 * cnt + 512) +
+                  sizeof(int) * (4 * (size_t)cnt + 512) + 1024;
+    void* p = nullptr;
+    if (h->ws.get(kWsAux, need, &p) == kOk) {
+      char* c = static_cast<char*>(p);
+      double* Ad = reinterpret_cast<double*>(c);
+      double* Wd = Ad + nA;
+      double* Bd = Wd + nA;
+      double* Xd = Bd + nB;
+      int64_t* oA = reinterpret_cast<int64_t*>(Xd + nB);
+      int64_t* oB = oA + cnt;
+      // size tables of zeros: every system has size N, so the working copies
+      // follow the position
+      int64_t* wb = oB + cnt;
+      int* bs = reinterpret_cast<int*>(wb + 512);
+      int* order = bs + 512;
+      int32_t* nn = order + cnt;
+      int32_t* inf = nn + cnt;
+      int32_t* its = inf + cnt;
+      (void)hipMemsetAsync(wb, 0, sizeof(int64_t) * 512, h->stream);
+      (void)hipMemsetAsync(bs, 0, sizeof(int) * 512, h->stream);
+      hipLaunchKernelGGL(lu_tune_fill_kernel, dim3(cnt), dim3(256), 0, h->stream, N, cnt, nrhs,
+                         Ad, oA, Bd, oB, order, nn);
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      float best_ms = 1e30f;
+      for (int nt : {64, 128, 256}) {
+        LuKernel k = lu_kernel_for(nt, nmax_for(N));
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          (void)hipEventRecord(e0, h->stream);
+          hipLaunchKernelGGL(k, dim3(lu_grid(h, cnt)), dim3(nt), 0, h->stream, cnt, 0, nullptr,
+                             nullptr, order, wb, bs, oA, nn, Ad, oB, nrhs, Bd, Xd, Wd, inf, its,
+                             nullptr, nullptr);
+          (void)hipEventRecord(e1, h->stream);
+          (void)hipEventSynchronize(e1);
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (rep > 0) ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < best_ms) {
+          best_ms = ms_min;
+          best = nt;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (hipGetLastError() != hipSuccess) best = lu_default_threads(n);
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_lu_tune_mu);
+  g_lu_tune[nrhs][bkt] = best;
+  return best;
+}
+
+}  // namespace
+
+int batched_lu_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int nrhs,
+                           eamd_batched_plan** out) {
+  *out = nullptr;
+  if (nrhs < 1 || nrhs > kMaxRhs) {
+    set_error("batched LU: nrhs %d outside [1, %d]", nrhs, kMaxRhs);
+    return -6;
+  }
+  eamd_batched_plan* p = nullptr;
+  EAMD_CHECK(batched_plan_sort(h, count, n, kMaxLuN, &p));
+  p->kind = 1;
+  p->nrhs = nrhs;
+  // working copies laid out by size, largest first (the order of positions)
+  int64_t wb[kMaxLuN + 1] = {0};
+  int bs[kMaxLuN + 1] = {0};
+  int64_t acc = 0;
+  double flops = 0.0, bytes = 0.0;
+  for (int v = kMaxLuN; v >= 1; --v) {
+    wb[v] = acc;
+    bs[v] = p->bucket_start[v];
+    const double c = p->bucket_count[v], dv = v;
+    acc += (int64_t)p->bucket_count[v] * v * v;
+    // LU (2/3 n^3) plus the two triangular solves per right-hand side
+    flops += c * (2.0 / 3.0 * dv * dv * dv + 2.0 * dv * dv * nrhs);
+    // A read, B read, X written, offsets, size and info
+    bytes += c * (8.0 * dv * dv + 16.0 * dv * nrhs + 16.0 + 8.0);
+    p->lanes[v] = p->bucket_count[v] ? lu_autotune(h, v, nrhs) : 0;
+  }
+  p->total_elems = acc;
+  p->lu_flops = flops;
+  p->lu_bytes = bytes;
+  hipError_t e = hipMalloc(&p->wbase, sizeof(wb));
+  if (e == hipSuccess) e = hipMalloc(&p->bstart, sizeof(bs));
+  if (e == hipSuccess) e = hipMemcpy(p->wbase, wb, sizeof(wb), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->bstart, bs, sizeof(bs), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    batched_plan_destroy(p);
+    return hip_fail(e, "batched LU plan", __FILE__, __LINE__);
+  }
+  *out = p;
+  return kOk;
+}
+
+int dgesv_ir_batched_exec(eamd_handle* h, const eamd_batched_plan* p, const int64_t* offA,
+                          const int32_t* n, const double* A, const int64_t* offB,
+                          const double* B, double* X, int32_t* info, int32_t* iters) {
+  if (p->kind != 1) {
+    set_error("batched LU: the plan was made for another solver");
+    return kErrUnsupported;
+  }
+  const int64_t count = p->count;
+  if (count == 0) return kOk;
+  // workspace: the working copies (FP64-sized, so the refactor pass fits
+  // at the same offsets), the refactor queue and its length
+  const int64_t elems = std::max<int64_t>(1, p->total_elems);
+  size_t need = sizeof(double) * (size_t)elems + sizeof(int) * (size_t)(count + 64);
+  void* base = nullptr;
+  EAMD_CHECK(h->ws.get(kWsBatch, need, &base));
+  double* W = static_cast<double*>(base);
+  int* queue_n = reinterpret_cast<int*>(W + elems);
+  int* queue = queue_n + 64;
+  const double flops = p->lu_flops;
+  PhaseScope ps(h, kPhBatched, p->lu_bytes, flops);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, h->stream, queue_n);
+  // consecutive sizes (descending) that share a workgroup size form one launch
+  int v = kMaxLuN;
+  while (v >= 1) {
+    if (!p->bucket_count[v]) { --v; continue; }
+    const int nt = p->lanes[v], nmax = nmax_for(v);
+    const int start = p->bucket_start[v];
+    int cnt = 0, u = v;
+    while (u >= 1 && (!p->bucket_count[u] || (p->lanes[u] == nt && nmax_for(u) == nmax))) {
+      cnt += p->bucket_count[u];
+      --u;
+    }
+    hipLaunchKernelGGL(lu_kernel_for(nt, nmax), dim3(lu_grid(h, cnt)), dim3(nt), 0, h->stream, cnt,
+                       start, nullptr, nullptr, p->order, p->wbase, p->bstart, offA, n, A, offB,
+                       p->nrhs, B, X, W, info, iters, queue, queue_n);
+    v = u;
+  }
+  // FP64 refactor of the systems the FP32 factor could not refine
+  hipLaunchKernelGGL((lu_ir_batch_kernel<double, 256, kMaxLuN>), dim3(h->num_cus), dim3(256), 0,
+                     h->stream, 0, 0, queue, queue_n, p->order, p->wbase, p->bstart, offA, n, A,
+                     offB, p->nrhs, B, X, W, info, iters, nullptr, nullptr);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
 int dgesv_ir_batched_impl(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n,
                           const double* A, const int64_t* offB, int64_t nrhs, const double* B,
                           double* X, int32_t* info, int32_t* iters) {
-  set_error("batched LU not built yet");
-  return kErrUnsupported;
+  eamd_batched_plan* p = nullptr;
+  EAMD_CHECK(batched_lu_plan_create(h, count, n, (int)nrhs, &p));
+  int st = dgesv_ir_batched_exec(h, p, offA, n, A, offB, B, X, info, iters);
+  (void)hipStreamSynchronize(h->stream);
+  batched_plan_destroy(p);
+  return st;
 }
+
 }  // namespace eamd

@@ -137,6 +137,13 @@ int dgesv_ir_batched_impl(eamd_handle* h, int64_t count, const int64_t* offA, co
                           double* X, int32_t* info, int32_t* iters);
 int batched_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
                         eamd_batched_plan** out);
+int batched_plan_sort(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
+                      eamd_batched_plan** out);
+int batched_lu_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int nrhs,
+                           eamd_batched_plan** out);
+int dgesv_ir_batched_exec(eamd_handle* h, const eamd_batched_plan* p, const int64_t* offA,
+                          const int32_t* n, const double* A, const int64_t* offB,
+                          const double* B, double* X, int32_t* info, int32_t* iters);
 void batched_plan_destroy(eamd_batched_plan* p);
 int batched_lanes(const eamd_batched_plan* p, int n);
 int spotrf_solve_batched_exec(eamd_handle* h, const eamd_batched_plan* p, const int64_t* offA,
@@ -605,6 +612,34 @@ int eamd_dgesv_ir_batched(eamd_handle_t h, int64_t count, const int64_t* offA, c
   if (!X) return -9;
   if (!info) return -10;
   return dgesv_ir_batched_impl(h, count, offA, n, A, offB, nrhs, B, X, info, iters);
+}
+
+int eamd_batched_lu_plan_create(eamd_handle_t h, int64_t count, const int32_t* n, int64_t nrhs,
+                                eamd_batched_plan_t* plan) {
+  if (!plan) return -5;
+  *plan = nullptr;
+  if (count < 0) return -2;
+  if (count > 0 && !n) return -3;
+  if (nrhs < 1 || nrhs > 8) return -4;
+  CHECK_HANDLE(h);
+  return batched_lu_plan_create(h, count, n, (int)nrhs, plan);
+}
+
+int eamd_dgesv_ir_batched_planned(eamd_handle_t h, eamd_batched_plan_t plan,
+                                  const int64_t* offA, const int32_t* n, const double* A,
+                                  const int64_t* offB, const double* B, double* X,
+                                  int32_t* info, int32_t* iters) {
+  if (!plan) return -2;
+  CHECK_HANDLE(h);
+  if (plan->count == 0) return kOk;
+  if (!offA) return -3;
+  if (!n) return -4;
+  if (!A) return -5;
+  if (!offB) return -6;
+  if (!B) return -7;
+  if (!X) return -8;
+  if (!info) return -9;
+  return dgesv_ir_batched_exec(h, plan, offA, n, A, offB, B, X, info, iters);
 }
 
 // ------------------------------------------------------------ device memory

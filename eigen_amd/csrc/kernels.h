@@ -10,9 +10,17 @@ struct eamd_batched_plan {
   int64_t count = 0;
   int max_n = 0;
   int* order = nullptr;     // device: system indices sorted by size (descending)
-  int bucket_start[258];    // host: first position of each size in `order`
-  int bucket_count[258];    // host: systems of each size
-  int lanes[258];           // host: lanes (or threads) per system chosen per size
+  int bucket_start[258] = {0};  // host: first position of each size in `order`
+  int bucket_count[258] = {0};  // host: systems of each size
+  int lanes[258] = {0};         // host: lanes (or threads) per system chosen per size
+  // batched LU only (kind 1)
+  int kind = 0;             // 0 batched LLT, 1 batched LU
+  int nrhs = 0;
+  int64_t total_elems = 0;  // sum of n_i^2: the working copies, laid out by size
+  int64_t* wbase = nullptr; // device [257]: first working-copy element of each size
+  int* bstart = nullptr;    // device [257]: bucket_start on the device
+  double lu_flops = 0.0;    // algorithmic flops of one execution
+  double lu_bytes = 0.0;    // algorithmic HBM bytes of one execution
 };
 
 namespace eamd {
@@ -72,5 +80,12 @@ int apply_transpositions(eamd_handle* h, int n, const int* t, bool forward, T* X
 // Lower-triangular in-place factorization of a column-major n x n matrix;
 // *fail_col = failing column or -1.
 int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_col, int nb);
+
+// ---- batched plans (batched.hip) ----
+// Size sort of n[] (sizes 1..max_n <= 256) into a plan; the solvers then
+// fill in their per-size choices.
+int batched_plan_sort(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
+                      eamd_batched_plan** out);
+void batched_plan_destroy(eamd_batched_plan* p);
 
 }  // namespace eamd

@@ -82,6 +82,85 @@ __device__ __forceinline__ double score_of(double v, bool first) {
   return s;
 }
 
+// Register-row access by a uniform runtime column: a switch over the 32
+// compile-time indices (scalar branches), never a runtime index into the
+// register array.
+#define EAMD_CASES(X)                                                                        \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+
+template <typename T>
+__device__ __forceinline__ T pick(const T (&v)[32], int j) {
+  T r = v[0];
+  switch (j) {
+#define EAMD_PICK(c) \
+  case c:            \
+    r = v[c];        \
+    break;
+    EAMD_CASES(EAMD_PICK)
+#undef EAMD_PICK
+    default:
+      break;
+  }
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void place(T (&v)[32], int j, T x) {
+  switch (j) {
+#define EAMD_PLACE(c) \
+  case c:             \
+    v[c] = x;         \
+    break;
+    EAMD_CASES(EAMD_PLACE)
+#undef EAMD_PLACE
+    default:
+      break;
+  }
+}
+
+// v[c] -= l * u[c] for c = j+1 .. 31: enter at case j and fall through.
+template <typename T>
+__device__ __forceinline__ void axpy_tail(T (&v)[32], const T (&u)[32], int j, T l) {
+  switch (j) {
+#define EAMD_AXPY(c)                         \
+  case c:                                    \
+    if (c + 1 < 32) v[(c + 1) & 31] -= l * u[(c + 1) & 31]; \
+    [[fallthrough]];
+    EAMD_CASES(EAMD_AXPY)
+#undef EAMD_AXPY
+    default:
+      break;
+  }
+}
+
+// Lane permutations of the DPP unit, applied to a double as two halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((u64)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((u64)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Maximum over the 64 lanes of a wave (inputs never NaN): xor-1 and xor-2
+// within quads, half-row and row mirrors within each 16-lane row, then the
+// four row results read as uniform values.
+__device__ __forceinline__ double wave_max(double s) {
+  s = fmax(s, dpp_d<0xB1>(s));   // quad_perm [1,0,3,2]
+  s = fmax(s, dpp_d<0x4E>(s));   // quad_perm [2,3,0,1]
+  s = fmax(s, dpp_d<0x141>(s));  // row_half_mirror
+  s = fmax(s, dpp_d<0x140>(s));  // row_mirror
+  return fmax(fmax(readlane_d(s, 0), readlane_d(s, 16)), fmax(readlane_d(s, 32), readlane_d(s, 48)));
+}
+
 // Slot (words): [0..1] candidate value (FP64 bits), [2] row or ~0u (no
 // candidate), [3] unused, [4, 4 + LW*Words) candidate row.  The slot after
 // the last workgroup's holds row j (from word 4).
@@ -122,6 +201,7 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   // bookkeeping (workgroup 0): leaf-level and panel-level slot tables
   __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
   __shared__ int pPos[512], pCont[512], s_pnS;
+  __shared__ int s_piv[LW];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = blockIdx.x;
@@ -152,32 +232,29 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   auto column = [&](int j) __attribute__((always_inline)) -> bool {
     const int par = j & 1;
     const unsigned epoch = epoch_base + (unsigned)j + 1u;
-    T vj = v[0];
-#pragma unroll
-    for (int c = 1; c < LW; ++c) vj = mask_sel(0u - (unsigned)(c == j), v[c], vj);
-    // ---- 1. my candidate, then the workgroup's
-    double best = -2.0, bval = 0.0;
-    int bi = 0x7fffffff;
-    if (valid && i >= j) {
-      best = score_of((double)vj, i == j);
-      bi = i;
-      bval = (double)vj;
+    const T vj = pick(v, j);
+    // ---- 1. my candidate, the wave's (maximum score, then the lowest lane
+    // holding it: lanes are in row order), then the workgroup's
+    double sc = -2.0;
+    if (valid && i >= j) sc = score_of((double)vj, i == j);
+    const double wmax = wave_max(sc);
+    const int wl = __ffsll((long long)__ballot(sc == wmax)) - 1;
+    const double wval = readlane_d((double)vj, wl);
+    if (lane == 0) {
+      red_s[wave] = wmax;
+      red_i[wave] = wmax > -2.0 ? g * BLOCK + wave * 64 + wl : 0x7fffffff;
+      red_v[wave] = wval;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      double os = __shfl_xor(best, off);
-      int oi = __shfl_xor(bi, off);
-      double ov = __shfl_xor(bval, off);
-      if (beats(os, oi, best, bi)) { best = os; bi = oi; bval = ov; }
-    }
-    if (lane == 0) { red_s[wave] = best; red_i[wave] = bi; red_v[wave] = bval; }
     __syncthreads();
-    best = red_s[0];
-    bi = red_i[0];
-    bval = red_v[0];
+    double best = red_s[0];
+    int bq = 0;
 #pragma unroll
-    for (int q = 1; q < NWAVE; ++q)
-      if (beats(red_s[q], red_i[q], best, bi)) { best = red_s[q]; bi = red_i[q]; bval = red_v[q]; }
+    for (int q = 1; q < NWAVE; ++q) {
+      const double s2 = red_s[q];
+      if (s2 > best) { best = s2; bq = q; }
+    }
+    const int bi = red_i[bq];
+    const double bval = red_v[bq];
     // ---- 2. candidate row and row j into LDS (the rows stay in registers)
     const bool none = bi == 0x7fffffff;
     if (i == bi) {
@@ -298,56 +375,17 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
         for (int c = 0; c < LW; ++c) v[c] = rp[c];
       }
     }
-    // ---- bookkeeping (workgroup 0, wave 0)
-    if (g == 0 && wave == 0) {
-      if (lane == 0) piv[j] = p;
-      if (p != j) {
-        // leaf-level table: rows 0..w-1 are slots 0..w-1
-        int slotb = p;
-        if (p >= w) {
-          const int nS = s_lnS;
-          int found = -1;
-          for (int s2 = w + lane; s2 < nS; s2 += 64)
-            if (lPos[s2] == p) found = s2;
-          unsigned long long bal = __ballot(found >= 0);
-          slotb = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
-          if (lane == 0 && !bal) { lPos[nS] = p; lCont[nS] = nS; s_lnS = nS + 1; }
-        }
-        // panel-level table: rows 0..pnb-1 are slots 0..pnb-1
-        const int pa = row0 + j, pb = row0 + p;
-        int pslot = pb;
-        if (pb >= pnb) {
-          const int nS = s_pnS;
-          int found = -1;
-          for (int s2 = pnb + lane; s2 < nS; s2 += 64)
-            if (pPos[s2] == pb) found = s2;
-          unsigned long long bal = __ballot(found >= 0);
-          pslot = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
-          if (lane == 0 && !bal) { pPos[nS] = pb; pCont[nS] = nS; s_pnS = nS + 1; }
-        }
-        if (lane == 0) {
-          int t = lCont[j];
-          lCont[j] = lCont[slotb];
-          lCont[slotb] = t;
-          t = pCont[pa];
-          pCont[pa] = pCont[pslot];
-          pCont[pslot] = t;
-        }
-      }
-    }
-    // ---- 6. scale and rank-1 update, branch-free over the register row
+    if (g == 0 && tid == 0) s_piv[j] = p;
+    // ---- 6. scale and rank-1 update of the rows below j
     const T pv = up[j];
     if (pv != T(0)) {
-      const bool act = valid && i > j;
-      T cur = v[0];
+      if (valid && i > j) {
+        T u[LW];
 #pragma unroll
-      for (int c = 1; c < LW; ++c) cur = mask_sel(0u - (unsigned)(c == j), v[c], cur);
-      const T l = act ? cur / pv : T(0);
-      const unsigned am = 0u - (unsigned)act;
-#pragma unroll
-      for (int c = 0; c < LW; ++c) {
-        const T uc = (c > j) ? up[c] : T(0);
-        v[c] = mask_sel(am & (0u - (unsigned)(c == j)), l, v[c] - l * uc);
+        for (int c = 0; c < LW; ++c) u[c] = up[c];
+        const T l = (i == p ? rp[j] : vj) / pv;
+        place(v, j, l);
+        axpy_tail(v, u, j, l);
       }
     } else if (first_zero < 0) {
       first_zero = j;
@@ -358,6 +396,47 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   for (int j = 0; j < w; ++j)
     if (!column(j)) break;
   if (s_abort) return;
+  __syncthreads();
+  // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
+  // permutation tables, replaying the interchanges in column order
+  if (g == 0 && wave == 0) {
+    for (int j = 0; j < w; ++j) {
+      const int p = s_piv[j];
+      if (lane == 0) piv[j] = p;
+      if (p == j) continue;
+      // leaf-level table: rows 0..w-1 are slots 0..w-1
+      int slotb = p;
+      if (p >= w) {
+        const int nS = s_lnS;
+        int found = -1;
+        for (int s2 = w + lane; s2 < nS; s2 += 64)
+          if (lPos[s2] == p) found = s2;
+        unsigned long long bal = __ballot(found >= 0);
+        slotb = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
+        if (lane == 0 && !bal) { lPos[nS] = p; lCont[nS] = nS; s_lnS = nS + 1; }
+      }
+      // panel-level table: rows 0..pnb-1 are slots 0..pnb-1
+      const int pa = row0 + j, pb = row0 + p;
+      int pslot = pb;
+      if (pb >= pnb) {
+        const int nS = s_pnS;
+        int found = -1;
+        for (int s2 = pnb + lane; s2 < nS; s2 += 64)
+          if (pPos[s2] == pb) found = s2;
+        unsigned long long bal = __ballot(found >= 0);
+        pslot = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
+        if (lane == 0 && !bal) { pPos[nS] = pb; pCont[nS] = nS; s_pnS = nS + 1; }
+      }
+      if (lane == 0) {
+        int t = lCont[j];
+        lCont[j] = lCont[slotb];
+        lCont[slotb] = t;
+        t = pCont[pa];
+        pCont[pa] = pCont[pslot];
+        pCont[pslot] = t;
+      }
+    }
+  }
   __syncthreads();
   if (valid) {
     T* dst = P + (int64_t)i * ldp;

@@ -199,20 +199,36 @@ typedef struct eamd_batched_plan* eamd_batched_plan_t;
 int eamd_batched_plan_create(eamd_handle_t h, int64_t count, const int32_t* n, int32_t max_n,
                              eamd_batched_plan_t* plan);
 int eamd_batched_plan_destroy(eamd_batched_plan_t plan);
-/* lanes of a wavefront per system chosen for size n (0: n not in the plan) */
+/* lanes of a wavefront per system (batched LLT plan) or threads per
+ * workgroup (batched LU plan) chosen for size n (0: n not in the plan) */
 int eamd_batched_plan_lanes(eamd_batched_plan_t plan, int32_t n);
 int eamd_spotrf_solve_batched_planned(eamd_handle_t h, eamd_batched_plan_t plan,
                                       const int64_t* offA, const float* A, const int64_t* offb,
                                       float* b, int32_t* info);
 
 /* Per system PartialPivLU<MatrixXd>(A_i).solve(B_i) (PartialPivLU.h:128,
- * 226-244), computed as an FP32 LU with FP64 iterative refinement.  B_i and
- * X_i are n_i x nrhs column-major at offB[i]; iters[i] (nullable) receives
- * the refinement steps and info[i] = 0 converged, 1 not converged.  Sizes
- * up to 256. */
+ * 226-244, 426-496), computed as an FP32 LU on matrix cores with FP64
+ * iterative refinement to a normwise backward error <= 1e-14 per right-hand
+ * side; a system the FP32 factor cannot refine is refactored in FP64.  A_i
+ * is n_i x n_i column-major at A + offA[i]; B_i and X_i are n_i x nrhs
+ * column-major at offB[i] (1 <= nrhs <= 8).  info[i] = 0 refined from the
+ * FP32 factor, 1 refined after the FP64 refactor, 2 not converged (e.g.
+ * singular); iters[i] (nullable) = refinement steps of the final factor.
+ * Sizes 1..256.  All arrays are device arrays. */
 int eamd_dgesv_ir_batched(eamd_handle_t h, int64_t count, const int64_t* offA, const int32_t* n,
                           const double* A, const int64_t* offB, int64_t nrhs, const double* B,
                           double* X, int32_t* info, int32_t* iters);
+
+/* The same split into a plan (size sort of n[], one host synchronisation,
+ * autotuned workgroup size per 16-wide size bucket, working-copy layout) and
+ * an execution that only queues kernels (capturable).  Destroy the plan with
+ * eamd_batched_plan_destroy. */
+int eamd_batched_lu_plan_create(eamd_handle_t h, int64_t count, const int32_t* n, int64_t nrhs,
+                                eamd_batched_plan_t* plan);
+int eamd_dgesv_ir_batched_planned(eamd_handle_t h, eamd_batched_plan_t plan,
+                                  const int64_t* offA, const int32_t* n, const double* A,
+                                  const int64_t* offB, const double* B, double* X,
+                                  int32_t* info, int32_t* iters);
 
 /* ------------------------------------------------- host-buffer variants -- */
 /* Same operations on host (pageable) buffers: upload, compute, download.

@@ -1,0 +1,230 @@
+/* eigen_plugin.h -- Eigen's own back-end plug points, specialised for
+ * double to run on the MI355X through the eigen_amd C ABI.
+ *
+ * Include it after <Eigen/Dense> (or after <Eigen/LU> and <Eigen/Cholesky>)
+ * and before the first use of the decompositions, exactly where Eigen's
+ * LAPACKE/BLAS bindings are included (Eigen/LU:30-37, Eigen/Cholesky:34-41,
+ * Eigen/Core:543-552).  Unmodified user code then factors on the GPU:
+ *
+ *   PartialPivLU<MatrixXd>(A)  -> partial_lu_impl<double, ColMajor, int>::blocked_lu
+ *                                 (PartialPivLU.h:426-496; same plug point and
+ *                                 conventions as PartialPivLU_LAPACKE.h:42-77)
+ *   LLT<MatrixXd, Lower|Upper>  -> llt_inplace<double, Lower|Upper>::blocked
+ *                                 (LLT.h:330-360; as LLT_LAPACKE.h:42-93)
+ *   C.noalias() += A * B        -> general_matrix_matrix_product<..., double, ..., ColMajor>::run
+ *                                 (GeneralMatrixMatrix.h:49-199; as
+ *                                 GeneralMatrixMatrix_BLAS.h:49-116), when
+ *                                 EIGEN_AMD_USE_GEMM is defined
+ *   T.solveInPlace(X), matrix X -> triangular_solve_matrix<double, ..., ColMajor>::run
+ *                                 (TriangularSolverMatrix.h:38-329; as
+ *                                 TriangularSolverMatrix_BLAS.h:41-157), when
+ *                                 EIGEN_AMD_USE_TRSM is defined
+ *
+ * Buffers are Eigen's host buffers: each call uploads, computes and
+ * downloads through the eamd_host_* entry points on a per-thread handle
+ * (handles are thread-confined, like Eigen objects).  A failure of the GPU
+ * path throws std::runtime_error with the library's message: there is no
+ * silent CPU fallback.  Rank updates of an existing LLT keep Eigen's own
+ * algorithm (llt_rank_update_lower, LLT.h:231-299). */
+#ifndef EIGEN_AMD_EIGEN_PLUGIN_H
+#define EIGEN_AMD_EIGEN_PLUGIN_H
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../eigen_amd.h"
+
+namespace eigen_amd_plugin {
+
+/* One handle per host thread, created on first use and destroyed with the
+ * thread. */
+inline eamd_handle_t handle() {
+  struct Owner {
+    eamd_handle_t h = nullptr;
+    Owner() {
+      if (eamd_create(&h) != 0) h = nullptr;
+    }
+    ~Owner() {
+      if (h) eamd_destroy(h);
+    }
+  };
+  thread_local Owner owner;
+  if (!owner.h) throw std::runtime_error(std::string("eigen_amd: ") + eamd_last_error());
+  return owner.h;
+}
+
+inline void check(int status, const char* what) {
+  if (status != 0) {
+    const char* msg = eamd_last_error();
+    throw std::runtime_error(std::string("eigen_amd: ") + what + " returned " +
+                             std::to_string(status) + (msg && *msg ? std::string(": ") + msg : ""));
+  }
+}
+
+/* A row-major block seen as the column-major transpose: copy it out as
+ * column-major, run, copy back. */
+inline void to_colmajor(const double* src, Eigen::Index rows, Eigen::Index cols,
+                        Eigen::Index stride, std::vector<double>& dst) {
+  dst.resize((size_t)rows * (size_t)cols);
+  for (Eigen::Index i = 0; i < rows; ++i)
+    for (Eigen::Index j = 0; j < cols; ++j) dst[(size_t)j * rows + i] = src[i * stride + j];
+}
+inline void from_colmajor(const std::vector<double>& src, Eigen::Index rows, Eigen::Index cols,
+                          double* dst, Eigen::Index stride) {
+  for (Eigen::Index i = 0; i < rows; ++i)
+    for (Eigen::Index j = 0; j < cols; ++j) dst[i * stride + j] = src[(size_t)j * rows + i];
+}
+
+}  // namespace eigen_amd_plugin
+
+namespace Eigen {
+namespace internal {
+
+/* PartialPivLU<Matrix<double,...>>: returns the first exactly-zero pivot or
+ * -1; transpositions are 0-based, nb_transpositions counts t[i] != i. */
+template <int StorageOrder>
+struct partial_lu_impl<double, StorageOrder, int> {
+  static Index blocked_lu(Index rows, Index cols, double* lu_data, Index luStride,
+                          int* row_transpositions, int& nb_transpositions,
+                          Index maxBlockSize = 256) {
+    EIGEN_UNUSED_VARIABLE(maxBlockSize);
+    int32_t nbt = 0;
+    int64_t first_zero = -1;
+    if (StorageOrder == ColMajor) {
+      eigen_amd_plugin::check(
+          eamd_host_dgetrf(eigen_amd_plugin::handle(), rows, cols, lu_data, luStride,
+                           row_transpositions, &nbt, &first_zero),
+          "eamd_host_dgetrf");
+    } else {
+      std::vector<double> tmp;
+      eigen_amd_plugin::to_colmajor(lu_data, rows, cols, luStride, tmp);
+      eigen_amd_plugin::check(
+          eamd_host_dgetrf(eigen_amd_plugin::handle(), rows, cols, tmp.data(), rows,
+                           row_transpositions, &nbt, &first_zero),
+          "eamd_host_dgetrf");
+      eigen_amd_plugin::from_colmajor(tmp, rows, cols, lu_data, luStride);
+    }
+    nb_transpositions = nbt;
+    return (Index)first_zero;
+  }
+};
+
+/* LLT<Matrix<double,...>, Lower|Upper>: the failing column or -1. */
+template <int UpLo>
+struct eigen_amd_llt {
+  template <typename MatrixType>
+  static Index potrf(MatrixType& m) {
+    eigen_assert(m.rows() == m.cols());
+    const Index n = m.rows();
+    int64_t fail = -1;
+    const char uplo = UpLo == Lower ? 'L' : 'U';
+    if (!(MatrixType::Flags & RowMajorBit)) {
+      eigen_amd_plugin::check(eamd_host_dpotrf(eigen_amd_plugin::handle(), uplo, n,
+                                               &m.coeffRef(0, 0), m.outerStride(), &fail),
+                              "eamd_host_dpotrf");
+    } else {
+      /* row-major: the stored array is the column-major transpose, whose
+       * opposite triangle holds the same factor */
+      eigen_amd_plugin::check(eamd_host_dpotrf(eigen_amd_plugin::handle(),
+                                               UpLo == Lower ? 'U' : 'L', n, &m.coeffRef(0, 0),
+                                               m.outerStride(), &fail),
+                              "eamd_host_dpotrf");
+    }
+    return (Index)fail;
+  }
+};
+
+template <>
+struct llt_inplace<double, Lower> {
+  template <typename MatrixType>
+  static Index blocked(MatrixType& m) {
+    return eigen_amd_llt<Lower>::potrf(m);
+  }
+  template <typename MatrixType, typename VectorType>
+  static Index rankUpdate(MatrixType& mat, const VectorType& vec,
+                          const typename MatrixType::RealScalar& sigma) {
+    return Eigen::internal::llt_rank_update_lower(mat, vec, sigma);
+  }
+};
+
+template <>
+struct llt_inplace<double, Upper> {
+  template <typename MatrixType>
+  static Index blocked(MatrixType& m) {
+    return eigen_amd_llt<Upper>::potrf(m);
+  }
+  template <typename MatrixType, typename VectorType>
+  static Index rankUpdate(MatrixType& mat, const VectorType& vec,
+                          const typename MatrixType::RealScalar& sigma) {
+    Transpose<MatrixType> matt(mat);
+    return llt_inplace<double, Lower>::rankUpdate(matt, vec.conjugate(), sigma);
+  }
+};
+
+#ifdef EIGEN_AMD_USE_GEMM
+/* res += alpha * op(lhs) * op(rhs), column-major result. */
+template <typename Index_, int LhsStorageOrder, bool ConjugateLhs, int RhsStorageOrder,
+          bool ConjugateRhs>
+struct general_matrix_matrix_product<Index_, double, LhsStorageOrder, ConjugateLhs, double,
+                                     RhsStorageOrder, ConjugateRhs, ColMajor> {
+  typedef gebp_traits<double, double> Traits;
+  static void run(Index_ rows, Index_ cols, Index_ depth, const double* lhs, Index_ lhsStride,
+                  const double* rhs, Index_ rhsStride, double* res, Index_ resStride,
+                  double alpha, level3_blocking<double, double>& /*blocking*/,
+                  GemmParallelInfo<Index_>* /*info*/ = 0) {
+    const char ta = LhsStorageOrder == RowMajor ? 'T' : 'N';
+    const char tb = RhsStorageOrder == RowMajor ? 'T' : 'N';
+    eigen_amd_plugin::check(eamd_host_dgemm(eigen_amd_plugin::handle(), ta, tb, rows, cols,
+                                            depth, alpha, lhs, lhsStride, rhs, rhsStride, 1.0,
+                                            res, resStride),
+                            "eamd_host_dgemm");
+  }
+};
+#endif
+
+#ifdef EIGEN_AMD_USE_TRSM
+/* other <- tri^-1 other (left) or other tri^-1 (right); other column-major. */
+template <typename Index_, int Side, int Mode, bool Conjugate, int TriStorageOrder>
+struct eigen_amd_trsm {
+  static void run(Index_ size, Index_ otherSize, const double* tri, Index_ triStride,
+                  double* other, Index_ otherStride) {
+    const bool lower = (Mode & Lower) == Lower;
+    const char diag = (Mode & UnitDiag) ? 'U' : 'N';
+    /* a row-major triangle is the transpose of the column-major array */
+    const char uplo = (TriStorageOrder == RowMajor) ? (lower ? 'U' : 'L') : (lower ? 'L' : 'U');
+    const char trans = TriStorageOrder == RowMajor ? 'T' : 'N';
+    const char side = Side == OnTheLeft ? 'L' : 'R';
+    const Index_ m = Side == OnTheLeft ? size : otherSize;
+    const Index_ n = Side == OnTheLeft ? otherSize : size;
+    eigen_amd_plugin::check(eamd_host_dtrsm(eigen_amd_plugin::handle(), side, uplo, trans, diag,
+                                            m, n, 1.0, tri, triStride, other, otherStride),
+                            "eamd_host_dtrsm");
+  }
+};
+
+template <typename Index_, int Mode, bool Conjugate, int TriStorageOrder>
+struct triangular_solve_matrix<double, Index_, OnTheLeft, Mode, Conjugate, TriStorageOrder,
+                               ColMajor> {
+  static void run(Index_ size, Index_ otherSize, const double* tri, Index_ triStride,
+                  double* other, Index_ otherStride, level3_blocking<double, double>&) {
+    eigen_amd_trsm<Index_, OnTheLeft, Mode, Conjugate, TriStorageOrder>::run(
+        size, otherSize, tri, triStride, other, otherStride);
+  }
+};
+
+template <typename Index_, int Mode, bool Conjugate, int TriStorageOrder>
+struct triangular_solve_matrix<double, Index_, OnTheRight, Mode, Conjugate, TriStorageOrder,
+                               ColMajor> {
+  static void run(Index_ size, Index_ otherSize, const double* tri, Index_ triStride,
+                  double* other, Index_ otherStride, level3_blocking<double, double>&) {
+    eigen_amd_trsm<Index_, OnTheRight, Mode, Conjugate, TriStorageOrder>::run(
+        size, otherSize, tri, triStride, other, otherStride);
+  }
+};
+#endif
+
+}  // namespace internal
+}  // namespace Eigen
+
+#endif  // EIGEN_AMD_EIGEN_PLUGIN_H

@@ -1,0 +1,94 @@
+"""GPU test of the Eigen plug-point specialisations
+(include/eigen_amd/eigen_plugin.h): unmodified Eigen user code
+(tests/adapter/eigen_plugin_check.cpp, compiled with g++ against the Eigen
+headers into oracle/_ref/) runs PartialPivLU<MatrixXd>, LLT<MatrixXd>,
+GEMM and triangular solves through the C ABI; its outputs are checked
+against the oracle and closed forms here."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "oracle", "_ref", "eigen_plugin_check")
+
+
+def _load(d, name, shape, dtype=np.float64):
+    a = np.fromfile(os.path.join(d, name), dtype=dtype)
+    return a.reshape(shape, order="F") if len(shape) > 1 else a
+
+
+@pytest.fixture(scope="module")
+def out(tmp_path_factory):
+    if not os.path.exists(BIN):
+        pytest.skip("oracle/_ref/eigen_plugin_check not built (needs the Eigen headers at build time)")
+    d = str(tmp_path_factory.mktemp("plugin"))
+    r = subprocess.run([BIN, d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    return d
+
+
+def berr(a, x, b):
+    r = b - a @ x
+    return (np.abs(r).max(axis=0) /
+            (np.abs(a).sum(axis=1).max() * np.abs(x).max(axis=0) + np.abs(b).max(axis=0))).max()
+
+
+def test_plugin_partialpivlu(gpu, out):
+    n = 300
+    A = _load(out, "lu_A.f64", (n, n))
+    b = _load(out, "lu_b.f64", (n, 2))
+    LU = _load(out, "lu_LU.f64", (n, n))
+    perm = _load(out, "lu_perm.i32", (n,), np.int32)
+    x = _load(out, "lu_x.f64", (n, 2))
+    lu_r, t_r, nbt_r, fz_r = po.lu(A)
+    np.testing.assert_array_equal(perm, po.perm_from_transpositions(t_r))
+    np.testing.assert_allclose(LU, lu_r, rtol=0, atol=1e-11 * np.abs(lu_r).max())
+    assert berr(A, x, b) <= 1e-14
+    det = _load(out, "lu_det.f64", (1,))[0]
+    det_r = (-1.0) ** nbt_r * np.prod(np.diag(lu_r))
+    assert abs(det - det_r) <= 1e-9 * abs(det_r)
+
+
+def test_plugin_llt(gpu, out):
+    n = 300
+    S = _load(out, "llt_S.f64", (n, n))
+    b = _load(out, "llt_b.f64", (n, 3))
+    L = _load(out, "llt_L.f64", (n, n))
+    U = _load(out, "llt_U.f64", (n, n))
+    x = _load(out, "llt_x.f64", (n, 3))
+    info = _load(out, "llt_info.i32", (2,), np.int32)
+    assert (info == 0).all()
+    l_r, fail = po.llt(S)
+    assert fail == -1
+    l_r = np.tril(l_r)
+    np.testing.assert_allclose(L, l_r, rtol=0, atol=1e-12 * np.abs(l_r).max())
+    np.testing.assert_allclose(U, l_r.T, rtol=0, atol=1e-12 * np.abs(l_r).max())
+    assert berr(S, x, b) <= 1e-14
+    assert _load(out, "llt_bad_info.i32", (1,), np.int32)[0] == 1
+
+
+def test_plugin_gemm(gpu, out):
+    A = _load(out, "gemm_A.f64", (211, 173))
+    B = _load(out, "gemm_B.f64", (173, 157))
+    C0 = _load(out, "gemm_C0.f64", (211, 157))
+    C = _load(out, "gemm_C.f64", (211, 157))
+    ref = C0 + A @ B
+    np.testing.assert_allclose(C, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    T = _load(out, "gemm_T.f64", (173, 64))
+    assert T.shape == (173, 64) and np.isfinite(T).all()
+
+
+def test_plugin_trsm(gpu, out):
+    T = _load(out, "trsm_T.f64", (180, 180))
+    X10 = _load(out, "trsm_X10.f64", (180, 40))
+    X20 = _load(out, "trsm_X20.f64", (55, 180))
+    X1 = _load(out, "trsm_X1.f64", (180, 40))
+    X2 = _load(out, "trsm_X2.f64", (55, 180))
+    np.testing.assert_allclose(np.tril(T) @ X1, X10, rtol=0, atol=1e-11 * np.abs(X10).max())
+    Uu = np.triu(T, 1) + np.eye(180)
+    np.testing.assert_allclose(X2 @ Uu, X20, rtol=0, atol=1e-9 * np.abs(X20).max())
