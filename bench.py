@@ -103,6 +103,73 @@ def cpu_baseline(n=4096):
                       f"{dt:.2f} s (HPL flop count)"}
 
 
+def _median_ms(fn, reps, warm=2):
+    import statistics
+
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return statistics.median(ts)
+
+
+def dense_fields():
+    """The plain FP64 drop-ins at their own configurations: PartialPivLU
+    factor + solve at n = 1024 (pivots compared with the oracle's), GEMM
+    8192^3, and LLT of A = G G^T / 8192 + I (n = 8192) with 16 RHS."""
+    import numpy as np
+    import torch
+
+    import eigen_amd as ea
+    from oracle import pyoracle as po
+    out = {}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1024)
+    n = 1024
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g).t()
+    b = torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=g)
+    state = {}
+
+    def lu_solve():
+        state["lu"] = ea.PartialPivLU(A)
+        state["x"] = state["lu"].solve(b)
+    ms = _median_ms(lu_solve, 20, 3)
+    _, t_ref, _, _ = po.lu(A.cpu().numpy())
+    t = state["lu"].transpositionsP().cpu().numpy()
+    out["lu_n1024"] = {"ms": round(ms, 3), "tflops": round(hpl_flops(n) / ms / 1e9, 4),
+                       "backward_error": ea.backward_error(A, state["x"], b),
+                       "pivots_differing_from_oracle": int((t != t_ref).sum())}
+    del A, b, state
+    n = 8192
+    Ga = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g).t()
+    Gb = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g).t()
+    Cm = ea.colmajor(torch.empty(n, n, dtype=torch.float64, device="cuda"))
+    ms = _median_ms(lambda: ea.gemm(Ga, Gb, Cm, beta=0.0), 5, 1)
+    out["gemm_8192"] = {"ms": round(ms, 2), "tflops": round(2.0 * n ** 3 / ms / 1e9, 2),
+                        "frac_fp64_peak": round(2.0 * n ** 3 / ms / 1e9 / 78.6, 4)}
+    S = ea.gemm(Ga, Ga, transb="T", alpha=1.0 / n)
+    S.diagonal().add_(1.0)
+    B = torch.randn(n, 16, dtype=torch.float64, device="cuda", generator=g).t().contiguous().t()
+    del Ga, Gb, Cm
+    st = {}
+
+    def llt():
+        st["f"] = ea.LLT(S)
+    ms_f = _median_ms(llt, 3, 1)
+    ms_s = _median_ms(lambda: st.__setitem__("x", st["f"].solve(B)), 3, 1)
+    out["llt_8192_16rhs"] = {"factor_ms": round(ms_f, 2),
+                             "factor_tflops": round(n ** 3 / 3.0 / ms_f / 1e9, 2),
+                             "solve_ms": round(ms_s, 2),
+                             "backward_error": ea.backward_error(S, st["x"], B)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +258,10 @@ def main():
             del Ah, xh
         except Exception as e:  # reported, not fatal for the headline
             extras["householder_kappa1e6"] = {"error": str(e)[:200]}
+        try:
+            extras.update(dense_fields())
+        except Exception as e:
+            extras["dense"] = {"error": str(e)[:200]}
         try:
             from eigen_amd import batched
             extras.update(batched.bench_fields())

@@ -126,6 +126,7 @@ enum WsSlot : int {
   kWsAux,
   kWsAux2,
   kWsFactor32,     // FP32 factors kept for refinement
+  kWsTsl,          // tournament-pivoting candidates and saved rows
 };
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

@@ -169,6 +169,50 @@ constexpr int slot_words() {
   return 4 + LW * Words<T>::n;
 }
 
+// Replays the interchanges piv_s[0..w) of a leaf (leaf-relative rows, in
+// column order) into the leaf-level slot table (rows 0..w-1 are slots
+// 0..w-1, rows swapped in from below are appended) and the panel-level one
+// (rows 0..pnb-1 are slots 0..pnb-1; the leaf starts at panel row row0).
+// One wave; lane 0 also stores piv[].
+__device__ void replay_tables(const int* piv_s, int w, int row0, int pnb, int lane, int* piv,
+                              int* lPos, int* lCont, int* s_lnS, int* pPos, int* pCont,
+                              int* s_pnS) {
+  for (int j = 0; j < w; ++j) {
+    const int p = piv_s[j];
+    if (lane == 0) piv[j] = p;
+    if (p == j) continue;
+    int slotb = p;
+    if (p >= w) {
+      const int nS = *s_lnS;
+      int found = -1;
+      for (int s2 = w + lane; s2 < nS; s2 += 64)
+        if (lPos[s2] == p) found = s2;
+      unsigned long long bal = __ballot(found >= 0);
+      slotb = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
+      if (lane == 0 && !bal) { lPos[nS] = p; lCont[nS] = nS; *s_lnS = nS + 1; }
+    }
+    const int pa = row0 + j, pb = row0 + p;
+    int pslot = pb;
+    if (pb >= pnb) {
+      const int nS = *s_pnS;
+      int found = -1;
+      for (int s2 = pnb + lane; s2 < nS; s2 += 64)
+        if (pPos[s2] == pb) found = s2;
+      unsigned long long bal = __ballot(found >= 0);
+      pslot = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
+      if (lane == 0 && !bal) { pPos[nS] = pb; pCont[nS] = nS; *s_pnS = nS + 1; }
+    }
+    if (lane == 0) {
+      int t = lCont[j];
+      lCont[j] = lCont[slotb];
+      lCont[slotb] = t;
+      t = pCont[pa];
+      pCont[pa] = pCont[pslot];
+      pCont[pslot] = t;
+    }
+  }
+}
+
 }  // namespace
 
 // Outputs: piv[c] leaf-relative transposition of leaf column c; leaf
@@ -399,44 +443,8 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   __syncthreads();
   // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
   // permutation tables, replaying the interchanges in column order
-  if (g == 0 && wave == 0) {
-    for (int j = 0; j < w; ++j) {
-      const int p = s_piv[j];
-      if (lane == 0) piv[j] = p;
-      if (p == j) continue;
-      // leaf-level table: rows 0..w-1 are slots 0..w-1
-      int slotb = p;
-      if (p >= w) {
-        const int nS = s_lnS;
-        int found = -1;
-        for (int s2 = w + lane; s2 < nS; s2 += 64)
-          if (lPos[s2] == p) found = s2;
-        unsigned long long bal = __ballot(found >= 0);
-        slotb = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
-        if (lane == 0 && !bal) { lPos[nS] = p; lCont[nS] = nS; s_lnS = nS + 1; }
-      }
-      // panel-level table: rows 0..pnb-1 are slots 0..pnb-1
-      const int pa = row0 + j, pb = row0 + p;
-      int pslot = pb;
-      if (pb >= pnb) {
-        const int nS = s_pnS;
-        int found = -1;
-        for (int s2 = pnb + lane; s2 < nS; s2 += 64)
-          if (pPos[s2] == pb) found = s2;
-        unsigned long long bal = __ballot(found >= 0);
-        pslot = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
-        if (lane == 0 && !bal) { pPos[nS] = pb; pCont[nS] = nS; s_pnS = nS + 1; }
-      }
-      if (lane == 0) {
-        int t = lCont[j];
-        lCont[j] = lCont[slotb];
-        lCont[slotb] = t;
-        t = pCont[pa];
-        pCont[pa] = pCont[pslot];
-        pCont[pslot] = t;
-      }
-    }
-  }
+  if (g == 0 && wave == 0)
+    replay_tables(s_piv, w, row0, pnb, lane, piv, lPos, lCont, &s_lnS, pPos, pCont, &s_pnS);
   __syncthreads();
   if (valid) {
     T* dst = P + (int64_t)i * ldp;
@@ -500,6 +508,293 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
   hipLaunchKernelGGL((leaf_lu_kernel<T, BLOCK>), dim3(G), dim3(BLOCK), 0, h->stream, P, ldp, m, w,
                      G, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb, zero_col,
                      reinterpret_cast<u64*>(b + 256), reinterpret_cast<int*>(b), epoch_base);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Tournament pivoting (CALU: Grigori, Demmel, Xiang, "CALU: a communication
+// optimal LU factorization algorithm", SIAM J. Matrix Anal. Appl. 32, 2011)
+// for the FP32 panel of the mixed-precision solver, whose result is judged
+// by the backward error after refinement rather than by Eigen's pivot
+// sequence.  A leaf of m rows costs three launches instead of one exchange
+// round trip per column:
+//   1. every workgroup runs partial pivoting on its own 1024 rows, in
+//      registers, and records the rows it picked (up to 32 candidates);
+//   2. one workgroup runs partial pivoting on the original values of all
+//      candidates: the rows it picks, in order, are the leaf's pivots and its
+//      first 32 rows after elimination are the factored top block L11\U11,
+//      written in place; it also saves the original rows 0..w-1, replays the
+//      interchanges into the permutation tables and lists where the displaced
+//      rows go;
+//   3. every other row is solved against U11 (l = a U11^-1) in parallel.
+namespace {
+
+constexpr int kTslBlock = 1024;
+
+// MODE 0: local selection over rows blockIdx.x*1024 + tid.  MODE 1: the
+// final round over the candidate lists.
+template <int MODE>
+__global__ __launch_bounds__(kTslBlock) void tsl_kernel(
+    float* __restrict__ P, int64_t ldp, int m, int w, int* __restrict__ cand,
+    int* __restrict__ cand_n, int G1, float* __restrict__ saved, int* __restrict__ moved,
+    int* __restrict__ piv, int* __restrict__ lperm_n, int* __restrict__ lpos,
+    int* __restrict__ lsrc, int* __restrict__ pperm_n, int* __restrict__ ppos,
+    int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col) {
+  constexpr int BLOCK = kTslBlock, NWAVE = BLOCK / 64;
+  __shared__ double red_s[NWAVE];
+  __shared__ int red_i[NWAVE];
+  __shared__ float prow[2][LW];  // pivot row of the step
+  __shared__ float jrow[2][LW];  // row at position j before the interchange
+  __shared__ int porig[2][2];    // their original (leaf-relative) rows
+  __shared__ int s_sel[LW];
+  __shared__ int s_off[kMaxG + 1];
+  __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
+  __shared__ int pPos[512], pCont[512], s_pnS;
+  __shared__ int s_piv[LW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int orig, nrows;
+  bool valid;
+  if (MODE == 0) {
+    orig = blockIdx.x * BLOCK + tid;
+    valid = orig < m;
+    nrows = min(BLOCK, m - (int)blockIdx.x * BLOCK);
+  } else {
+    if (tid == 0) {
+      int acc = 0;
+      for (int g = 0; g < G1; ++g) { s_off[g] = acc; acc += cand_n[g]; }
+      s_off[G1] = acc;
+    }
+    __syncthreads();
+    nrows = s_off[G1];
+    valid = tid < nrows;
+    orig = 0;
+    if (valid) {
+      int g = 0;
+      while (s_off[g + 1] <= tid) ++g;
+      orig = cand[g * LW + tid - s_off[g]];
+    }
+  }
+  float v[LW];
+  {
+    const float* src = P + (int64_t)orig * ldp;
+#pragma unroll
+    for (int c = 0; c < LW; ++c) v[c] = (valid && c < w) ? src[c] : 0.f;
+  }
+  if (MODE == 1)  // the original rows 0..w-1, before the top block overwrites them
+    for (int e = tid; e < w * LW; e += BLOCK) {
+      const int r = e / LW, c = e - r * LW;
+      saved[e] = c < w ? P[(int64_t)r * ldp + c] : 0.f;
+    }
+  const int steps = min(w, nrows);
+  int first_zero = -1;
+#pragma unroll 1
+  for (int j = 0; j < steps; ++j) {
+    const int par = j & 1;
+    const float vj = pick(v, j);
+    double sc = -2.0;
+    if (valid && tid >= j) sc = score_of((double)vj, tid == j);
+    const double wmax = wave_max(sc);
+    const int wl = __ffsll((long long)__ballot(sc == wmax)) - 1;
+    if (lane == 0) {
+      red_s[wave] = wmax;
+      red_i[wave] = wmax > -2.0 ? wave * 64 + wl : 0x7fffffff;
+    }
+    __syncthreads();
+    double best = red_s[0];
+    int bq = 0;
+#pragma unroll
+    for (int q = 1; q < NWAVE; ++q) {
+      const double s2 = red_s[q];
+      if (s2 > best) { best = s2; bq = q; }
+    }
+    const int bi = red_i[bq];  // position j holds a valid row, so bi is set
+    if (tid == bi) {
+#pragma unroll
+      for (int c = 0; c < LW; ++c) prow[par][c] = v[c];
+      porig[par][0] = orig;
+    }
+    if (tid == j) {
+#pragma unroll
+      for (int c = 0; c < LW; ++c) jrow[par][c] = v[c];
+      porig[par][1] = orig;
+    }
+    __syncthreads();
+    const float* up = prow[par];
+    const float* rp = jrow[par];
+    if (bi != j) {
+      if (tid == j) {
+#pragma unroll
+        for (int c = 0; c < LW; ++c) v[c] = up[c];
+        orig = porig[par][0];
+      } else if (tid == bi) {
+#pragma unroll
+        for (int c = 0; c < LW; ++c) v[c] = rp[c];
+        orig = porig[par][1];
+      }
+    }
+    if (tid == 0) s_sel[j] = porig[par][0];
+    const float pv = up[j];
+    if (pv != 0.f) {
+      if (valid && tid > j) {
+        float u[LW];
+#pragma unroll
+        for (int c = 0; c < LW; ++c) u[c] = up[c];
+        const float l = (tid == bi ? rp[j] : vj) / pv;
+        place(v, j, l);
+        axpy_tail(v, u, j, l);
+      }
+    } else if (first_zero < 0) {
+      first_zero = j;
+    }
+  }
+  __syncthreads();
+  if (MODE == 0) {
+    if (tid < steps) cand[blockIdx.x * LW + tid] = s_sel[tid];
+    if (tid == 0) cand_n[blockIdx.x] = steps;
+    return;
+  }
+  // ---- final round: the top block in place (rows 0..w-1)
+  if (tid < steps) {
+    float* dst = P + (int64_t)tid * ldp;
+#pragma unroll
+    for (int c = 0; c < LW; ++c)
+      if (c < w) dst[c] = v[c];
+  }
+  // ---- transpositions from the selection order (wave 0): lane k holds the
+  // row now at touched position tpos_k; positions 0..w-1 are entries 0..w-1
+  if (wave == 0) {
+    int tpos = lane < w ? lane : -1, trow = lane < w ? lane : -1;
+    int nent = w;
+    for (int c = 0; c < w; ++c) {
+      const int p = s_sel[c];
+      const unsigned long long hit = __ballot(trow == p);
+      int k;
+      if (hit) {
+        k = __ffsll((long long)hit) - 1;
+      } else {  // untouched row p (>= w): it sits at position p
+        k = nent++;
+        if (lane == k) { tpos = p; trow = p; }
+      }
+      const int pos = __shfl(tpos, k);
+      if (lane == 0) s_piv[c] = pos;
+      const int rc = __shfl(trow, c), rk = __shfl(trow, k);
+      if (lane == c) trow = rk;
+      if (lane == k) trow = rc;
+    }
+    // positions >= w now hold displaced original rows < w: list them
+    const bool mv = tpos >= w && lane < nent;
+    const unsigned long long mb = __ballot(mv);
+    const int idx = __popcll(mb & ((1ull << lane) - 1ull));
+    if (mv) {
+      moved[1 + 2 * idx] = tpos;
+      moved[2 + 2 * idx] = trow;
+    }
+    if (lane == 0) moved[0] = __popcll(mb);
+    // permutation tables, as the partial-pivoting leaf writes them
+    for (int e = lane; e < w; e += 64) { lPos[e] = e; lCont[e] = e; }
+    if (lane == 0) { s_lnS = w; s_pnS = *pperm_n; }
+    const int pn0 = *pperm_n;
+    for (int e = lane; e < pn0; e += 64) { pPos[e] = ppos[e]; pCont[e] = psrc[e]; }
+    replay_tables(s_piv, w, row0, pnb, lane, piv, lPos, lCont, &s_lnS, pPos, pCont, &s_pnS);
+    const int ln = s_lnS, pn = s_pnS;
+    for (int e = lane; e < ln; e += 64) { lpos[e] = lPos[e]; lsrc[e] = lCont[e]; }
+    for (int e = lane; e < pn; e += 64) { ppos[e] = pPos[e]; psrc[e] = pCont[e]; }
+    if (lane == 0) {
+      *lperm_n = ln;
+      *pperm_n = pn;
+      *zero_col = first_zero;
+    }
+  }
+}
+
+// Rows w..m-1 of the leaf: l = a U11^-1, where a is the row's original
+// value (its own row, or a saved row 0..w-1 for a position listed in moved).
+// FULL: w == 32, without the column guards (which would multiply the code).
+template <bool FULL>
+__global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, int64_t ldp, int m,
+                                                        int w, const float* __restrict__ saved,
+                                                        const int* __restrict__ moved) {
+  if (FULL) w = LW;
+  __shared__ float U[LW][LW + 1];
+  __shared__ int mpos[LW], msrc[LW];
+  __shared__ int s_nm;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < LW * LW; e += 256) {
+    const int r = e / LW, c = e - r * LW;
+    U[r][c] = (r < w && c < w) ? P[(int64_t)r * ldp + c] : 0.f;
+  }
+  if (tid == 0) s_nm = moved[0];
+  if (tid < LW) {
+    mpos[tid] = moved[1 + 2 * tid];
+    msrc[tid] = moved[2 + 2 * tid];
+  }
+  __syncthreads();
+  const int i = w + blockIdx.x * 256 + tid;
+  if (i >= m) return;
+  int src = -1;
+  for (int k = 0; k < s_nm; ++k)
+    if (mpos[k] == i) src = msrc[k];
+  float a[LW];
+  if (src >= 0) {
+#pragma unroll
+    for (int c = 0; c < LW; ++c) a[c] = saved[src * LW + c];
+  } else {
+    const float* row = P + (int64_t)i * ldp;
+#pragma unroll
+    for (int c = 0; c < LW; ++c) a[c] = (FULL || c < w) ? row[c] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < LW; ++k) {
+    const float d = U[k][k];
+    const float x = d != 0.f ? a[k] / d : a[k];  // a zero pivot leaves its column unscaled
+    a[k] = x;
+#pragma unroll
+    for (int c = k + 1; c < LW; ++c) a[c] -= x * U[k][c];
+  }
+  float* dst = P + (int64_t)i * ldp;
+#pragma unroll
+  for (int c = 0; c < LW; ++c)
+    if (FULL || c < w) dst[c] = a[c];
+}
+
+}  // namespace
+
+// Same outputs as leaf_lu; FP32 only.  Panels of up to 1024 rows take the
+// partial-pivoting leaf (one workgroup).
+int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
+                       int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
+                       int row0, int pnb, int* zero_col) {
+  const int G1 = ceil_div(m, kTslBlock);
+  if (G1 <= 1)
+    return leaf_lu<float>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
+                          pnb, zero_col);
+  if (w > LW || pnb > 256 || G1 > kMaxG) {
+    set_error("tournament leaf: %d rows x %d columns unsupported", m, w);
+    return kErrUnsupported;
+  }
+  void* base = nullptr;
+  const size_t need = sizeof(int) * (kMaxG * LW + kMaxG + 2 * LW + 8) + sizeof(float) * LW * LW;
+  EAMD_CHECK(h->ws.get(kWsTsl, need, &base));
+  int* cand = static_cast<int*>(base);
+  int* cand_n = cand + kMaxG * LW;
+  int* moved = cand_n + kMaxG;
+  float* saved = reinterpret_cast<float*>(moved + 2 * LW + 8);
+  hipStream_t s = h->stream;
+  hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(kTslBlock), 0, s, P, ldp, m, w, cand, cand_n,
+                     G1, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
+                     zero_col);
+  hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(kTslBlock), 0, s, P, ldp, m, w, cand, cand_n,
+                     G1, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
+                     zero_col);
+  if (m > w) {
+    if (w == LW)
+      hipLaunchKernelGGL(tsl_apply_kernel<true>, dim3(ceil_div(m - w, 256)), dim3(256), 0, s, P,
+                         ldp, m, w, saved, moved);
+    else
+      hipLaunchKernelGGL(tsl_apply_kernel<false>, dim3(ceil_div(m - w, 256)), dim3(256), 0, s, P,
+                         ldp, m, w, saved, moved);
+  }
   EAMD_HIP(hipGetLastError());
   return kOk;
 }

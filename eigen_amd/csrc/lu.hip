@@ -16,6 +16,8 @@
 //     (laswp_perm_kernel, from the panel's composed row permutation), the
 //     triangular solve for U12 and the trailing update (FP64 MFMA GEMM, or
 //     the mixed-precision update for the FP32 working copy).
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 #include "../../include/eigen_amd.h"
@@ -212,7 +214,29 @@ template <typename T>
 int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n, int* lpos,
             int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col);
 
+int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
+                       int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
+                       int row0, int pnb, int* zero_col);
+int env_int(const char* name, int dflt);
+
 constexpr int kLeaf = 32;
+
+// FP32 leaves use tournament pivoting (the mixed-precision solver judges
+// its factor by the refined backward error); EAMD_TOURNAMENT=0 selects
+// partial pivoting, as the FP64 factorization always uses.
+template <typename T>
+int panel_leaf(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n,
+               int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb,
+               int* zero_col) {
+  if constexpr (std::is_same<T, float>::value) {
+    static const bool tournament = env_int("EAMD_TOURNAMENT", 1) != 0;
+    if (tournament)
+      return leaf_lu_tournament(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
+                                row0, pnb, zero_col);
+  }
+  return leaf_lu<T>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
+                    zero_col);
+}
 
 // Row-major blocked LU driver.  `update(A12, A21, A22, mrows, ncols, k,
 // within_panel)` performs A22 -= A21 * A12 (row-major, ld ldw).  ord
@@ -250,8 +274,8 @@ int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int
       for (int j0 = 0; j0 < bs; j0 += kLeaf) {
         const int w = min(kLeaf, bs - j0);
         T* Lk = Pk + (int64_t)j0 * ldw + j0;
-        EAMD_CHECK(leaf_lu<T>(h, Lk, ldw, m - j0, w, piv_dev + k + j0, lperm_n, lpos, lsrc,
-                              pperm_n, ppos, psrc, j0, bs, zero_col));
+        EAMD_CHECK(panel_leaf<T>(h, Lk, ldw, m - j0, w, piv_dev + k + j0, lperm_n, lpos, lsrc,
+                                 pperm_n, ppos, psrc, j0, bs, zero_col));
         hipLaunchKernelGGL(record_zero_kernel, dim3(1), dim3(64), 0, s, zero_col, k + j0,
                            fz_dev);
         hipLaunchKernelGGL(add_offset_kernel, dim3(1), dim3(64), 0, s, piv_dev + k + j0, w,

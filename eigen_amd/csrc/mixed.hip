@@ -52,7 +52,9 @@ __device__ __forceinline__ void split_store(float x, float s, unsigned short* hi
     _Float16 h = (_Float16)y;
     *hi = *reinterpret_cast<unsigned short*>(&h);
     if (MODE == kModeF16x3) {
-      _Float16 l = (_Float16)((y - (float)h) * 2048.0f);
+      // the low part unscaled: below 2^-14 it is an FP16 subnormal, an
+      // absolute error of at most 2^-25 against a block maximum near 2^14
+      _Float16 l = (_Float16)(y - (float)h);
       *lo = *reinterpret_cast<unsigned short*>(&l);
     }
   }
@@ -192,14 +194,13 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int l16 = lane & 15, lk = lane >> 4;
 
-  v4f acc[4][4], accx[4][4];
+  // one accumulator per 16x16 tile: hi*hi and both cross terms (the low
+  // parts are unscaled) accumulate together in FP32
+  v4f acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-      accx[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
   // staging: each thread moves 32 bytes of each array per slab:
   // row = tid / 2, byte half = (tid & 1) * 32 of the 64-byte K slab
@@ -234,6 +235,20 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
 
   const int kslabs = KP / KS;
   gload(0);
+  // the C tile is read while the products run: result row (N index)
+  // 4*lk + r of each 16x16 tile, column (M index) l16
+  float cv[4][4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm + i * 16 + l16;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn + j * 16 + lk * 4 + r;
+        cv[i][j][r] = (gi < M && gj < N) ? C[(int64_t)gj * ldc + gi] : 0.f;
+      }
+  }
   lstore(0);
   __syncthreads();
   for (int ks = 0; ks < kslabs; ++ks) {
@@ -252,22 +267,26 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
       if (X3)
         bfl[j] = *reinterpret_cast<const uint4*>(&lds[buf][3][(wn + j * 16 + l16) * LROW + lk * 8]);
     }
+    // small terms first, so each tile's sum ends with its largest part
+    if (X3) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(bfl[j], af[i], acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(bf[j], afl[i], acc[i][j]);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc[i][j] = mfma16<MODE>(bf[j], af[i], acc[i][j]);
-        if (X3) {
-          accx[i][j] = mfma16<MODE>(bfl[j], af[i], accx[i][j]);
-          accx[i][j] = mfma16<MODE>(bf[j], afl[i], accx[i][j]);
-        }
-      }
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(bf[j], af[i], acc[i][j]);
     if (ks + 1 < kslabs) lstore(buf ^ 1);
     __syncthreads();
   }
 
   const float inv = 1.0f / (sa[tm] * sb[tn]);
-  // result row (N index) 4*lk + r of each 16x16 tile, column (M index) l16
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int gi = m0 + wm + i * 16 + l16;
@@ -278,10 +297,7 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
       for (int r = 0; r < 4; ++r) {
         const int gj = n0 + wn + j * 16 + lk * 4 + r;
         if (gj >= N) continue;
-        float v = acc[i][j][r];
-        if (X3) v += accx[i][j][r] * (1.0f / 2048.0f);
-        float* c = C + (int64_t)gj * ldc + gi;
-        *c -= v * inv;
+        C[(int64_t)gj * ldc + gi] = cv[i][j][r] - acc[i][j][r] * inv;
       }
   }
 }

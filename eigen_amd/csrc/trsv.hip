@@ -37,10 +37,11 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
                                                          int64_t ldx, int c0, int nrhs,
                                                          int* __restrict__ flags, int epoch,
                                                          int* __restrict__ err) {
+  static_assert(BS == 64, "four threads per row of a 64-row block");
+  constexpr int SEG = BS / 4;  // columns of a predecessor block per thread
   __shared__ T Dinv[BS][BS + 1];
   __shared__ T Tb[BS][BS + 1];
-  __shared__ T Xj[BS][NRC];
-  __shared__ T acc_s[BS][NRC];
+  __shared__ T accs[BS][NRC];
   __shared__ int s_abort;
   const int tid = threadIdx.x;
   const int nblk = (n + BS - 1) / BS;
@@ -49,98 +50,125 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
   const int rb0 = b * BS;
   const int bs = min(BS, n - rb0);
   const int nc = min(NRC, nrhs - c0);
+  const int r = tid >> 2, q = tid & 3;  // my row of the block, my quarter of each column block
 
-  // ---- diagonal block into LDS
+  // ---- diagonal block into LDS and its inverse (independent of the
+  // predecessors: overlaps their solves)
   for (int e = tid; e < BS * BS; e += 256) {
-    int r = e / BS, c = e - r * BS;
-    T v = T(0);
-    if (r < bs && c < bs) v = *tptr<T, ROWMAJOR>(A, lda, rb0 + r, rb0 + c);
-    Tb[r][c] = v;
+    const int rr = e / BS, cc = e - rr * BS;
+    Tb[rr][cc] = (rr < bs && cc < bs) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + rr, rb0 + cc) : T(0);
+    Dinv[rr][cc] = T(0);  // the padding of a partial last block stays exactly zero
   }
-  for (int e = tid; e < BS * NRC; e += 256) {
-    int r = e / NRC, c = e - r * NRC;
-    acc_s[r][c] = (r < bs && c < nc) ? X[(int64_t)(c0 + c) * ldx + rb0 + r] : T(0);
-  }
-  // the padding of a partial last block must be exactly zero
-  for (int e = tid; e < BS * BS; e += 256) Dinv[e / BS][e % BS] = T(0);
   if (tid == 0) s_abort = 0;
   __syncthreads();
-  // ---- invert it: thread c owns column c of the inverse
   if (tid < BS) {
     const int c = tid;
     if (c < bs) {
       if (LOWER) {
-        for (int r = 0; r < c; ++r) Dinv[r][c] = T(0);
-        T d = UNIT ? T(1) : T(1) / Tb[c][c];
+        const T d = UNIT ? T(1) : T(1) / Tb[c][c];
         Dinv[c][c] = d;
-        for (int r = c + 1; r < bs; ++r) {
-          T s = T(0);
-          for (int q = c; q < r; ++q) s += Tb[r][q] * Dinv[q][c];
-          Dinv[r][c] = UNIT ? -s : -s / Tb[r][r];
+        for (int rr = c + 1; rr < bs; ++rr) {
+          T sum = T(0);
+          for (int qq = c; qq < rr; ++qq) sum += Tb[rr][qq] * Dinv[qq][c];
+          Dinv[rr][c] = UNIT ? -sum : -sum / Tb[rr][rr];
         }
       } else {
-        for (int r = c + 1; r < bs; ++r) Dinv[r][c] = T(0);
-        T d = UNIT ? T(1) : T(1) / Tb[c][c];
+        const T d = UNIT ? T(1) : T(1) / Tb[c][c];
         Dinv[c][c] = d;
-        for (int r = c - 1; r >= 0; --r) {
-          T s = T(0);
-          for (int q = r + 1; q <= c; ++q) s += Tb[r][q] * Dinv[q][c];
-          Dinv[r][c] = UNIT ? -s : -s / Tb[r][r];
+        for (int rr = c - 1; rr >= 0; --rr) {
+          T sum = T(0);
+          for (int qq = rr + 1; qq <= c; ++qq) sum += Tb[rr][qq] * Dinv[qq][c];
+          Dinv[rr][c] = UNIT ? -sum : -sum / Tb[rr][rr];
         }
       }
     }
   }
-  // ---- accumulate the predecessors in dependency order
+
+  // ---- off-diagonal part, no workgroup barrier per predecessor: thread
+  // (r, q) accumulates row r over columns q*SEG .. q*SEG+SEG-1 of every
+  // predecessor block, prefetching the next block's matrix values (which do
+  // not depend on X) before waiting on that block's flag.
+  T acc[NRC];
+#pragma unroll
+  for (int c = 0; c < NRC; ++c) acc[c] = T(0);
   const int ndep = LOWER ? b : nblk - 1 - b;
-  for (int d = 0; d < ndep; ++d) {
+  const bool rvalid = r < bs;
+  auto load_seg = [&](int d, T (&tv)[SEG]) {
     const int jb = LOWER ? d : nblk - 1 - d;
-    const int cb0 = jb * BS;
-    const int cbs = min(BS, n - cb0);
-    __syncthreads();
-    // off-diagonal block T[rb0:, cb0:] (independent of X: load before waiting)
-    for (int e = tid; e < BS * BS; e += 256) {
-      int r, c;
-      if (ROWMAJOR) { r = e / BS; c = e - r * BS; } else { c = e / BS; r = e - c * BS; }
-      Tb[r][c] = (r < bs && c < cbs) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + r, cb0 + c) : T(0);
+    const int cb = jb * BS + q * SEG;
+#pragma unroll
+    for (int i = 0; i < SEG; ++i)
+      tv[i] = (rvalid && cb + i < n) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + r, cb + i) : T(0);
+  };
+  T tcur[SEG], tnext[SEG];
+  if (ndep > 0) load_seg(0, tcur);
+  int bad = 0;
+  for (int d = 0; d < ndep; ++d) {
+    if (d + 1 < ndep) load_seg(d + 1, tnext);
+    const int jb = LOWER ? d : nblk - 1 - d;
+    int spins = 0;
+    while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      if ((++spins & 255) == 0 &&
+          (spins > kSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        bad = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
     }
-    if (tid == 0) {
-      int spins = 0;
-      while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > kSpin ||
-            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          s_abort = 1;
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
+    if (bad) break;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int cb = jb * BS + q * SEG;
+#pragma unroll
+    for (int c = 0; c < NRC; ++c) {
+      if (c < nc) {
+        const T* xc = X + (int64_t)(c0 + c) * ldx + cb;
+        T sum = T(0);
+#pragma unroll
+        for (int i = 0; i < SEG; ++i)
+          if (cb + i < n) sum += tcur[i] * tload(xc + i);
+        acc[c] += sum;
       }
     }
-    __syncthreads();
-    if (s_abort) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int e = tid; e < BS * NRC; e += 256) {
-      int r = e / NRC, c = e - r * NRC;
-      Xj[r][c] = (r < cbs && c < nc) ? tload(&X[(int64_t)(c0 + c) * ldx + cb0 + r]) : T(0);
-    }
-    __syncthreads();
-    for (int e = tid; e < BS * NRC; e += 256) {
-      int r = e % BS, c = e / BS;
-      T s = T(0);
-#pragma unroll 8
-      for (int q = 0; q < BS; ++q) s += Tb[r][q] * Xj[q][c];
-      acc_s[r][c] -= s;
-    }
+#pragma unroll
+    for (int i = 0; i < SEG; ++i) tcur[i] = tnext[i];
+  }
+  if (bad) {
+    s_abort = 1;
+    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the four quarters of a row sit in lanes 4r .. 4r+3 of one wave
+#pragma unroll
+  for (int c = 0; c < NRC; ++c) {
+    acc[c] += __shfl_xor(acc[c], 1);
+    acc[c] += __shfl_xor(acc[c], 2);
+  }
+  if (q == 0) {
+#pragma unroll
+    for (int c = 0; c < NRC; ++c)
+      accs[r][c] = (rvalid && c < nc) ? X[(int64_t)(c0 + c) * ldx + rb0 + r] - acc[c] : T(0);
   }
   __syncthreads();
-  // ---- X_b = inv(T_bb) * acc
-  for (int e = tid; e < BS * NRC; e += 256) {
-    int r = e % BS, c = e / BS;
-    T s = T(0);
-#pragma unroll 8
-    for (int q = 0; q < BS; ++q) s += Dinv[r][q] * acc_s[q][c];
-    if (r < bs && c < nc)
-      __hip_atomic_store(&X[(int64_t)(c0 + c) * ldx + rb0 + r], s, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+  if (s_abort) return;
+  // ---- X_b = inv(T_bb) * (B_b - sum): thread (r, q) takes a quarter of the row of Dinv
+  T xs[NRC];
+#pragma unroll
+  for (int c = 0; c < NRC; ++c) {
+    T sum = T(0);
+#pragma unroll
+    for (int i = 0; i < SEG; ++i) sum += Dinv[r][q * SEG + i] * accs[q * SEG + i][c];
+    xs[c] = sum;
+  }
+#pragma unroll
+  for (int c = 0; c < NRC; ++c) {
+    xs[c] += __shfl_xor(xs[c], 1);
+    xs[c] += __shfl_xor(xs[c], 2);
+  }
+  if (q == 0 && rvalid) {
+#pragma unroll
+    for (int c = 0; c < NRC; ++c)
+      if (c < nc)
+        __hip_atomic_store(&X[(int64_t)(c0 + c) * ldx + rb0 + r], xs[c], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
