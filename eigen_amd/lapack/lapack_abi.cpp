@@ -1,0 +1,170 @@
+// lapack_abi.cpp -- dgetrf_/dgetrs_/dpotrf_/dpotrs_/dgemm_/dtrsm_ over the
+// eigen_amd host-buffer entry points.  Argument checks and info codes follow
+// lapack/lu.cpp:14-89, lapack/cholesky.cpp:14-72 and blas/level3_impl.h:12-179;
+// each host thread gets its own handle (handles are thread-confined).
+#include "../../include/eigen_amd_lapack.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/eigen_amd.h"
+
+namespace {
+
+eamd_handle_t handle() {
+  struct Owner {
+    eamd_handle_t h = nullptr;
+    Owner() {
+      if (eamd_create(&h) != 0) h = nullptr;
+    }
+    ~Owner() {
+      if (h) eamd_destroy(h);
+    }
+  };
+  thread_local Owner owner;
+  return owner.h;
+}
+
+int illegal(const char* name, int arg, int* info) {
+  std::fprintf(stderr, "On entry to %s, parameter number %d had an illegal value\n", name, arg);
+  if (info) *info = -arg;
+  return 0;
+}
+
+int device_failure(const char* name, int status, int* info) {
+  const char* msg = eamd_last_error();
+  std::fprintf(stderr, "%s: eigen_amd status %d%s%s\n", name, status, msg && *msg ? ": " : "",
+               msg ? msg : "");
+  if (info) *info = -1000;
+  return 0;
+}
+
+bool is_trans(char c) {
+  c = (char)std::toupper((unsigned char)c);
+  return c == 'N' || c == 'T' || c == 'C';
+}
+bool is_uplo(char c) {
+  c = (char)std::toupper((unsigned char)c);
+  return c == 'U' || c == 'L';
+}
+char up(char c) { return (char)std::toupper((unsigned char)c); }
+
+}  // namespace
+
+extern "C" {
+
+int dgetrf_(int* m, int* n, double* a, int* lda, int* ipiv, int* info) {
+  *info = 0;
+  if (*m < 0) return illegal("DGETRF", 1, info);
+  if (*n < 0) return illegal("DGETRF", 2, info);
+  if (*lda < std::max(1, *m)) return illegal("DGETRF", 4, info);
+  if (*m == 0 || *n == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DGETRF", -1000, info);
+  int32_t nbt = 0;
+  int64_t fz = -1;
+  const int st = eamd_host_dgetrf(h, *m, *n, a, *lda, ipiv, &nbt, &fz);
+  if (st != 0) return device_failure("DGETRF", st, info);
+  for (int i = 0; i < std::min(*m, *n); ++i) ipiv[i] += 1;
+  if (fz >= 0) *info = (int)fz + 1;
+  return 0;
+}
+
+int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb,
+            int* info) {
+  *info = 0;
+  if (!is_trans(*trans)) return illegal("DGETRS", 1, info);
+  if (*n < 0) return illegal("DGETRS", 2, info);
+  if (*nrhs < 0) return illegal("DGETRS", 3, info);
+  if (*lda < std::max(1, *n)) return illegal("DGETRS", 5, info);
+  if (*ldb < std::max(1, *n)) return illegal("DGETRS", 8, info);
+  if (*n == 0 || *nrhs == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DGETRS", -1000, info);
+  std::vector<int32_t> t(ipiv, ipiv + *n);
+  for (auto& v : t) v -= 1;
+  const char tr = up(*trans) == 'N' ? 'N' : 'T';  // real: adjoint = transpose
+  const int st = eamd_host_dgetrs(h, tr, *n, *nrhs, a, *lda, t.data(), b, *ldb);
+  if (st != 0) return device_failure("DGETRS", st, info);
+  return 0;
+}
+
+int dpotrf_(char* uplo, int* n, double* a, int* lda, int* info) {
+  *info = 0;
+  if (!is_uplo(*uplo)) return illegal("DPOTRF", 1, info);
+  if (*n < 0) return illegal("DPOTRF", 2, info);
+  if (*lda < std::max(1, *n)) return illegal("DPOTRF", 4, info);
+  if (*n == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DPOTRF", -1000, info);
+  int64_t fail = -1;
+  const int st = eamd_host_dpotrf(h, up(*uplo), *n, a, *lda, &fail);
+  if (st != 0) return device_failure("DPOTRF", st, info);
+  if (fail >= 0) *info = (int)fail + 1;
+  return 0;
+}
+
+int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, int* info) {
+  *info = 0;
+  if (!is_uplo(*uplo)) return illegal("DPOTRS", 1, info);
+  if (*n < 0) return illegal("DPOTRS", 2, info);
+  if (*nrhs < 0) return illegal("DPOTRS", 3, info);
+  if (*lda < std::max(1, *n)) return illegal("DPOTRS", 5, info);
+  if (*ldb < std::max(1, *n)) return illegal("DPOTRS", 7, info);
+  if (*n == 0 || *nrhs == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DPOTRS", -1000, info);
+  const int st = eamd_host_dpotrs(h, up(*uplo), *n, *nrhs, a, *lda, b, *ldb);
+  if (st != 0) return device_failure("DPOTRS", st, info);
+  return 0;
+}
+
+int dgemm_(const char* opa, const char* opb, const int* m, const int* n, const int* k,
+           const double* alpha, const double* a, const int* lda, const double* b, const int* ldb,
+           const double* beta, double* c, const int* ldc) {
+  int info = 0;
+  const char ta = up(*opa), tb = up(*opb);
+  if (!is_trans(ta)) info = 1;
+  else if (!is_trans(tb)) info = 2;
+  else if (*m < 0) info = 3;
+  else if (*n < 0) info = 4;
+  else if (*k < 0) info = 5;
+  else if (*lda < std::max(1, ta == 'N' ? *m : *k)) info = 8;
+  else if (*ldb < std::max(1, tb == 'N' ? *k : *n)) info = 10;
+  else if (*ldc < std::max(1, *m)) info = 13;
+  if (info) return illegal("DGEMM", info, nullptr);
+  if (*m == 0 || *n == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DGEMM", -1000, nullptr);
+  const int st = eamd_host_dgemm(h, ta == 'N' ? 'N' : 'T', tb == 'N' ? 'N' : 'T', *m, *n, *k,
+                                 *alpha, a, *lda, b, *ldb, *beta, c, *ldc);
+  if (st != 0) return device_failure("DGEMM", st, nullptr);
+  return 0;
+}
+
+int dtrsm_(const char* side, const char* uplo, const char* opa, const char* diag, const int* m,
+           const int* n, const double* alpha, const double* a, const int* lda, double* b,
+           const int* ldb) {
+  int info = 0;
+  const char sd = up(*side), ul = up(*uplo), ta = up(*opa), dg = up(*diag);
+  if (sd != 'L' && sd != 'R') info = 1;
+  else if (!is_uplo(ul)) info = 2;
+  else if (!is_trans(ta)) info = 3;
+  else if (dg != 'U' && dg != 'N') info = 4;
+  else if (*m < 0) info = 5;
+  else if (*n < 0) info = 6;
+  else if (*lda < std::max(1, sd == 'L' ? *m : *n)) info = 9;
+  else if (*ldb < std::max(1, *m)) info = 11;
+  if (info) return illegal("DTRSM", info, nullptr);
+  if (*m == 0 || *n == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DTRSM", -1000, nullptr);
+  const int st = eamd_host_dtrsm(h, sd, ul, ta == 'N' ? 'N' : 'T', dg, *m, *n, *alpha, a, *lda,
+                                 b, *ldb);
+  if (st != 0) return device_failure("DTRSM", st, nullptr);
+  return 0;
+}
+
+}  // extern "C"

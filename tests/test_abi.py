@@ -3,6 +3,8 @@ include/eigen_amd.h declares, and rejects bad arguments with the
 reference's xerbla numbering before touching the GPU (lapack/lu.cpp:16-24,
 lapack/cholesky.cpp:17-24, blas/level3_impl.h:47-57)."""
 import ctypes as C
+import os
+import re
 
 import pytest
 
@@ -25,6 +27,7 @@ def test_version():
 
 
 NULL = None
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_dgetrf_argument_codes():
@@ -80,3 +83,51 @@ def test_ir_defaults():
     eigen_amd.lib().eamd_ir_opts_default(C.byref(o))
     assert o.tol == pytest.approx(1e-14)
     assert o.factor_precision == 4 and o.gmres == 1
+
+
+# ------------------------------------------------ Fortran-ABI layer (CPU) --
+LAPACK_H = os.path.join(ROOT, "include", "eigen_amd_lapack.h")
+LAPACK_SO = os.path.join(ROOT, "eigen_amd", "libeigen_amd_lapack.so")
+
+
+def _lapack():
+    return C.CDLL(LAPACK_SO)
+
+
+def test_lapack_layer_exports_every_declared_symbol():
+    text = re.sub(r"/\*.*?\*/", "", open(LAPACK_H).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(d\w+_)\s*\(", text)))
+    assert names == ["dgemm_", "dgetrf_", "dgetrs_", "dpotrf_", "dpotrs_", "dtrsm_"]
+    L = _lapack()
+    assert all(hasattr(L, n) for n in names)
+
+
+def _ip(v):
+    return C.byref(C.c_int(v))
+
+
+def test_lapack_layer_argument_codes():
+    """info = -(index of the illegal argument), checked before any device
+    work (lapack/lu.cpp:16-24, lapack/cholesky.cpp:17-24)."""
+    L = _lapack()
+    info = C.c_int(0)
+    piv = (C.c_int * 4)()
+    L.dgetrf_(_ip(-1), _ip(4), None, _ip(4), piv, C.byref(info))
+    assert info.value == -1
+    L.dgetrf_(_ip(4), _ip(-1), None, _ip(4), piv, C.byref(info))
+    assert info.value == -2
+    L.dgetrf_(_ip(4), _ip(4), None, _ip(3), piv, C.byref(info))
+    assert info.value == -4
+    L.dgetrs_(C.c_char_p(b"X"), _ip(4), _ip(1), None, _ip(4), piv, None, _ip(4), C.byref(info))
+    assert info.value == -1
+    L.dgetrs_(C.c_char_p(b"N"), _ip(4), _ip(1), None, _ip(4), piv, None, _ip(3), C.byref(info))
+    assert info.value == -8
+    L.dpotrf_(C.c_char_p(b"Q"), _ip(4), None, _ip(4), C.byref(info))
+    assert info.value == -1
+    L.dpotrf_(C.c_char_p(b"L"), _ip(4), None, _ip(2), C.byref(info))
+    assert info.value == -4
+    L.dpotrs_(C.c_char_p(b"L"), _ip(4), _ip(-1), None, _ip(4), None, _ip(4), C.byref(info))
+    assert info.value == -3
+    # empty problems return at once with info 0
+    L.dgetrf_(_ip(0), _ip(0), None, _ip(1), piv, C.byref(info))
+    assert info.value == 0
