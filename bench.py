@@ -179,6 +179,9 @@ def main():
     ap.add_argument("--precision", default="fp16x3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device (a kernel of known traffic, to "
+                         "calibrate hardware byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -190,6 +193,14 @@ def main():
     from eigen_amd import mixed
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 4, "launches": 3}))
+        return
     n = args.n
     g = torch.Generator(device="cuda")
     g.manual_seed(16384)

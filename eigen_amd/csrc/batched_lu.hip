@@ -68,7 +68,7 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-// wave reductions (inputs never NaN): quad xor-1, xor-2, half-row and row
+// wave reductions (inputs must not be NaN): quad xor-1, xor-2, half-row and row
 // mirrors, then the four 16-lane rows read as uniform values
 __device__ __forceinline__ double wave_max(double s) {
   s = fmax(s, dpp_d<0xB1>(s));
@@ -113,9 +113,14 @@ struct LuSmem {
                                                                    : (kDiag + 2 * kVec);
 };
 
+// |x| with a NaN mapped to +infinity, so maxima keep it
+__device__ __forceinline__ double absinf(double x) { return x == x ? fabs(x) : INFINITY; }
+
+// maximum over the workgroup; a NaN counts as +infinity (fmax alone would
+// drop it and let a NaN residual pass the convergence test)
 template <int NW>
 __device__ __forceinline__ double block_max(double v, double* red, int lane, int wave) {
-  const double w = wave_max(v);
+  const double w = wave_max(v == v ? v : INFINITY);
   if (lane == 0) red[wave] = w;
   __syncthreads();
   double m = red[0];
@@ -406,7 +411,7 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
     for (int r = tid; r < n; r += NT) {
       double sacc = 0.0;
       for (int t = 0; t < n; ++t) sacc += fabs(As[r + (int64_t)t * n]);
-      arow = fmax(arow, sacc);
+      arow = fmax(arow, absinf(sacc));
     }
     const double anorm = block_max<NW>(arow, red_d, lane, wave);
     double bnorm[kMaxRhs];
@@ -414,7 +419,7 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
     for (int q = 0; q < kMaxRhs; ++q) {
       double v = 0.0;
       if (q < nrhs)
-        for (int r = tid; r < n; r += NT) v = fmax(v, fabs(Bs[r + (int64_t)q * n]));
+        for (int r = tid; r < n; r += NT) v = fmax(v, absinf(Bs[r + (int64_t)q * n]));
       bnorm[q] = q < nrhs ? block_max<NW>(v, red_d, lane, wave) : 0.0;
     }
 
@@ -439,8 +444,8 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
         for (int q = 0; q < kMaxRhs; ++q)
           if (q < nrhs) {
             rv[r * kMaxRhs + q] = (TF)acc[q];
-            rmax[q] = fmax(rmax[q], fabs(acc[q]));
-            xmax[q] = fmax(xmax[q], fabs(Xs[r + (int64_t)q * n]));
+            rmax[q] = fmax(rmax[q], absinf(acc[q]));
+            xmax[q] = fmax(xmax[q], absinf(Xs[r + (int64_t)q * n]));
           }
       }
       double worst = 0.0;

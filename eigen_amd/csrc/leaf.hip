@@ -16,6 +16,9 @@
 // aligned 8-byte store is single-copy atomic, so a word with the right epoch
 // has the right payload: no fences).  Every workgroup then reads all
 // candidates and row j in one parallel poll and resolves the winner locally.
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -134,6 +137,21 @@ __device__ __forceinline__ void axpy_tail(T (&v)[32], const T (&u)[32], int j, T
   }
 }
 
+// Rotation of a register row by 8 slots (compile-time moves): the column
+// loops work on groups of 8 columns that always sit in slots 0..7, so every
+// register access has a compile-time index; after 4 rotations the row is back
+// in its original order.
+template <typename T>
+__device__ __forceinline__ void rotate8(T (&v)[32]) {
+  T t[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) t[q] = v[q];
+#pragma unroll
+  for (int c = 0; c < 24; ++c) v[c] = v[c + 8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[24 + q] = t[q];
+}
+
 // Lane permutations of the DPP unit, applied to a double as two halves.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
@@ -178,6 +196,10 @@ __device__ void replay_tables(const int* piv_s, int w, int row0, int pnb, int la
                               int* lPos, int* lCont, int* s_lnS, int* pPos, int* pCont,
                               int* s_pnS) {
   for (int j = 0; j < w; ++j) {
+    // lane 0 appends to the tables that the other lanes search: keep the
+    // compiler from reusing values it loaded in an earlier iteration (a wave
+    // executes its LDS accesses in order)
+    asm volatile("" ::: "memory");
     const int p = piv_s[j];
     if (lane == 0) piv[j] = p;
     if (p == j) continue;
@@ -273,10 +295,14 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   // compile-time index, with the column selected by branch-free selects and
   // masked FMAs (a runtime index into a register array would go to scratch
   // memory; a fully unrolled loop overflows the instruction cache).
-  auto column = [&](int j) __attribute__((always_inline)) -> bool {
+  // Columns in groups of 8 that sit in register slots 0..7 (see rotate8):
+  // column j = 8*g8 + jj is slot jj, a compile-time index; slots [live, 32)
+  // hold columns finished in earlier groups and take no update.
+  auto column = [&](auto jjc, int j, int live) __attribute__((always_inline)) -> bool {
+    constexpr int jj = decltype(jjc)::value;
     const int par = j & 1;
     const unsigned epoch = epoch_base + (unsigned)j + 1u;
-    const T vj = pick(v, j);
+    const T vj = v[jj];
     // ---- 1. my candidate, the wave's (maximum score, then the lowest lane
     // holding it: lanes are in row order), then the workgroup's
     double sc = -2.0;
@@ -421,24 +447,36 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     }
     if (g == 0 && tid == 0) s_piv[j] = p;
     // ---- 6. scale and rank-1 update of the rows below j
-    const T pv = up[j];
+    const T pv = up[jj];
     if (pv != T(0)) {
       if (valid && i > j) {
-        T u[LW];
+        const T l = (i == p ? rp[jj] : vj) / pv;
+        v[jj] = l;
 #pragma unroll
-        for (int c = 0; c < LW; ++c) u[c] = up[c];
-        const T l = (i == p ? rp[j] : vj) / pv;
-        place(v, j, l);
-        axpy_tail(v, u, j, l);
+        for (int c = jj + 1; c < LW; ++c) {
+          const T u = (c < 8 || c < live) ? up[c] : T(0);
+          v[c] -= l * u;
+        }
       }
     } else if (first_zero < 0) {
       first_zero = j;
     }
     return true;
   };
+  const int ngroups = (w + 7) / 8;
+  bool ok = true;
 #pragma unroll 1
-  for (int j = 0; j < w; ++j)
-    if (!column(j)) break;
+  for (int g8 = 0; g8 < ngroups && ok; ++g8) {
+    const int live = LW - 8 * g8, j0 = 8 * g8;
+#define EAMD_COL(JJ)                                                                   \
+    if (ok && j0 + JJ < w) ok = column(std::integral_constant<int, JJ>{}, j0 + JJ, live);
+    EAMD_COL(0) EAMD_COL(1) EAMD_COL(2) EAMD_COL(3) EAMD_COL(4) EAMD_COL(5) EAMD_COL(6)
+    EAMD_COL(7)
+#undef EAMD_COL
+    rotate8(v);
+  }
+#pragma unroll 1
+  for (int r = ngroups; r < 4; ++r) rotate8(v);
   if (s_abort) return;
   __syncthreads();
   // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
@@ -530,14 +568,17 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
 //   3. every other row is solved against U11 (l = a U11^-1) in parallel.
 namespace {
 
-constexpr int kTslBlock = 1024;
+constexpr int kTslBlock = 256;  // rows (candidates) per workgroup
+constexpr int kTslFan = kTslBlock / LW;  // candidate groups merged per workgroup
 
-// MODE 0: local selection over rows blockIdx.x*1024 + tid.  MODE 1: the
-// final round over the candidate lists.
+// MODE 0: local selection over rows blockIdx.x*256 + tid, candidates to
+// cand_out.  MODE 2: merge of candidate groups [8 b, 8 b + 8) of cand_in into
+// group b of cand_out.  MODE 1: the final round over all G_in (<= 8) groups.
 template <int MODE>
 __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
-    float* __restrict__ P, int64_t ldp, int m, int w, int* __restrict__ cand,
-    int* __restrict__ cand_n, int G1, float* __restrict__ saved, int* __restrict__ moved,
+    float* __restrict__ P, int64_t ldp, int m, int w, const int* __restrict__ cand_in,
+    const int* __restrict__ cand_n_in, int G_in, int* __restrict__ cand,
+    int* __restrict__ cand_n, float* __restrict__ saved, int* __restrict__ moved,
     int* __restrict__ piv, int* __restrict__ lperm_n, int* __restrict__ lpos,
     int* __restrict__ lsrc, int* __restrict__ pperm_n, int* __restrict__ ppos,
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col) {
@@ -548,7 +589,7 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
   __shared__ float jrow[2][LW];  // row at position j before the interchange
   __shared__ int porig[2][2];    // their original (leaf-relative) rows
   __shared__ int s_sel[LW];
-  __shared__ int s_off[kMaxG + 1];
+  __shared__ int s_off[kTslFan + 1];
   __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
   __shared__ int pPos[512], pCont[512], s_pnS;
   __shared__ int s_piv[LW];
@@ -560,19 +601,21 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     valid = orig < m;
     nrows = min(BLOCK, m - (int)blockIdx.x * BLOCK);
   } else {
+    const int g0 = MODE == 2 ? blockIdx.x * kTslFan : 0;
+    const int ng = min(kTslFan, G_in - g0);
     if (tid == 0) {
       int acc = 0;
-      for (int g = 0; g < G1; ++g) { s_off[g] = acc; acc += cand_n[g]; }
-      s_off[G1] = acc;
+      for (int g = 0; g < ng; ++g) { s_off[g] = acc; acc += cand_n_in[g0 + g]; }
+      s_off[ng] = acc;
     }
     __syncthreads();
-    nrows = s_off[G1];
+    nrows = s_off[ng];
     valid = tid < nrows;
     orig = 0;
     if (valid) {
       int g = 0;
       while (s_off[g + 1] <= tid) ++g;
-      orig = cand[g * LW + tid - s_off[g]];
+      orig = cand_in[(g0 + g) * LW + tid - s_off[g]];
     }
   }
   float v[LW];
@@ -587,69 +630,82 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
       saved[e] = c < w ? P[(int64_t)r * ldp + c] : 0.f;
     }
   const int steps = min(w, nrows);
+  const int ngroups = (steps + 7) / 8;
   int first_zero = -1;
+  // groups of 8 columns in slots 0..7 (see rotate8); slots [live, 32) hold
+  // columns finished in earlier groups and take no update
 #pragma unroll 1
-  for (int j = 0; j < steps; ++j) {
-    const int par = j & 1;
-    const float vj = pick(v, j);
-    double sc = -2.0;
-    if (valid && tid >= j) sc = score_of((double)vj, tid == j);
-    const double wmax = wave_max(sc);
-    const int wl = __ffsll((long long)__ballot(sc == wmax)) - 1;
-    if (lane == 0) {
-      red_s[wave] = wmax;
-      red_i[wave] = wmax > -2.0 ? wave * 64 + wl : 0x7fffffff;
-    }
-    __syncthreads();
-    double best = red_s[0];
-    int bq = 0;
+  for (int g8 = 0; g8 < ngroups; ++g8) {
+    const int live = LW - 8 * g8;
 #pragma unroll
-    for (int q = 1; q < NWAVE; ++q) {
-      const double s2 = red_s[q];
-      if (s2 > best) { best = s2; bq = q; }
-    }
-    const int bi = red_i[bq];  // position j holds a valid row, so bi is set
-    if (tid == bi) {
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = g8 * 8 + jj;
+      if (j >= steps) break;
+      const int par = j & 1;
+      const float vj = v[jj];
+      double sc = -2.0;
+      if (valid && tid >= j) sc = score_of((double)vj, tid == j);
+      const double wmax = wave_max(sc);
+      const int wl = __ffsll((long long)__ballot(sc == wmax)) - 1;
+      if (lane == 0) {
+        red_s[wave] = wmax;
+        red_i[wave] = wmax > -2.0 ? wave * 64 + wl : 0x7fffffff;
+      }
+      __syncthreads();
+      double best = red_s[0];
+      int bq = 0;
 #pragma unroll
-      for (int c = 0; c < LW; ++c) prow[par][c] = v[c];
-      porig[par][0] = orig;
-    }
-    if (tid == j) {
+      for (int q = 1; q < NWAVE; ++q) {
+        const double s2 = red_s[q];
+        if (s2 > best) { best = s2; bq = q; }
+      }
+      const int bi = red_i[bq];  // position j holds a valid row, so bi is set
+      if (tid == bi) {
 #pragma unroll
-      for (int c = 0; c < LW; ++c) jrow[par][c] = v[c];
-      porig[par][1] = orig;
-    }
-    __syncthreads();
-    const float* up = prow[par];
-    const float* rp = jrow[par];
-    if (bi != j) {
+        for (int c = 0; c < LW; ++c) prow[par][c] = v[c];
+        porig[par][0] = orig;
+      }
       if (tid == j) {
 #pragma unroll
-        for (int c = 0; c < LW; ++c) v[c] = up[c];
-        orig = porig[par][0];
-      } else if (tid == bi) {
+        for (int c = 0; c < LW; ++c) jrow[par][c] = v[c];
+        porig[par][1] = orig;
+      }
+      __syncthreads();
+      const float* up = prow[par];
+      const float* rp = jrow[par];
+      if (bi != j) {
+        if (tid == j) {
 #pragma unroll
-        for (int c = 0; c < LW; ++c) v[c] = rp[c];
-        orig = porig[par][1];
+          for (int c = 0; c < LW; ++c) v[c] = up[c];
+          orig = porig[par][0];
+        } else if (tid == bi) {
+#pragma unroll
+          for (int c = 0; c < LW; ++c) v[c] = rp[c];
+          orig = porig[par][1];
+        }
+      }
+      if (tid == 0) s_sel[j] = porig[par][0];
+      const float pv = up[jj];
+      if (pv != 0.f) {
+        if (valid && tid > j) {
+          const float l = (tid == bi ? rp[jj] : vj) / pv;
+          v[jj] = l;
+#pragma unroll
+          for (int c = jj + 1; c < LW; ++c) {
+            const float u = (c < 8 || c < live) ? up[c] : 0.f;
+            v[c] -= l * u;
+          }
+        }
+      } else if (first_zero < 0) {
+        first_zero = j;
       }
     }
-    if (tid == 0) s_sel[j] = porig[par][0];
-    const float pv = up[j];
-    if (pv != 0.f) {
-      if (valid && tid > j) {
-        float u[LW];
-#pragma unroll
-        for (int c = 0; c < LW; ++c) u[c] = up[c];
-        const float l = (tid == bi ? rp[j] : vj) / pv;
-        place(v, j, l);
-        axpy_tail(v, u, j, l);
-      }
-    } else if (first_zero < 0) {
-      first_zero = j;
-    }
+    rotate8(v);
   }
+#pragma unroll 1
+  for (int r = ngroups; r < 4; ++r) rotate8(v);
   __syncthreads();
-  if (MODE == 0) {
+  if (MODE != 1) {
     if (tid < steps) cand[blockIdx.x * LW + tid] = s_sel[tid];
     if (tid == 0) cand_n[blockIdx.x] = steps;
     return;
@@ -697,6 +753,7 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     const int pn0 = *pperm_n;
     for (int e = lane; e < pn0; e += 64) { pPos[e] = ppos[e]; pCont[e] = psrc[e]; }
     replay_tables(s_piv, w, row0, pnb, lane, piv, lPos, lCont, &s_lnS, pPos, pCont, &s_pnS);
+    asm volatile("" ::: "memory");
     const int ln = s_lnS, pn = s_pnS;
     for (int e = lane; e < ln; e += 64) { lpos[e] = lPos[e]; lsrc[e] = lCont[e]; }
     for (int e = lane; e < pn; e += 64) { ppos[e] = pPos[e]; psrc[e] = pCont[e]; }
@@ -760,8 +817,9 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
 
 }  // namespace
 
-// Same outputs as leaf_lu; FP32 only.  Panels of up to 1024 rows take the
-// partial-pivoting leaf (one workgroup).
+// Same outputs as leaf_lu; FP32 only.  Panels of up to 256 rows take the
+// partial-pivoting leaf.  Candidate groups are merged 8 at a time until at
+// most 8 remain for the final round.
 int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
                        int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
                        int row0, int pnb, int* zero_col) {
@@ -769,24 +827,40 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   if (G1 <= 1)
     return leaf_lu<float>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
                           pnb, zero_col);
-  if (w > LW || pnb > 256 || G1 > kMaxG) {
+  if (w > LW || pnb > 256) {
     set_error("tournament leaf: %d rows x %d columns unsupported", m, w);
     return kErrUnsupported;
   }
   void* base = nullptr;
-  const size_t need = sizeof(int) * (kMaxG * LW + kMaxG + 2 * LW + 8) + sizeof(float) * LW * LW;
+  // two candidate buffers (ping-pong between merge levels), counts, moved
+  // list and the saved rows
+  const size_t need =
+      sizeof(int) * (2 * (size_t)G1 * LW + 2 * (size_t)G1 + 2 * LW + 8) + sizeof(float) * LW * LW;
   EAMD_CHECK(h->ws.get(kWsTsl, need, &base));
-  int* cand = static_cast<int*>(base);
-  int* cand_n = cand + kMaxG * LW;
-  int* moved = cand_n + kMaxG;
+  int* candA = static_cast<int*>(base);
+  int* candB = candA + (size_t)G1 * LW;
+  int* nA = candB + (size_t)G1 * LW;
+  int* nB = nA + G1;
+  int* moved = nB + G1;
   float* saved = reinterpret_cast<float*>(moved + 2 * LW + 8);
   hipStream_t s = h->stream;
-  hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(kTslBlock), 0, s, P, ldp, m, w, cand, cand_n,
-                     G1, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
-                     zero_col);
-  hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(kTslBlock), 0, s, P, ldp, m, w, cand, cand_n,
-                     G1, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
-                     zero_col);
+  hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(kTslBlock), 0, s, P, ldp, m, w, nullptr,
+                     nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos,
+                     psrc, row0, pnb, zero_col);
+  int G = G1;
+  int *cin = candA, *nin = nA, *cout = candB, *nout = nB;
+  while (G > kTslFan) {
+    const int Gn = ceil_div(G, kTslFan);
+    hipLaunchKernelGGL(tsl_kernel<2>, dim3(Gn), dim3(kTslBlock), 0, s, P, ldp, m, w, cin, nin, G,
+                       cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
+                       row0, pnb, zero_col);
+    G = Gn;
+    std::swap(cin, cout);
+    std::swap(nin, nout);
+  }
+  hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(kTslBlock), 0, s, P, ldp, m, w, cin, nin, G,
+                     cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
+                     pnb, zero_col);
   if (m > w) {
     if (w == LW)
       hipLaunchKernelGGL(tsl_apply_kernel<true>, dim3(ceil_div(m - w, 256)), dim3(256), 0, s, P,

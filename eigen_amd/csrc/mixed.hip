@@ -60,83 +60,103 @@ __device__ __forceinline__ void split_store(float x, float s, unsigned short* hi
   }
 }
 
+// Block maximum of |x| over a workgroup of 256 threads.
+__device__ __forceinline__ float block_absmax(float m, float* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// Eight consecutive k of one packed row: converted and stored as one 16-byte
+// write per array.
+template <int MODE>
+__device__ __forceinline__ void store8(const float (&x)[8], float s, unsigned short* ph,
+                                       unsigned short* pl) {
+  unsigned short hb[8], lb[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) split_store<MODE>(x[q], s, &hb[q], &lb[q]);
+  uint4 h, l;
+  __builtin_memcpy(&h, hb, 16);
+  *reinterpret_cast<uint4*>(ph) = h;
+  if (MODE == kModeF16x3) {
+    __builtin_memcpy(&l, lb, 16);
+    *reinterpret_cast<uint4*>(pl) = l;
+  }
+}
+
 // Pack rows of a row-major source (N rows x K, element (j,k) at S[j*ld+k])
 // into P[j*KP + k] (+ lo copy), scale per 128-row block.  One workgroup per
-// 128-row block; first pass: block max, second: convert.
+// 128-row block; a thread takes 8 consecutive k of a row per step (one
+// 16-byte store per array); first pass the block maximum, second the
+// conversion (its reads hit the cache).
 template <int MODE>
 __global__ __launch_bounds__(256) void pack_rows_kernel(const float* __restrict__ S, int64_t ld,
                                                         int N, int K, int KP,
                                                         unsigned short* __restrict__ Ph,
                                                         unsigned short* __restrict__ Pl,
                                                         float* __restrict__ scale) {
-  __shared__ float red[256];
+  __shared__ float red[4];
   const int j0 = blockIdx.x * MT;
   const int rows = min(MT, N - j0);
+  const int kc = KP / 8, chunks = MT * kc;
   float m = 0.0f;
-  for (int e = threadIdx.x; e < rows * K; e += 256) {
-    int jj = e / K, k = e - jj * K;
-    m = fmaxf(m, fabsf(S[(int64_t)(j0 + jj) * ld + k]));
+  for (int e = threadIdx.x; e < chunks; e += 256) {
+    const int jj = e / kc, k = (e - jj * kc) * 8;
+    if (jj < rows) {
+      const float* src = S + (int64_t)(j0 + jj) * ld + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (k + q < K) m = fmaxf(m, fabsf(src[q]));
+    }
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const float s = pow2_scale(red[0]);
+  const float s = pow2_scale(block_absmax(m, red));
   if (threadIdx.x == 0) scale[blockIdx.x] = s;
-  for (int e = threadIdx.x; e < MT * KP; e += 256) {
-    int jj = e / KP, k = e - jj * KP;
-    float x = (jj < rows && k < K) ? S[(int64_t)(j0 + jj) * ld + k] : 0.0f;
-    size_t o = (size_t)(j0 + jj) * KP + k;
-    unsigned short hbits = 0, lbits = 0;
-    split_store<MODE>(x, s, &hbits, &lbits);
-    Ph[o] = hbits;
-    if (MODE == kModeF16x3) Pl[o] = lbits;
+  for (int e = threadIdx.x; e < chunks; e += 256) {
+    const int jj = e / kc, k = (e - jj * kc) * 8;
+    const float* src = S + (int64_t)(j0 + jj) * ld + k;
+    float x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = (jj < rows && k + q < K) ? src[q] : 0.0f;
+    const size_t o = (size_t)(j0 + jj) * KP + k;
+    store8<MODE>(x, s, Ph + o, Pl + o);
   }
 }
 
 // Pack columns of a row-major source (K rows x M, element (k,i) at
-// S[k*ld+i]) into P[i*KP + k]: a transpose through LDS, 128 columns per
-// workgroup, K in chunks of 32.
+// S[k*ld+i]) into P[i*KP + k]: 128 columns per workgroup; a thread takes 8
+// consecutive k of one column per step (each of the 8 loads is coalesced
+// across the threads of a wave, the store is one 16-byte write per array).
 template <int MODE>
 __global__ __launch_bounds__(256) void pack_cols_kernel(const float* __restrict__ S, int64_t ld,
                                                         int M, int K, int KP,
                                                         unsigned short* __restrict__ Ph,
                                                         unsigned short* __restrict__ Pl,
                                                         float* __restrict__ scale) {
-  __shared__ float red[256];
-  __shared__ float tile[32][MT + 1];
+  __shared__ float red[4];
   const int i0 = blockIdx.x * MT;
   const int cols = min(MT, M - i0);
+  const int ii = threadIdx.x & (MT - 1), kg = threadIdx.x >> 7;  // 2 groups of k chunks
+  const int kc = KP / 8;
+  const bool cv = ii < cols;
   float m = 0.0f;
-  for (int e = threadIdx.x; e < K * MT; e += 256) {
-    int k = e / MT, ii = e - k * MT;
-    if (ii < cols) m = fmaxf(m, fabsf(S[(int64_t)k * ld + i0 + ii]));
+  for (int c = kg; c < kc; c += 2) {
+    const int k = c * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (cv && k + q < K) m = fmaxf(m, fabsf(S[(int64_t)(k + q) * ld + i0 + ii]));
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const float s = pow2_scale(red[0]);
+  const float s = pow2_scale(block_absmax(m, red));
   if (threadIdx.x == 0) scale[blockIdx.x] = s;
-  for (int k0 = 0; k0 < KP; k0 += 32) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < 32 * MT; e += 256) {
-      int kk = e / MT, ii = e - kk * MT;
-      tile[kk][ii] = (k0 + kk < K && ii < cols) ? S[(int64_t)(k0 + kk) * ld + i0 + ii] : 0.0f;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 32 * MT; e += 256) {
-      int ii = e / 32, kk = e - ii * 32;
-      size_t o = (size_t)(i0 + ii) * KP + k0 + kk;
-      unsigned short hbits = 0, lbits = 0;
-      split_store<MODE>(tile[kk][ii], s, &hbits, &lbits);
-      Ph[o] = hbits;
-      if (MODE == kModeF16x3) Pl[o] = lbits;
-    }
+  for (int c = kg; c < kc; c += 2) {
+    const int k = c * 8;
+    float x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      x[q] = (cv && k + q < K) ? S[(int64_t)(k + q) * ld + i0 + ii] : 0.0f;
+    const size_t o = (size_t)(i0 + ii) * KP + k;
+    store8<MODE>(x, s, Ph + o, Pl + o);
   }
 }
 

@@ -121,9 +121,13 @@ int main(int argc, char** argv) {
     T.diagonal().array() += 20.0;
     MatrixXd X1 = normal(180, 40, 20), X2 = normal(55, 180, 21);
     MatrixXd X10 = X1, X20 = X2;
+    // a unit triangle with N(0,1) entries is exponentially ill-conditioned:
+    // the unit-upper case uses entries scaled by 1/20
+    MatrixXd Tu = T / 20.0;
     T.triangularView<Eigen::Lower>().solveInPlace(X1);
-    T.triangularView<Eigen::UnitUpper>().solveInPlace<Eigen::OnTheRight>(X2);
+    Tu.triangularView<Eigen::UnitUpper>().solveInPlace<Eigen::OnTheRight>(X2);
     dump("trsm_T.f64", T);
+    dump("trsm_Tu.f64", Tu);
     dump("trsm_X10.f64", X10);
     dump("trsm_X20.f64", X20);
     dump("trsm_X1.f64", X1);

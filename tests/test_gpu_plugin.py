@@ -90,5 +90,6 @@ def test_plugin_trsm(gpu, out):
     X1 = _load(out, "trsm_X1.f64", (180, 40))
     X2 = _load(out, "trsm_X2.f64", (55, 180))
     np.testing.assert_allclose(np.tril(T) @ X1, X10, rtol=0, atol=1e-11 * np.abs(X10).max())
-    Uu = np.triu(T, 1) + np.eye(180)
-    np.testing.assert_allclose(X2 @ Uu, X20, rtol=0, atol=1e-9 * np.abs(X20).max())
+    Tu = _load(out, "trsm_Tu.f64", (180, 180))
+    Uu = np.triu(Tu, 1) + np.eye(180)
+    np.testing.assert_allclose(X2 @ Uu, X20, rtol=0, atol=1e-11 * np.abs(X20).max())
