@@ -191,17 +191,17 @@ constexpr int slot_words() {
 // column order) into the leaf-level slot table (rows 0..w-1 are slots
 // 0..w-1, rows swapped in from below are appended) and the panel-level one
 // (rows 0..pnb-1 are slots 0..pnb-1; the leaf starts at panel row row0).
-// One wave; lane 0 also stores piv[].
+// One wave; lane 0 also stores piv[] (plus piv_offset).
 __device__ void replay_tables(const int* piv_s, int w, int row0, int pnb, int lane, int* piv,
-                              int* lPos, int* lCont, int* s_lnS, int* pPos, int* pCont,
-                              int* s_pnS) {
+                              int piv_offset, int* lPos, int* lCont, int* s_lnS, int* pPos,
+                              int* pCont, int* s_pnS) {
   for (int j = 0; j < w; ++j) {
     // lane 0 appends to the tables that the other lanes search: keep the
     // compiler from reusing values it loaded in an earlier iteration (a wave
     // executes its LDS accesses in order)
     asm volatile("" ::: "memory");
     const int p = piv_s[j];
-    if (lane == 0) piv[j] = p;
+    if (lane == 0) piv[j] = p + piv_offset;
     if (p == j) continue;
     int slotb = p;
     if (p >= w) {
@@ -243,13 +243,15 @@ __device__ void replay_tables(const int* piv_s, int w, int row0, int pnb, int la
 // the leaves of a panel: on entry they hold the permutation of the previous
 // leaves; `row0` is the leaf's first row inside the panel, `pnb` the panel
 // width); zero_col the first exactly-zero pivot column of the leaf or -1.
+// piv[] is stored plus piv_offset, and a zero pivot is also recorded in
+// *fz_global (nullable) as first_zero + piv_offset when that is still < 0.
 template <typename T, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     T* __restrict__ P, int64_t ldp, int m, int w, int G, int* __restrict__ piv,
     int* __restrict__ lperm_n, int* __restrict__ lpos, int* __restrict__ lsrc,
     int* __restrict__ pperm_n, int* __restrict__ ppos, int* __restrict__ psrc, int row0,
     int pnb, int* __restrict__ zero_col, u64* __restrict__ xbuf, int* __restrict__ err,
-    unsigned epoch_base) {
+    unsigned epoch_base, int piv_offset, int* __restrict__ fz_global) {
   constexpr int NW = Words<T>::n;
   constexpr int SW = slot_words<T>();
   constexpr int NWAVE = BLOCK / 64;
@@ -482,7 +484,8 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
   // permutation tables, replaying the interchanges in column order
   if (g == 0 && wave == 0)
-    replay_tables(s_piv, w, row0, pnb, lane, piv, lPos, lCont, &s_lnS, pPos, pCont, &s_pnS);
+    replay_tables(s_piv, w, row0, pnb, lane, piv, piv_offset, lPos, lCont, &s_lnS, pPos, pCont,
+                  &s_pnS);
   __syncthreads();
   if (valid) {
     T* dst = P + (int64_t)i * ldp;
@@ -498,6 +501,7 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
       *lperm_n = ln;
       *pperm_n = pn;
       *zero_col = first_zero;
+      if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
     }
   }
 }
@@ -519,7 +523,8 @@ int panel_perm_init(hipStream_t s, int* pn, int* ppos, int* psrc, int nb) {
 // columns [0, w), w <= 32.
 template <typename T>
 int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n, int* lpos,
-            int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col) {
+            int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col,
+            int piv_offset, int* fz_global) {
   constexpr int BLOCK = LeafBlock<T>::v;
   if (w > LW || pnb > 256) {
     set_error("leaf width %d / panel width %d unsupported", w, pnb);
@@ -545,7 +550,8 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
   char* b = static_cast<char*>(base);
   hipLaunchKernelGGL((leaf_lu_kernel<T, BLOCK>), dim3(G), dim3(BLOCK), 0, h->stream, P, ldp, m, w,
                      G, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb, zero_col,
-                     reinterpret_cast<u64*>(b + 256), reinterpret_cast<int*>(b), epoch_base);
+                     reinterpret_cast<u64*>(b + 256), reinterpret_cast<int*>(b), epoch_base,
+                     piv_offset, fz_global);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
@@ -581,7 +587,8 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     int* __restrict__ cand_n, float* __restrict__ saved, int* __restrict__ moved,
     int* __restrict__ piv, int* __restrict__ lperm_n, int* __restrict__ lpos,
     int* __restrict__ lsrc, int* __restrict__ pperm_n, int* __restrict__ ppos,
-    int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col) {
+    int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
+    int* __restrict__ fz_global) {
   constexpr int BLOCK = kTslBlock, NWAVE = BLOCK / 64;
   __shared__ double red_s[NWAVE];
   __shared__ int red_i[NWAVE];
@@ -752,7 +759,8 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     if (lane == 0) { s_lnS = w; s_pnS = *pperm_n; }
     const int pn0 = *pperm_n;
     for (int e = lane; e < pn0; e += 64) { pPos[e] = ppos[e]; pCont[e] = psrc[e]; }
-    replay_tables(s_piv, w, row0, pnb, lane, piv, lPos, lCont, &s_lnS, pPos, pCont, &s_pnS);
+    replay_tables(s_piv, w, row0, pnb, lane, piv, piv_offset, lPos, lCont, &s_lnS, pPos, pCont,
+                  &s_pnS);
     asm volatile("" ::: "memory");
     const int ln = s_lnS, pn = s_pnS;
     for (int e = lane; e < ln; e += 64) { lpos[e] = lPos[e]; lsrc[e] = lCont[e]; }
@@ -761,6 +769,7 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
       *lperm_n = ln;
       *pperm_n = pn;
       *zero_col = first_zero;
+      if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
     }
   }
 }
@@ -822,11 +831,11 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
 // most 8 remain for the final round.
 int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
                        int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
-                       int row0, int pnb, int* zero_col) {
+                       int row0, int pnb, int* zero_col, int piv_offset, int* fz_global) {
   const int G1 = ceil_div(m, kTslBlock);
   if (G1 <= 1)
     return leaf_lu<float>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
-                          pnb, zero_col);
+                          pnb, zero_col, piv_offset, fz_global);
   if (w > LW || pnb > 256) {
     set_error("tournament leaf: %d rows x %d columns unsupported", m, w);
     return kErrUnsupported;
@@ -846,21 +855,21 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   hipStream_t s = h->stream;
   hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(kTslBlock), 0, s, P, ldp, m, w, nullptr,
                      nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos,
-                     psrc, row0, pnb, zero_col);
+                     psrc, row0, pnb, zero_col, piv_offset, fz_global);
   int G = G1;
   int *cin = candA, *nin = nA, *cout = candB, *nout = nB;
   while (G > kTslFan) {
     const int Gn = ceil_div(G, kTslFan);
     hipLaunchKernelGGL(tsl_kernel<2>, dim3(Gn), dim3(kTslBlock), 0, s, P, ldp, m, w, cin, nin, G,
                        cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
-                       row0, pnb, zero_col);
+                       row0, pnb, zero_col, piv_offset, fz_global);
     G = Gn;
     std::swap(cin, cout);
     std::swap(nin, nout);
   }
   hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(kTslBlock), 0, s, P, ldp, m, w, cin, nin, G,
                      cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
-                     pnb, zero_col);
+                     pnb, zero_col, piv_offset, fz_global);
   if (m > w) {
     if (w == LW)
       hipLaunchKernelGGL(tsl_apply_kernel<true>, dim3(ceil_div(m - w, 256)), dim3(256), 0, s, P,
@@ -874,8 +883,8 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
 }
 
 template int leaf_lu<double>(eamd_handle*, double*, int64_t, int, int, int*, int*, int*, int*,
-                             int*, int*, int*, int, int, int*);
+                             int*, int*, int*, int, int, int*, int, int*);
 template int leaf_lu<float>(eamd_handle*, float*, int64_t, int, int, int*, int*, int*, int*,
-                            int*, int*, int*, int, int, int*);
+                            int*, int*, int*, int, int, int*, int, int*);
 
 }  // namespace eamd

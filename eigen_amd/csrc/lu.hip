@@ -56,7 +56,10 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(T* __restrict__ W, int6
 }
 
 // X <- L^{-1} X, L unit lower nb x nb (row-major, ld ldl), X nb x ncols
-// (row-major).  One workgroup per CB columns; X block resident in LDS.
+// (row-major).  One workgroup per CB columns, the X block resident in LDS.
+// Per 32-row diagonal block: one thread per column solves the block from
+// registers (compile-time indices, L broadcast from LDS, no barrier inside),
+// then all threads update the rows below with 32-long dot products.
 template <typename T, int CB>
 __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, int64_t ldl,
                                                        int nb, T* __restrict__ X, int64_t ldx,
@@ -68,48 +71,46 @@ __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, 
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CB;
   const int cw = min(CB, ncols - c0);
-  for (int e = tid; e < nb * CB; e += 256) {
-    int i = e / CB, c = e - i * CB;
-    Xs[i * XP + c] = c < cw ? X[(int64_t)i * ldx + c0 + c] : T(0);
+  const int nbp = (nb + 31) & ~31;  // rows padded to whole blocks, padding zero
+  for (int e = tid; e < nbp * CB; e += 256) {
+    const int i = e / CB, c = e - i * CB;
+    Xs[i * XP + c] = (c < cw && i < nb) ? X[(int64_t)i * ldx + c0 + c] : T(0);
   }
   for (int b = 0; b < nb; b += 32) {
     const int bs = min(32, nb - b);
-    __syncthreads();
     for (int e = tid; e < 32 * 32; e += 256) {
-      int i = e >> 5, s = e & 31;
-      Ls[i][s] = (i < bs && s < bs) ? L[(int64_t)(b + i) * ldl + b + s] : T(0);
+      const int i = e >> 5, q = e & 31;
+      Ls[i][q] = (i < bs && q < bs) ? L[(int64_t)(b + i) * ldl + b + q] : T(0);
     }
     __syncthreads();
-    // diagonal block, column-oriented substitution
-    for (int s = 0; s < bs - 1; ++s) {
-      for (int e = tid; e < 32 * CB; e += 256) {
-        int i = e / CB, c = e - i * CB;
-        if (i > s && i < bs) Xs[(b + i) * XP + c] -= Ls[i][s] * Xs[(b + s) * XP + c];
+    if (tid < CB) {
+      T x[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) x[i] = Xs[(b + i) * XP + tid];
+#pragma unroll
+      for (int i = 1; i < 32; ++i) {
+        T acc = x[i];
+#pragma unroll
+        for (int q = 0; q < i; ++q) acc -= Ls[i][q] * x[q];
+        x[i] = acc;
       }
-      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 32; ++i) Xs[(b + i) * XP + tid] = x[i];
     }
-    // rows below the block: X[r] -= L[r, b:b+bs] * X[b:b+bs]
-    for (int rb = b + bs; rb < nb; rb += 32) {
-      const int rs = min(32, nb - rb);
-      __syncthreads();
-      for (int e = tid; e < 32 * 32; e += 256) {
-        int i = e >> 5, s = e & 31;
-        Ls[i][s] = (i < rs && s < bs) ? L[(int64_t)(rb + i) * ldl + b + s] : T(0);
-      }
-      __syncthreads();
-      for (int e = tid; e < 32 * CB; e += 256) {
-        int i = e / CB, c = e - i * CB;
-        if (i < rs) {
-          T acc = T(0);
-          for (int s = 0; s < bs; ++s) acc += Ls[i][s] * Xs[(b + s) * XP + c];
-          Xs[(rb + i) * XP + c] -= acc;
-        }
-      }
+    __syncthreads();
+    // rows below the block: X[r] -= L[r, b:b+bs] X[b:b+bs]
+    const int below = nb - b - bs;
+    for (int e = tid; e < below * CB; e += 256) {
+      const int i = e / CB, c = e - i * CB;
+      const T* lrow = L + (int64_t)(b + bs + i) * ldl + b;
+      T acc = T(0);
+      for (int q = 0; q < bs; ++q) acc += lrow[q] * Xs[(b + q) * XP + c];
+      Xs[(b + bs + i) * XP + c] -= acc;
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int e = tid; e < nb * CB; e += 256) {
-    int i = e / CB, c = e - i * CB;
+    const int i = e / CB, c = e - i * CB;
     if (c < cw) X[(int64_t)i * ldx + c0 + c] = Xs[i * XP + c];
   }
 }
@@ -135,14 +136,6 @@ __global__ __launch_bounds__(256) void transpose_kernel(const Ti* __restrict__ A
   }
 }
 
-__global__ void add_offset_kernel(int* v, int n, int off) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] += off;
-}
-
-__global__ void record_zero_kernel(const int* zc, int k, int* first_zero) {
-  if (threadIdx.x == 0 && *zc >= 0 && *first_zero < 0) *first_zero = k + *zc;
-}
 
 }  // namespace
 
@@ -172,7 +165,7 @@ int trsm_llu_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int6
                       int ncols) {
   if (ncols <= 0 || nb <= 0) return kOk;
   constexpr int CB = 32;
-  size_t lds = sizeof(T) * (size_t)nb * (CB + 1);
+  size_t lds = sizeof(T) * (size_t)((nb + 31) & ~31) * (CB + 1);
   auto kern = trsm_llu_kernel<T, CB>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -212,11 +205,12 @@ int init_row_order(hipStream_t s, int* ord, int n);
 int panel_perm_init(hipStream_t s, int* pn, int* ppos, int* psrc, int nb);
 template <typename T>
 int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n, int* lpos,
-            int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col);
+            int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col,
+            int piv_offset, int* fz_global);
 
 int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
                        int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
-                       int row0, int pnb, int* zero_col);
+                       int row0, int pnb, int* zero_col, int piv_offset, int* fz_global);
 int env_int(const char* name, int dflt);
 
 constexpr int kLeaf = 32;
@@ -227,15 +221,15 @@ constexpr int kLeaf = 32;
 template <typename T>
 int panel_leaf(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n,
                int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb,
-               int* zero_col) {
+               int* zero_col, int piv_offset, int* fz_global) {
   if constexpr (std::is_same<T, float>::value) {
     static const bool tournament = env_int("EAMD_TOURNAMENT", 1) != 0;
     if (tournament)
       return leaf_lu_tournament(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
-                                row0, pnb, zero_col);
+                                row0, pnb, zero_col, piv_offset, fz_global);
   }
   return leaf_lu<T>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
-                    zero_col);
+                    zero_col, piv_offset, fz_global);
 }
 
 // Row-major blocked LU driver.  `update(A12, A21, A22, mrows, ncols, k,
@@ -274,12 +268,9 @@ int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int
       for (int j0 = 0; j0 < bs; j0 += kLeaf) {
         const int w = min(kLeaf, bs - j0);
         T* Lk = Pk + (int64_t)j0 * ldw + j0;
+        // transpositions stored with the panel offset, zero pivot recorded
         EAMD_CHECK(panel_leaf<T>(h, Lk, ldw, m - j0, w, piv_dev + k + j0, lperm_n, lpos, lsrc,
-                                 pperm_n, ppos, psrc, j0, bs, zero_col));
-        hipLaunchKernelGGL(record_zero_kernel, dim3(1), dim3(64), 0, s, zero_col, k + j0,
-                           fz_dev);
-        hipLaunchKernelGGL(add_offset_kernel, dim3(1), dim3(64), 0, s, piv_dev + k + j0, w,
-                           k + j0);
+                                 pperm_n, ppos, psrc, j0, bs, zero_col, k + j0, fz_dev));
         // the leaf's interchanges on the other columns of the panel
         EAMD_CHECK(laswp_perm_rowmajor<T>(s, Pk + (int64_t)j0 * ldw, ldw, 0, lperm_n, lpos, lsrc,
                                           w, j0, w, bs - w));
