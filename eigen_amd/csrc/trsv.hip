@@ -7,10 +7,12 @@
 //
 // One workgroup per BS-row block, dispatched in dependency order.  Workgroup
 // b first inverts its diagonal block in LDS (independent of the others, so
-// it overlaps the wait), accumulates B_b - sum_j T_bj X_j as each
-// predecessor's flag appears, then X_b = inv(T_bb) * (...) and releases its
-// own flag.  Only the last predecessor's product is on the critical path.
-// Flags carry a per-call epoch so no clearing pass is needed.
+// it overlaps the wait), accumulates B_b - sum_j T_bj X_j over its
+// predecessors j in order, then X_b = inv(T_bb) * (...).  Solved values are
+// published as 8-byte words carrying 32 bits of payload and the call's
+// epoch (an aligned 8-byte store is single-copy atomic, so a word with the
+// right epoch has the right payload): consumers poll the words themselves,
+// with no flag and no cache write-back or invalidate on the critical path.
 #include "common.h"
 #include "kernels.h"
 
@@ -31,17 +33,28 @@ __device__ __forceinline__ const T* tptr(const T* A, int64_t ld, int r, int c) {
   return ROWMAJOR ? A + (int64_t)r * ld + c : A + r + (int64_t)c * ld;
 }
 
+typedef unsigned long long u64;
+
+template <typename T>
+struct XWords {
+  static constexpr int n = sizeof(T) / 4;  // 32-bit payload words per value
+};
+
 template <typename T, bool ROWMAJOR, bool LOWER, bool UNIT, int BS, int NRC>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restrict__ A,
                                                          int64_t lda, T* __restrict__ X,
                                                          int64_t ldx, int c0, int nrhs,
-                                                         int* __restrict__ flags, int epoch,
+                                                         u64* __restrict__ xw, unsigned epoch,
                                                          int* __restrict__ err) {
-  static_assert(BS == 64, "four threads per row of a 64-row block");
+  static_assert(BS == 64 && NRC == 4, "four threads per row of a 64-row block, 4 RHS");
   constexpr int SEG = BS / 4;  // columns of a predecessor block per thread
+  constexpr int NW = XWords<T>::n;
+  constexpr int CHUNK = BS * NRC * NW;  // words of one block's solution
+  constexpr int WPT = CHUNK / 256;      // words staged per thread
   __shared__ T Dinv[BS][BS + 1];
   __shared__ T Tb[BS][BS + 1];
   __shared__ T accs[BS][NRC];
+  __shared__ unsigned xsw[2][CHUNK];  // staged solution of the current predecessor
   __shared__ int s_abort;
   const int tid = threadIdx.x;
   const int nblk = (n + BS - 1) / BS;
@@ -84,10 +97,11 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
     }
   }
 
-  // ---- off-diagonal part, no workgroup barrier per predecessor: thread
-  // (r, q) accumulates row r over columns q*SEG .. q*SEG+SEG-1 of every
-  // predecessor block, prefetching the next block's matrix values (which do
-  // not depend on X) before waiting on that block's flag.
+  // ---- off-diagonal part: thread (r, q) accumulates row r over columns
+  // q*SEG .. q*SEG+SEG-1 of every predecessor block.  Per predecessor the
+  // workgroup stages its solution from the exchange words into LDS (one
+  // barrier); the matrix values, which do not depend on X, are prefetched
+  // one predecessor ahead.
   T acc[NRC];
 #pragma unroll
   for (int c = 0; c < NRC; ++c) acc[c] = T(0);
@@ -103,29 +117,53 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
   T tcur[SEG], tnext[SEG];
   if (ndep > 0) load_seg(0, tcur);
   int bad = 0;
-  for (int d = 0; d < ndep; ++d) {
+  for (int d = 0; d < ndep && !bad; ++d) {
     if (d + 1 < ndep) load_seg(d + 1, tnext);
     const int jb = LOWER ? d : nblk - 1 - d;
-    int spins = 0;
-    while (__hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-      if ((++spins & 255) == 0 &&
-          (spins > kSpin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-        bad = 1;
-        break;
+    const int par = d & 1;
+    // my words of the predecessor's solution: row jr, column c, half h
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+      const int widx = tid + k * 256;
+      const int row = jb * BS + widx / (NRC * NW);
+      const int c = (widx / NW) % NRC;
+      unsigned payload = 0;
+      if (row < n && c < nc) {
+        const u64* src = xw + (size_t)row * NRC * NW + (widx % (NRC * NW));
+        int spins = 0;
+        for (;;) {
+          const u64 wv = __hip_atomic_load(const_cast<u64*>(src), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+          if ((unsigned)(wv >> 32) == epoch) { payload = (unsigned)wv; break; }
+          if ((++spins & 255) == 0 &&
+              (spins > kSpin ||
+               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            bad = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
       }
-      __builtin_amdgcn_s_sleep(1);
+      xsw[par][widx] = payload;
     }
-    if (bad) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (bad) s_abort = 1;
+    __syncthreads();
+    if (s_abort) { bad = 1; break; }
     const int cb = jb * BS + q * SEG;
 #pragma unroll
     for (int c = 0; c < NRC; ++c) {
       if (c < nc) {
-        const T* xc = X + (int64_t)(c0 + c) * ldx + cb;
         T sum = T(0);
 #pragma unroll
-        for (int i = 0; i < SEG; ++i)
-          if (cb + i < n) sum += tcur[i] * tload(xc + i);
+        for (int i = 0; i < SEG; ++i) {
+          const unsigned* wp = &xsw[par][((q * SEG + i) * NRC + c) * NW];
+          T xv;
+          if (NW == 2)
+            xv = (T)__longlong_as_double((long long)(((u64)wp[1] << 32) | wp[0]));
+          else
+            xv = (T)__uint_as_float(wp[0]);
+          if (cb + i < n) sum += tcur[i] * xv;
+        }
         acc[c] += sum;
       }
     }
@@ -163,17 +201,25 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
     xs[c] += __shfl_xor(xs[c], 1);
     xs[c] += __shfl_xor(xs[c], 2);
   }
+  // publish: the result, and its exchange words for the successors
   if (q == 0 && rvalid) {
 #pragma unroll
     for (int c = 0; c < NRC; ++c)
-      if (c < nc)
-        __hip_atomic_store(&X[(int64_t)(c0 + c) * ldx + rb0 + r], xs[c], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      if (c < nc) {
+        X[(int64_t)(c0 + c) * ldx + rb0 + r] = xs[c];
+        u64* dst = xw + ((size_t)(rb0 + r) * NRC + c) * NW;
+        if (NW == 2) {
+          const u64 bits = (u64)__double_as_longlong((double)xs[c]);
+          __hip_atomic_store(dst, ((u64)epoch << 32) | (unsigned)bits, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(dst + 1, ((u64)epoch << 32) | (unsigned)(bits >> 32),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          __hip_atomic_store(dst, ((u64)epoch << 32) | __float_as_uint((float)xs[c]),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (tid == 0)
-    __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -186,24 +232,26 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
   if (n <= 0 || nrhs <= 0) return kOk;
   constexpr int BS = 64, NRC = 4;
   const int nblk = ceil_div(n, BS);
-  // flags[nblk] + err word; reuse the status slot
-  size_t need = sizeof(int) * ((size_t)nblk + 64);
+  // error word | exchange words (NRC values per row, 32 payload bits each)
+  const size_t nwords = (size_t)nblk * BS * NRC * (sizeof(T) / 4);
+  size_t need = 256 + sizeof(unsigned long long) * nwords;
   void* base = nullptr;
   bool fresh = h->ws.bytes[kWsStatus] < need;
   EAMD_CHECK(h->ws.get(kWsStatus, need, &base));
   int* err = static_cast<int*>(base);
-  int* flags = err + 64;
+  unsigned long long* xw =
+      reinterpret_cast<unsigned long long*>(static_cast<char*>(base) + 256);
   if (fresh || h->trsv_epoch > (1 << 30)) {
-    EAMD_HIP(hipMemsetAsync(base, 0, need, h->stream));
+    EAMD_HIP(hipMemsetAsync(base, 0, h->ws.bytes[kWsStatus], h->stream));
     h->trsv_epoch = 0;
   }
   PhaseScope ps(h, kPhTrsv, sizeof(T) * (0.5 * (double)n * n + 2.0 * (double)n * nrhs),
                 (double)n * n * nrhs);
   for (int c0 = 0; c0 < nrhs; c0 += NRC) {
-    int epoch = ++h->trsv_epoch;
+    const unsigned epoch = (unsigned)++h->trsv_epoch;
 #define EAMD_TRSV(RM, LO, UN)                                                                   \
   hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC>), dim3(nblk), dim3(256), 0,     \
-                     h->stream, n, A, lda, X, ldx, c0, nrhs, flags, epoch, err)
+                     h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV(true, true, true); else EAMD_TRSV(true, true, false); }
       else { if (unit) EAMD_TRSV(true, false, true); else EAMD_TRSV(true, false, false); }
