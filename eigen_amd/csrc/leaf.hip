@@ -179,6 +179,28 @@ __device__ __forceinline__ double wave_max(double s) {
   return fmax(fmax(readlane_d(s, 0), readlane_d(s, 16)), fmax(readlane_d(s, 32), readlane_d(s, 48)));
 }
 
+// Maximum of a 64-bit key over the 64 lanes of a wave (same lane pattern).
+template <int CTRL>
+__device__ __forceinline__ u64 dpp_u64(u64 x) {
+  const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xf, 0xf, false);
+  return ((u64)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ u64 readlane_u64(u64 x, int lane) {
+  const int lo = __builtin_amdgcn_readlane((int)x, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+  return ((u64)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ u64 umax64(u64 a, u64 b) { return a > b ? a : b; }
+__device__ __forceinline__ u64 wave_max_u64(u64 k) {
+  k = umax64(k, dpp_u64<0xB1>(k));
+  k = umax64(k, dpp_u64<0x4E>(k));
+  k = umax64(k, dpp_u64<0x141>(k));
+  k = umax64(k, dpp_u64<0x140>(k));
+  return umax64(umax64(readlane_u64(k, 0), readlane_u64(k, 16)),
+                umax64(readlane_u64(k, 32), readlane_u64(k, 48)));
+}
+
 // Slot (words): [0..1] candidate value (FP64 bits), [2] row or ~0u (no
 // candidate), [3] unused, [4, 4 + LW*Words) candidate row.  The slot after
 // the last workgroup's holds row j (from word 4).
@@ -257,12 +279,14 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   constexpr int NWAVE = BLOCK / 64;
   constexpr int STAGE = (kMaxG + 1) * SW;
   constexpr int KMAX = (STAGE + BLOCK - 1) / BLOCK;  // words polled per thread
-  __shared__ double red_s[NWAVE];
-  __shared__ int red_i[NWAVE];
-  __shared__ double red_v[NWAVE];
+  // per column, double-buffered by parity (one barrier per column): each
+  // wave's best score, row index, value and row, and row j when owned here
+  __shared__ double red_s[2][NWAVE];
+  __shared__ int red_i[2][NWAVE];
+  __shared__ double red_v[2][NWAVE];
+  __shared__ T candw[2][NWAVE][LW];
   __shared__ T urow[2][LW];   // pivot row received from the exchange
   __shared__ T rowj[2][LW];   // old row j received from the exchange
-  __shared__ T cand[2][LW];   // this workgroup's candidate row
   __shared__ T rowjL[2][LW];  // row j, when this workgroup owns it
   __shared__ unsigned stage[STAGE];
   __shared__ int s_abort;
@@ -311,35 +335,33 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     if (valid && i >= j) sc = score_of((double)vj, i == j);
     const double wmax = wave_max(sc);
     const int wl = __ffsll((long long)__ballot(sc == wmax)) - 1;
-    const double wval = readlane_d((double)vj, wl);
-    if (lane == 0) {
-      red_s[wave] = wmax;
-      red_i[wave] = wmax > -2.0 ? g * BLOCK + wave * 64 + wl : 0x7fffffff;
-      red_v[wave] = wval;
-    }
-    __syncthreads();
-    double best = red_s[0];
-    int bq = 0;
+    // the wave's candidate lane also stores its row, and the owner of row j
+    // stores that row, before the column's single barrier
+    if (lane == wl) {
+      red_s[par][wave] = wmax;
+      red_i[par][wave] = wmax > -2.0 ? g * BLOCK + wave * 64 + wl : 0x7fffffff;
+      red_v[par][wave] = (double)vj;
 #pragma unroll
-    for (int q = 1; q < NWAVE; ++q) {
-      const double s2 = red_s[q];
-      if (s2 > best) { best = s2; bq = q; }
-    }
-    const int bi = red_i[bq];
-    const double bval = red_v[bq];
-    // ---- 2. candidate row and row j into LDS (the rows stay in registers)
-    const bool none = bi == 0x7fffffff;
-    if (i == bi) {
-#pragma unroll
-      for (int c = 0; c < LW; ++c) cand[par][c] = v[c];
+      for (int c = 0; c < LW; ++c) candw[par][wave][c] = v[c];
     }
     if (i == j) {
 #pragma unroll
       for (int c = 0; c < LW; ++c) rowjL[par][c] = v[c];
     }
     __syncthreads();
-    const T* up = cand[par];   // pivot row
-    const T* rp = rowjL[par];  // old row j
+    double best = red_s[par][0];
+    int bq = 0;
+#pragma unroll
+    for (int q = 1; q < NWAVE; ++q) {
+      const double s2 = red_s[par][q];
+      if (s2 > best) { best = s2; bq = q; }
+    }
+    const int bi = red_i[par][bq];
+    const double bval = red_v[par][bq];
+    // ---- 2. the workgroup's candidate row is the winning wave's
+    const bool none = bi == 0x7fffffff;
+    const T* up = candw[par][bq];  // pivot row
+    const T* rp = rowjL[par];      // old row j
     int p = bi;
     if (G > 1) {
       // ---- 3. publish: header, candidate row and (owner) row j, one word per lane
@@ -353,7 +375,7 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
         put_word(myslot + 3, 0u, epoch);
       } else if (tid >= 64 && tid < 64 + LW * NW) {
         const int q = tid - 64, c = q / NW, half = q - c * NW;
-        const T val = none ? T(0) : cand[par][c];
+        const T val = none ? T(0) : up[c];
         unsigned pl;
         if (sizeof(T) == 8) {
           u64 bits = (u64)__double_as_longlong((double)val);
@@ -449,19 +471,24 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     }
     if (g == 0 && tid == 0) s_piv[j] = p;
     // ---- 6. scale and rank-1 update of the rows below j
+    // branch-free (a branch around the register row makes the compiler copy
+    // the whole row at the join); l = 0 leaves a row unchanged
     const T pv = up[jj];
-    if (pv != T(0)) {
-      if (valid && i > j) {
-        const T l = (i == p ? rp[jj] : vj) / pv;
-        v[jj] = l;
+    if (pv == T(0) && first_zero < 0) first_zero = j;
+    const bool act = pv != T(0) && valid && i > j;
+    const T qv = (i == p ? rp[jj] : vj) / pv;
+    T l = act ? qv : T(0);
+    asm volatile("" : "+v"(l));  // keep the update unconditional
+    v[jj] = act ? l : v[jj];
+    const unsigned lm = 0u - (unsigned)(live > 8), lm16 = 0u - (unsigned)(live > 16),
+                   lm24 = 0u - (unsigned)(live > 24);
 #pragma unroll
-        for (int c = jj + 1; c < LW; ++c) {
-          const T u = (c < 8 || c < live) ? up[c] : T(0);
-          v[c] -= l * u;
-        }
-      }
-    } else if (first_zero < 0) {
-      first_zero = j;
+    for (int c = jj + 1; c < LW; ++c) {
+      // slot c is live iff c < live; an integer mask on the loaded value keeps
+      // the LDS reads unconditional
+      const unsigned m = c < 8 ? ~0u : (c < 16 ? lm : (c < 24 ? lm16 : lm24));
+      const T u = mask_sel(m, up[c], T(0));
+      v[c] -= l * u;
     }
     return true;
   };
@@ -590,11 +617,11 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
     int* __restrict__ fz_global) {
   constexpr int BLOCK = kTslBlock, NWAVE = BLOCK / 64;
-  __shared__ double red_s[NWAVE];
-  __shared__ int red_i[NWAVE];
-  __shared__ float prow[2][LW];  // pivot row of the step
-  __shared__ float jrow[2][LW];  // row at position j before the interchange
-  __shared__ int porig[2][2];    // their original (leaf-relative) rows
+  // per step (double-buffered by parity, one barrier per step): each wave's
+  // best key, its row and that row's original index
+  __shared__ u64 red_k[2][NWAVE];
+  __shared__ float candw[2][NWAVE][LW];
+  __shared__ int cand_orig[2][NWAVE];
   __shared__ int s_sel[LW];
   __shared__ int s_off[kTslFan + 1];
   __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
@@ -639,6 +666,11 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
   const int steps = min(w, nrows);
   const int ngroups = (steps + 7) / 8;
   int first_zero = -1;
+  // Rows never move between threads: an interchange only relabels
+  // positions (pos), so a step costs one reduction and one barrier.  The
+  // reduction key orders by score, then by lower position (the first
+  // maximum in row order); a NaN scores only from position j, as infinity.
+  int pos = valid ? tid : 0x7fffffff;
   // groups of 8 columns in slots 0..7 (see rotate8); slots [live, 32) hold
   // columns finished in earlier groups and take no update
 #pragma unroll 1
@@ -650,61 +682,53 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
       if (j >= steps) break;
       const int par = j & 1;
       const float vj = v[jj];
-      double sc = -2.0;
-      if (valid && tid >= j) sc = score_of((double)vj, tid == j);
-      const double wmax = wave_max(sc);
-      const int wl = __ffsll((long long)__ballot(sc == wmax)) - 1;
-      if (lane == 0) {
-        red_s[wave] = wmax;
-        red_i[wave] = wmax > -2.0 ? wave * 64 + wl : 0x7fffffff;
+      u64 key = 0;
+      if (pos >= j && pos != 0x7fffffff) {
+        float a = fabsf(vj);
+        if (a != a) a = pos == j ? INFINITY : -1.0f;
+        if (a >= 0.0f) key = ((u64)__float_as_uint(a) << 32) | (0xffffffffu - (unsigned)pos);
+      }
+      const u64 wk = wave_max_u64(key);
+      const int wl = __ffsll((long long)__ballot(key == wk)) - 1;
+      if (lane == wl) {
+#pragma unroll
+        for (int c = 0; c < LW; ++c) candw[par][wave][c] = v[c];
+        cand_orig[par][wave] = orig;
+        red_k[par][wave] = wk;
       }
       __syncthreads();
-      double best = red_s[0];
+      u64 best = red_k[par][0];
       int bq = 0;
 #pragma unroll
       for (int q = 1; q < NWAVE; ++q) {
-        const double s2 = red_s[q];
-        if (s2 > best) { best = s2; bq = q; }
+        const u64 k2 = red_k[par][q];
+        if (k2 > best) { best = k2; bq = q; }
       }
-      const int bi = red_i[bq];  // position j holds a valid row, so bi is set
-      if (tid == bi) {
-#pragma unroll
-        for (int c = 0; c < LW; ++c) prow[par][c] = v[c];
-        porig[par][0] = orig;
-      }
-      if (tid == j) {
-#pragma unroll
-        for (int c = 0; c < LW; ++c) jrow[par][c] = v[c];
-        porig[par][1] = orig;
-      }
-      __syncthreads();
-      const float* up = prow[par];
-      const float* rp = jrow[par];
-      if (bi != j) {
-        if (tid == j) {
-#pragma unroll
-          for (int c = 0; c < LW; ++c) v[c] = up[c];
-          orig = porig[par][0];
-        } else if (tid == bi) {
-#pragma unroll
-          for (int c = 0; c < LW; ++c) v[c] = rp[c];
-          orig = porig[par][1];
-        }
-      }
-      if (tid == 0) s_sel[j] = porig[par][0];
+      // position j holds a valid row, so the winner is set
+      const int bpos = (int)(0xffffffffu - (unsigned)best);
+      const float* up = candw[par][bq];
+      if (tid == 0) s_sel[j] = cand_orig[par][bq];
+      // interchange positions j and bpos
+      if (pos == bpos) pos = j;
+      else if (pos == j) pos = bpos;
+      // branch-free (a branch around the register row makes the compiler
+      // copy the whole row at the join)
       const float pv = up[jj];
-      if (pv != 0.f) {
-        if (valid && tid > j) {
-          const float l = (tid == bi ? rp[jj] : vj) / pv;
-          v[jj] = l;
+      if (pv == 0.f && first_zero < 0) first_zero = j;
+      const bool act = pv != 0.f && pos > j && pos != 0x7fffffff;
+      const float qv = vj / pv;
+      float l = act ? qv : 0.f;
+      asm volatile("" : "+v"(l));  // keep the update unconditional
+      v[jj] = act ? l : v[jj];
+      const unsigned lm = 0u - (unsigned)(live > 8), lm16 = 0u - (unsigned)(live > 16),
+                     lm24 = 0u - (unsigned)(live > 24);
 #pragma unroll
-          for (int c = jj + 1; c < LW; ++c) {
-            const float u = (c < 8 || c < live) ? up[c] : 0.f;
-            v[c] -= l * u;
-          }
-        }
-      } else if (first_zero < 0) {
-        first_zero = j;
+      for (int c = jj + 1; c < LW; ++c) {
+        // slot c is live iff c < live; an integer mask on the loaded value
+        // keeps the LDS reads unconditional
+        const unsigned m = c < 8 ? ~0u : (c < 16 ? lm : (c < 24 ? lm16 : lm24));
+        const float u = mask_sel(m, up[c], 0.f);
+        v[c] -= l * u;
       }
     }
     rotate8(v);
@@ -718,8 +742,8 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     return;
   }
   // ---- final round: the top block in place (rows 0..w-1)
-  if (tid < steps) {
-    float* dst = P + (int64_t)tid * ldp;
+  if (pos < steps) {
+    float* dst = P + (int64_t)pos * ldp;
 #pragma unroll
     for (int c = 0; c < LW; ++c)
       if (c < w) dst[c] = v[c];
