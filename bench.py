@@ -103,6 +103,20 @@ def cpu_baseline(n=4096):
                       f"{dt:.2f} s (HPL flop count)"}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel, as recorded from the
+    FETCH_SIZE / WRITE_SIZE counter runs (profiles/, each counter scaled by
+    what it reads for a copy of known size); None when not recorded."""
+    import re
+    path = os.path.join(ROOT, "profiles", "mixed_update_kernel.n16384.traffic.md")
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def _median_ms(fn, reps, warm=2):
     import statistics
 
@@ -282,7 +296,8 @@ def main():
     ach_gbs = upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(ach_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(ach_gbs / PEAK_HBM_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic() if args.n == 16384 and
+            args.precision == "fp16x3" else None,
             "kernel": f"mixed_update_kernel ({args.precision})",
             "launches": int(upd_cnt),
             "avg_launch_us": round(upd_ms * 1e3 / max(1, upd_cnt), 2),
