@@ -5,7 +5,7 @@
 //   potf2_kernel   the nb x nb diagonal block is factored by one workgroup in
 //                  LDS (x <= 0 fails at that column, NaN passes, as in the
 //                  reference's `if (x <= RealScalar(0)) return k`);
-//   trsm_rlt_kernel A21 <- A21 * L11^{-T}, one workgroup per 64 rows;
+//   trsm_rlt_kernel A21 <- A21 * L11^{-T}, one workgroup per 32 rows;
 //   dgemm (lower)  A22 -= A21 * A21^T on MFMA, tiles above the diagonal
 //                  skipped (the SYRK "bottleneck" of LLT.h:357).
 // A failure is recorded on the device; later kernels of the same call see
@@ -19,50 +19,97 @@ namespace {
 
 constexpr int kMaxNb = 128;
 
-// Factor the bs x bs lower block at A (column-major, lda).  fail[0] keeps
-// the first failing absolute column (k0 + local) or -1.
+// Factor the bs x bs (bs <= 128) lower block at A (column-major, lda) in
+// LDS, in 32-column sub-blocks: wave 0 factors the 32 x 32 diagonal
+// sub-block with one row per lane in registers (broadcasts by readlane, no
+// barrier), the rows below solve against it one row per thread, and all
+// threads update the trailing lower part.  fail[0] keeps the first failing
+// absolute column (k0 + local) or -1: a diagonal x <= 0 fails, a NaN passes,
+// as in the reference's `if (x <= RealScalar(0)) return k` (LLT.h:316).
 __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int64_t lda, int bs,
                                                     int k0, int* __restrict__ fail) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* S = reinterpret_cast<double*>(smem_raw);  // bs x (bs+1), column-major
-  const int ld = bs + 1;
+  double* S = reinterpret_cast<double*>(smem_raw);  // bsp x (bsp+1), column-major, padded
+  const int bsp = (bs + 31) & ~31;
+  const int ld = bsp + 1;
   __shared__ int s_fail;
-  __shared__ double s_d;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (*fail >= 0) return;
-  for (int e = tid; e < bs * bs; e += 256) {
-    int j = e / bs, i = e - j * bs;
-    S[i + j * ld] = (i >= j) ? A[i + (int64_t)j * lda] : 0.0;
+  // padding: identity on the diagonal, zero elsewhere (factors to itself)
+  for (int e = tid; e < bsp * bsp; e += 256) {
+    const int j = e / bsp, i = e - j * bsp;
+    double v = 0.0;
+    if (i < bs && j < bs) v = (i >= j) ? A[i + (int64_t)j * lda] : 0.0;
+    else if (i == j) v = 1.0;
+    S[i + j * ld] = v;
   }
   if (tid == 0) s_fail = -1;
   __syncthreads();
-  for (int k = 0; k < bs; ++k) {
-    if (tid == 0) {
-      double x = S[k + k * ld];
-      if (x <= 0.0) {
-        s_fail = k;
-      } else {
-        x = sqrt(x);
-        S[k + k * ld] = x;
-        s_d = x;
+  for (int kb = 0; kb < bsp; kb += 32) {
+    // ---- 32 x 32 diagonal sub-block on wave 0, row kb+lane in x[]
+    if (wave == 0) {
+      double x[32];
+      const int i = kb + (lane & 31);
+#pragma unroll
+      for (int c = 0; c < 32; ++c) x[c] = S[i + (kb + c) * ld];
+      int f = -1;
+#pragma unroll
+      for (int c = 0; c < 32; ++c) {
+        const double diag = __longlong_as_double(
+            ((long long)__builtin_amdgcn_readlane((int)(__double_as_longlong(x[c]) >> 32), c) << 32) |
+            (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(x[c]), c));
+        if (f < 0 && diag <= 0.0) f = c;
+        const double d = sqrt(diag);
+        x[c] = (lane & 31) == c ? d : ((lane & 31) > c ? x[c] / d : x[c]);
+#pragma unroll
+        for (int j = c + 1; j < 32; ++j) {
+          const long long bits = __double_as_longlong(x[c]);
+          const double ljc = __longlong_as_double(
+              ((long long)__builtin_amdgcn_readlane((int)(bits >> 32), j) << 32) |
+              (unsigned)__builtin_amdgcn_readlane((int)bits, j));
+          if ((lane & 31) >= j) x[j] -= x[c] * ljc;
+        }
       }
+      if (lane < 32) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+          if (c <= lane) S[i + (kb + c) * ld] = x[c];
+      }
+      if (lane == 0 && f >= 0) s_fail = kb + f;
     }
     __syncthreads();
     if (s_fail >= 0) break;
-    const double d = s_d;
-    for (int i = k + 1 + tid; i < bs; i += 256) S[i + k * ld] /= d;
+    const int below = bsp - kb - 32;
+    if (below <= 0) break;
+    // ---- rows below: y L11^T = s, one row per thread
+    if (tid < below) {
+      const int i = kb + 32 + tid;
+      double y[32];
+#pragma unroll
+      for (int c = 0; c < 32; ++c) y[c] = S[i + (kb + c) * ld];
+#pragma unroll
+      for (int c = 0; c < 32; ++c) {
+        double v = y[c];
+#pragma unroll
+        for (int q = 0; q < c; ++q) v -= y[q] * S[kb + c + (kb + q) * ld];
+        y[c] = v / S[kb + c + (kb + c) * ld];
+      }
+#pragma unroll
+      for (int c = 0; c < 32; ++c) S[i + (kb + c) * ld] = y[c];
+    }
     __syncthreads();
-    // trailing lower update: S[i][j] -= S[i][k] * S[j][k], k < j <= i
-    const int rs = bs - k - 1;
-    const int tri = rs * (rs + 1) / 2;
-    for (int e = tid; e < rs * rs; e += 256) {
-      int jj = e / rs, ii = e - jj * rs;
+    // ---- trailing lower update: S[i][j] -= sum_q S[i][kb+q] S[j][kb+q]
+    const int t0 = kb + 32;
+    for (int e = tid; e < below * below; e += 256) {
+      const int jj = e / below, ii = e - jj * below;
       if (ii >= jj) {
-        int i = k + 1 + ii, j = k + 1 + jj;
-        S[i + j * ld] -= S[i + k * ld] * S[j + k * ld];
+        const int i = t0 + ii, j = t0 + jj;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < 32; ++q) acc += S[i + (kb + q) * ld] * S[j + (kb + q) * ld];
+        S[i + j * ld] -= acc;
       }
     }
-    (void)tri;
     __syncthreads();
   }
   if (s_fail >= 0) {
@@ -70,70 +117,72 @@ __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int6
     return;
   }
   for (int e = tid; e < bs * bs; e += 256) {
-    int j = e / bs, i = e - j * bs;
+    const int j = e / bs, i = e - j * bs;
     if (i >= j) A[i + (int64_t)j * lda] = S[i + j * ld];
   }
 }
 
-// X (m x bs, column-major) <- X * L^{-T}, L lower bs x bs (column-major).
-// Row r of X solves L y = x_r^T.  One workgroup per RB rows; X rows in LDS,
-// L staged in 32x32 tiles.
-template <int RB>
+// X (m x bs, column-major) <- X * L^{-T}, L lower bs x bs (column-major):
+// row r of X solves y L^T = x_r.  One workgroup per 32 rows, X block in LDS;
+// per 32-column block of L: one thread per row solves the block from
+// registers (L broadcast from LDS), then 8 threads per row update the later
+// columns.
 __global__ __launch_bounds__(256) void trsm_rlt_kernel(const double* __restrict__ L, int64_t ldl,
                                                        int bs, double* __restrict__ X,
                                                        int64_t ldx, int m,
                                                        const int* __restrict__ fail) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* Xs = reinterpret_cast<double*>(smem_raw);  // RB x (bs+1)... stored [col][row]
-  __shared__ double Ls[32][33];
+  constexpr int RB = 32;
+  const int bsp = (bs + 31) & ~31;
+  double* Xs = reinterpret_cast<double*>(smem_raw);  // [col][row], bsp x (RB+1)
+  double* Lp = Xs + (size_t)bsp * (RB + 1);          // [row j][q], bsp x 33
   const int tid = threadIdx.x;
   if (*fail >= 0) return;
   const int r0 = blockIdx.x * RB;
   const int rr = min(RB, m - r0);
-  const int ld = RB + 1;
-  for (int e = tid; e < RB * bs; e += 256) {
-    int c = e / RB, r = e - c * RB;
-    Xs[r + c * ld] = r < rr ? X[(r0 + r) + (int64_t)c * ldx] : 0.0;
+  const int r = tid & (RB - 1), qd = tid >> 5;  // 8 threads per row
+  for (int e = tid; e < RB * bsp; e += 256) {
+    const int c = e / RB, rw = e - c * RB;
+    Xs[rw + c * (RB + 1)] = (rw < rr && c < bs) ? X[(r0 + rw) + (int64_t)c * ldx] : 0.0;
   }
-  for (int b = 0; b < bs; b += 32) {
+  for (int b = 0; b < bsp; b += 32) {
     const int w = min(32, bs - b);
+    const int rest = bsp - b;  // rows j of the L panel (b .. bsp-1)
     __syncthreads();
-    for (int e = tid; e < 32 * 32; e += 256) {
-      int i = e & 31, j = e >> 5;  // Ls[i][j] = L(b+i, b+j)
-      Ls[i][j] = (i < w && j < w) ? L[(b + i) + (int64_t)(b + j) * ldl] : 0.0;
+    for (int e = tid; e < rest * 32; e += 256) {
+      const int j = e >> 5, q = e & 31;  // Lp[j][q] = L(b + j, b + q); padding: identity
+      double v = 0.0;
+      if (b + j < bs && q < w) v = L[(b + j) + (int64_t)(b + q) * ldl];
+      else if (j == q) v = 1.0;
+      Lp[j * 33 + q] = v;
     }
     __syncthreads();
-    // diagonal block: y_c = (x_c - sum_{q<c} y_q L(c,q)) / L(c,c), column by column
-    for (int c = 0; c < w; ++c) {
-      for (int r = tid; r < RB; r += 256) {
-        double v = Xs[r + (b + c) * ld];
-        for (int q = 0; q < c; ++q) v -= Xs[r + (b + q) * ld] * Ls[c][q];
-        Xs[r + (b + c) * ld] = v / Ls[c][c];
+    if (qd == 0) {
+      double y[32];
+#pragma unroll
+      for (int c = 0; c < 32; ++c) y[c] = Xs[r + (b + c) * (RB + 1)];
+#pragma unroll
+      for (int c = 0; c < 32; ++c) {
+        double v = y[c];
+#pragma unroll
+        for (int q = 0; q < c; ++q) v -= y[q] * Lp[c * 33 + q];
+        y[c] = v / Lp[c * 33 + c];
       }
+#pragma unroll
+      for (int c = 0; c < 32; ++c) Xs[r + (b + c) * (RB + 1)] = y[c];
     }
-    // later columns: x_j -= sum_{q in block} y_q L(j, q)
-    for (int jb = b + w; jb < bs; jb += 32) {
-      const int w2 = min(32, bs - jb);
-      __syncthreads();
-      for (int e = tid; e < 32 * 32; e += 256) {
-        int i = e & 31, j = e >> 5;  // Ls[i][j] = L(jb+i, b+j)
-        Ls[i][j] = (i < w2 && j < w) ? L[(jb + i) + (int64_t)(b + j) * ldl] : 0.0;
-      }
-      __syncthreads();
-      for (int e = tid; e < RB * 32; e += 256) {
-        int r = e % RB, i = e / RB;
-        if (i < w2) {
-          double acc = 0.0;
-          for (int q = 0; q < w; ++q) acc += Xs[r + (b + q) * ld] * Ls[i][q];
-          Xs[r + (jb + i) * ld] -= acc;
-        }
-      }
+    __syncthreads();
+    for (int j = 32 + qd; j < rest; j += 8) {
+      double acc = 0.0;
+#pragma unroll 8
+      for (int q = 0; q < 32; ++q) acc += Xs[r + (b + q) * (RB + 1)] * Lp[j * 33 + q];
+      Xs[r + (b + j) * (RB + 1)] -= acc;
     }
   }
   __syncthreads();
   for (int e = tid; e < RB * bs; e += 256) {
-    int c = e / RB, r = e - c * RB;
-    if (r < rr) X[(r0 + r) + (int64_t)c * ldx] = Xs[r + c * ld];
+    const int c = e / RB, rw = e - c * RB;
+    if (rw < rr) X[(r0 + rw) + (int64_t)c * ldx] = Xs[rw + c * (RB + 1)];
   }
 }
 
@@ -155,7 +204,7 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
   if (!attr) {
     EAMD_HIP(hipFuncSetAttribute((const void*)potf2_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    EAMD_HIP(hipFuncSetAttribute((const void*)trsm_rlt_kernel<64>,
+    EAMD_HIP(hipFuncSetAttribute((const void*)trsm_rlt_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attr = true;
   }
@@ -163,13 +212,14 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
     const int bs = min(nb, n - k);
     const int rs = n - k - bs;
     double* A11 = A + k + (int64_t)k * lda;
-    size_t lds = sizeof(double) * (size_t)bs * (bs + 1);
+    const int bsp = (bs + 31) & ~31;
+    size_t lds = sizeof(double) * (size_t)bsp * (bsp + 1);
     hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(256), lds, s, A11, lda, bs, k, fail);
     if (rs > 0) {
       double* A21 = A11 + bs;
-      size_t lds2 = sizeof(double) * (size_t)(64 + 1) * bs;
-      hipLaunchKernelGGL(trsm_rlt_kernel<64>, dim3(ceil_div(rs, 64)), dim3(256), lds2, s, A11,
-                         lda, bs, A21, lda, rs, fail);
+      size_t lds2 = sizeof(double) * (size_t)bsp * (32 + 1) * 2;
+      hipLaunchKernelGGL(trsm_rlt_kernel, dim3(ceil_div(rs, 32)), dim3(256), lds2, s, A11, lda,
+                         bs, A21, lda, rs, fail);
       double* A22 = A21 + (int64_t)bs * lda;
       EAMD_CHECK(dgemm_device(s, false, true, rs, rs, bs, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
                               1, fail));
