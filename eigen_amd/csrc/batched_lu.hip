@@ -284,39 +284,38 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
         Pn[r * (PB + 1) + c] = c < pb ? W[(int64_t)(k0 + r) * n + k0 + c] : TF(0);
       }
       __syncthreads();
-      for (int c = 0; c < pb; ++c) {
-        double sc = -1.0;
-        int ri = 0x7fffffff;
-        for (int r = c + tid; r < R; r += NT) {
-          const double v = score_of((double)Pn[r * (PB + 1) + c], r == c);
-          if (v > sc) { sc = v; ri = r; }
-        }
-        const double M = block_max<NW>(sc, red_d, lane, wave);
-        const int wmin = wave_min_i(sc == M ? ri : 0x7fffffff);
-        if (lane == 0) red_i[wave] = wmin;
-        __syncthreads();
-        int p = red_i[0];
-#pragma unroll
-        for (int q = 1; q < NW; ++q) p = min(p, red_i[q]);
-        const TF pv = Pn[p * (PB + 1) + c];
-        if (tid == 0) piv[k0 + c] = k0 + p;
-        __syncthreads();
-        if (p != c && tid < PB) {
-          const TF t = Pn[c * (PB + 1) + tid];
-          Pn[c * (PB + 1) + tid] = Pn[p * (PB + 1) + tid];
-          Pn[p * (PB + 1) + tid] = t;
-        }
-        __syncthreads();
-        if (pv != TF(0)) {
-          for (int r = c + 1 + tid; r < R; r += NT) {
-            const TF l = Pn[r * (PB + 1) + c] / pv;
-            Pn[r * (PB + 1) + c] = l;
-            for (int cc = c + 1; cc < pb; ++cc)
-              Pn[r * (PB + 1) + cc] -= l * Pn[c * (PB + 1) + cc];
+      // the panel is factored by wave 0 alone: wave reductions and the
+      // wave's in-order LDS accesses need no workgroup barrier per column
+      if (wave == 0) {
+        for (int c = 0; c < pb; ++c) {
+          double sc = -1.0;
+          int ri = 0x7fffffff;
+          for (int r = c + lane; r < R; r += 64) {
+            const double v = score_of((double)Pn[r * (PB + 1) + c], r == c);
+            if (v > sc) { sc = v; ri = r; }
           }
+          const double M = wave_max(sc);
+          const int p = wave_min_i(sc == M ? ri : 0x7fffffff);
+          const TF pv = Pn[p * (PB + 1) + c];
+          if (lane == 0) piv[k0 + c] = k0 + p;
+          if (p != c && lane < PB) {
+            const TF t = Pn[c * (PB + 1) + lane];
+            Pn[c * (PB + 1) + lane] = Pn[p * (PB + 1) + lane];
+            Pn[p * (PB + 1) + lane] = t;
+          }
+          asm volatile("" ::: "memory");
+          if (pv != TF(0)) {
+            for (int r = c + 1 + lane; r < R; r += 64) {
+              const TF l = Pn[r * (PB + 1) + c] / pv;
+              Pn[r * (PB + 1) + c] = l;
+              for (int cc = c + 1; cc < pb; ++cc)
+                Pn[r * (PB + 1) + cc] -= l * Pn[c * (PB + 1) + cc];
+            }
+          }
+          asm volatile("" ::: "memory");
         }
-        __syncthreads();
       }
+      __syncthreads();
       // panel back to W
       for (int e = tid; e < R * pb; e += NT) {
         const int r = e / pb, c = e - r * pb;

@@ -98,14 +98,22 @@ __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, 
       for (int i = 0; i < 32; ++i) Xs[(b + i) * XP + tid] = x[i];
     }
     __syncthreads();
-    // rows below the block: X[r] -= L[r, b:b+bs] X[b:b+bs]
-    const int below = nb - b - bs;
-    for (int e = tid; e < below * CB; e += 256) {
-      const int i = e / CB, c = e - i * CB;
-      const T* lrow = L + (int64_t)(b + bs + i) * ldl + b;
-      T acc = T(0);
-      for (int q = 0; q < bs; ++q) acc += lrow[q] * Xs[(b + q) * XP + c];
-      Xs[(b + bs + i) * XP + c] -= acc;
+    // rows below the block, 32 at a time: X[r] -= L[r, b:b+32] X[b:b+32],
+    // the L rows staged in LDS (zero padded)
+    for (int rb = b + 32; rb < nbp; rb += 32) {
+      for (int e = tid; e < 32 * 32; e += 256) {
+        const int i = e >> 5, q = e & 31;
+        Ls[i][q] = (rb + i < nb && q < bs) ? L[(int64_t)(rb + i) * ldl + b + q] : T(0);
+      }
+      __syncthreads();
+      for (int e = tid; e < 32 * CB; e += 256) {
+        const int i = e / CB, c = e - i * CB;
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < 32; ++q) acc += Ls[i][q] * Xs[(b + q) * XP + c];
+        Xs[(rb + i) * XP + c] -= acc;
+      }
+      __syncthreads();
     }
     __syncthreads();
   }
