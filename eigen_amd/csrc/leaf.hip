@@ -85,58 +85,6 @@ __device__ __forceinline__ double score_of(double v, bool first) {
   return s;
 }
 
-// Register-row access by a uniform runtime column: a switch over the 32
-// compile-time indices (scalar branches), never a runtime index into the
-// register array.
-#define EAMD_CASES(X)                                                                        \
-  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
-  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
-
-template <typename T>
-__device__ __forceinline__ T pick(const T (&v)[32], int j) {
-  T r = v[0];
-  switch (j) {
-#define EAMD_PICK(c) \
-  case c:            \
-    r = v[c];        \
-    break;
-    EAMD_CASES(EAMD_PICK)
-#undef EAMD_PICK
-    default:
-      break;
-  }
-  return r;
-}
-
-template <typename T>
-__device__ __forceinline__ void place(T (&v)[32], int j, T x) {
-  switch (j) {
-#define EAMD_PLACE(c) \
-  case c:             \
-    v[c] = x;         \
-    break;
-    EAMD_CASES(EAMD_PLACE)
-#undef EAMD_PLACE
-    default:
-      break;
-  }
-}
-
-// v[c] -= l * u[c] for c = j+1 .. 31: enter at case j and fall through.
-template <typename T>
-__device__ __forceinline__ void axpy_tail(T (&v)[32], const T (&u)[32], int j, T l) {
-  switch (j) {
-#define EAMD_AXPY(c)                         \
-  case c:                                    \
-    if (c + 1 < 32) v[(c + 1) & 31] -= l * u[(c + 1) & 31]; \
-    [[fallthrough]];
-    EAMD_CASES(EAMD_AXPY)
-#undef EAMD_AXPY
-    default:
-      break;
-  }
-}
-
 // Rotation of a register row by 8 slots (compile-time moves): the column
 // loops work on groups of 8 columns that always sit in slots 0..7, so every
 // register access has a compile-time index; after 4 rotations the row is back
@@ -209,52 +157,106 @@ constexpr int slot_words() {
   return 4 + LW * Words<T>::n;
 }
 
-// Replays the interchanges piv_s[0..w) of a leaf (leaf-relative rows, in
-// column order) into the leaf-level slot table (rows 0..w-1 are slots
-// 0..w-1, rows swapped in from below are appended) and the panel-level one
-// (rows 0..pnb-1 are slots 0..pnb-1; the leaf starts at panel row row0).
-// One wave; lane 0 also stores piv[] (plus piv_offset).
-__device__ void replay_tables(const int* piv_s, int w, int row0, int pnb, int lane, int* piv,
-                              int piv_offset, int* lPos, int* lCont, int* s_lnS, int* pPos,
-                              int* pCont, int* s_pnS) {
-  for (int j = 0; j < w; ++j) {
-    // lane 0 appends to the tables that the other lanes search: keep the
-    // compiler from reusing values it loaded in an earlier iteration (a wave
-    // executes its LDS accesses in order)
-    asm volatile("" ::: "memory");
-    const int p = piv_s[j];
-    if (lane == 0) piv[j] = p + piv_offset;
-    if (p == j) continue;
-    int slotb = p;
-    if (p >= w) {
-      const int nS = *s_lnS;
-      int found = -1;
-      for (int s2 = w + lane; s2 < nS; s2 += 64)
-        if (lPos[s2] == p) found = s2;
-      unsigned long long bal = __ballot(found >= 0);
-      slotb = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
-      if (lane == 0 && !bal) { lPos[nS] = p; lCont[nS] = nS; *s_lnS = nS + 1; }
+__device__ __forceinline__ int readlane_i(int x, int lane) {
+  return __builtin_amdgcn_readlane(x, lane);
+}
+__device__ __forceinline__ float readlane_f(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+// The leaf-level permutation of a leaf's row interchanges, in registers (one
+// wave): entry k (lane k < ln) is leaf row tpos, which receives the content
+// of entry tsrc and so holds original row trow (rows 0..w-1 are entries
+// 0..w-1, rows swapped in from below are appended).  FROM_SEL: lane c < w
+// holds x = the original row selected for row c, and x returns the
+// transposition of column c (the row swapped with row c); otherwise x is
+// that transposition.
+template <bool FROM_SEL>
+__device__ __forceinline__ void leaf_perm(int& x, int w, int lane, int& tpos, int& trow,
+                                          int& tsrc, int& ln) {
+  tpos = lane < w ? lane : -1;
+  trow = tpos;
+  tsrc = lane;
+  int nent = w, pivv = 0;
+  for (int c = 0; c < w; ++c) {
+    const int p = readlane_i(x, c);
+    const u64 hit = __ballot((FROM_SEL ? trow : tpos) == p);
+    int k;
+    if (hit) {
+      k = __ffsll((long long)hit) - 1;
+    } else {  // an untouched row p (>= w) sits at position p
+      k = nent++;
+      if (lane == k) { tpos = p; trow = p; tsrc = k; }
     }
-    const int pa = row0 + j, pb = row0 + p;
-    int pslot = pb;
-    if (pb >= pnb) {
-      const int nS = *s_pnS;
-      int found = -1;
-      for (int s2 = pnb + lane; s2 < nS; s2 += 64)
-        if (pPos[s2] == pb) found = s2;
-      unsigned long long bal = __ballot(found >= 0);
-      pslot = bal ? __shfl(found, __ffsll((long long)bal) - 1) : nS;
-      if (lane == 0 && !bal) { pPos[nS] = pb; pCont[nS] = nS; *s_pnS = nS + 1; }
+    if (FROM_SEL) {
+      const int pk = readlane_i(tpos, k);
+      if (lane == c) pivv = pk;
     }
-    if (lane == 0) {
-      int t = lCont[j];
-      lCont[j] = lCont[slotb];
-      lCont[slotb] = t;
-      t = pCont[pa];
-      pCont[pa] = pCont[pslot];
-      pCont[pslot] = t;
-    }
+    const int rc = readlane_i(trow, c), rk = readlane_i(trow, k);
+    const int sc = readlane_i(tsrc, c), sk = readlane_i(tsrc, k);
+    if (lane == c) { trow = rk; tsrc = sk; }
+    if (lane == k) { trow = rc; tsrc = sc; }
   }
+  if (FROM_SEL) x = pivv;
+  ln = nent;
+}
+
+// Composes a leaf's list (lane k < ln: leaf row tpos receives the content of
+// entry tsrc) into the panel-level list ppos/psrc/pperm_n (the leaf starts at
+// panel row row0; panel rows 0..pnb-1 are slots 0..pnb-1, rows below them are
+// appended slots; slot s: row ppos[s] receives the content of slot psrc[s]).
+// All entries at once: a touched slot's new source is the old source of its
+// entry's source slot, and a newly appended slot's old source is itself.
+// One wave; s_pp is LDS scratch of at least pnb + 4 ints.
+__device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int lane,
+                              int* pperm_n, int* ppos, int* psrc, int* s_pp) {
+  const int pn0 = *pperm_n, na = pn0 - pnb, na4 = (na + 3) & ~3;
+  for (int e = lane; e < na4; e += 64) s_pp[e] = e < na ? ppos[pnb + e] : -1;
+  asm volatile("" ::: "memory");  // LDS accesses of one wave execute in order
+  const bool mine = lane < ln;
+  const int prow = row0 + tpos;
+  int slot = prow;
+  bool need = false;
+  if (mine && prow >= pnb) {
+    int f = -1;
+    for (int e = 0; e < na4; e += 4) {
+      const int4 q = *reinterpret_cast<const int4*>(&s_pp[e]);
+      f = q.x == prow ? pnb + e : f;
+      f = q.y == prow ? pnb + e + 1 : f;
+      f = q.z == prow ? pnb + e + 2 : f;
+      f = q.w == prow ? pnb + e + 3 : f;
+    }
+    slot = f;
+    need = f < 0;
+  }
+  const u64 nb = __ballot(need);
+  if (need) slot = pn0 + __popcll(nb & ((1ull << lane) - 1ull));
+  int old = slot;
+  if (mine && !need) old = psrc[slot];
+  const int src_old = __shfl(old, tsrc);
+  if (mine) {
+    psrc[slot] = src_old;
+    if (need) ppos[slot] = prow;
+  }
+  if (lane == 0) *pperm_n = pn0 + __popcll(nb);
+}
+
+// All lists of a leaf from its interchanges (one wave): piv[] (plus
+// piv_offset), the leaf-level list lperm_n/lpos/lsrc and the panel-level one.
+// FROM_SEL as for leaf_perm; tpos/trow return the leaf list's entries.
+template <bool FROM_SEL>
+__device__ void leaf_lists(int x, int w, int row0, int pnb, int lane, int* piv, int piv_offset,
+                           int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos,
+                           int* psrc, int* s_pp, int& tpos, int& trow, int& ln) {
+  int tsrc;
+  leaf_perm<FROM_SEL>(x, w, lane, tpos, trow, tsrc, ln);
+  if (lane < w) piv[lane] = x + piv_offset;
+  if (lane < ln) {
+    lpos[lane] = tpos;
+    lsrc[lane] = tsrc;
+  }
+  if (lane == 0) *lperm_n = ln;
+  panel_compose(tpos, tsrc, ln, row0, pnb, lane, pperm_n, ppos, psrc, s_pp);
 }
 
 }  // namespace
@@ -290,10 +292,9 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   __shared__ T rowjL[2][LW];  // row j, when this workgroup owns it
   __shared__ unsigned stage[STAGE];
   __shared__ int s_abort;
-  // bookkeeping (workgroup 0): leaf-level and panel-level slot tables
-  __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
-  __shared__ int pPos[512], pCont[512], s_pnS;
+  // bookkeeping (workgroup 0): the leaf's pivots and the panel list's rows
   __shared__ int s_piv[LW];
+  __shared__ __align__(16) int s_pp[256 + 4];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = blockIdx.x;
@@ -305,13 +306,6 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     const T* src = P + (int64_t)i * ldp;
 #pragma unroll
     for (int c = 0; c < LW; ++c) v[c] = (valid && c < w) ? src[c] : T(0);
-  }
-  if (g == 0) {
-    // leaf rows 0..w-1 are slots 0..w-1; rows swapped in from below are appended
-    for (int s = tid; s < w; s += BLOCK) { lPos[s] = s; lCont[s] = s; }
-    if (tid == 0) { s_lnS = w; s_pnS = *pperm_n; }
-    __syncthreads();
-    for (int s = tid; s < s_pnS; s += BLOCK) { pPos[s] = ppos[s]; pCont[s] = psrc[s]; }
   }
   if (tid == 0) s_abort = 0;
   __syncthreads();
@@ -509,27 +503,21 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   if (s_abort) return;
   __syncthreads();
   // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
-  // permutation tables, replaying the interchanges in column order
-  if (g == 0 && wave == 0)
-    replay_tables(s_piv, w, row0, pnb, lane, piv, piv_offset, lPos, lCont, &s_lnS, pPos, pCont,
-                  &s_pnS);
-  __syncthreads();
+  // permutation lists, replaying the interchanges in column order
+  if (g == 0 && wave == 0) {
+    int tpos, trow, ln;
+    leaf_lists<false>(lane < w ? s_piv[lane] : 0, w, row0, pnb, lane, piv, piv_offset, lperm_n,
+                      lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, ln);
+  }
   if (valid) {
     T* dst = P + (int64_t)i * ldp;
 #pragma unroll
     for (int c = 0; c < LW; ++c)
       if (c < w) dst[c] = v[c];
   }
-  if (g == 0) {
-    const int ln = s_lnS, pn = s_pnS;
-    for (int s = tid; s < ln; s += BLOCK) { lpos[s] = lPos[s]; lsrc[s] = lCont[s]; }
-    for (int s = tid; s < pn; s += BLOCK) { ppos[s] = pPos[s]; psrc[s] = pCont[s]; }
-    if (tid == 0) {
-      *lperm_n = ln;
-      *pperm_n = pn;
-      *zero_col = first_zero;
-      if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
-    }
+  if (g == 0 && tid == 0) {
+    *zero_col = first_zero;
+    if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
   }
 }
 
@@ -588,27 +576,66 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
 // optimal LU factorization algorithm", SIAM J. Matrix Anal. Appl. 32, 2011)
 // for the FP32 panel of the mixed-precision solver, whose result is judged
 // by the backward error after refinement rather than by Eigen's pivot
-// sequence.  A leaf of m rows costs three launches instead of one exchange
-// round trip per column:
-//   1. every workgroup runs partial pivoting on its own 1024 rows, in
-//      registers, and records the rows it picked (up to 32 candidates);
-//   2. one workgroup runs partial pivoting on the original values of all
-//      candidates: the rows it picks, in order, are the leaf's pivots and its
-//      first 32 rows after elimination are the factored top block L11\U11,
-//      written in place; it also saves the original rows 0..w-1, replays the
-//      interchanges into the permutation tables and lists where the displaced
+// sequence.  A leaf of m rows costs a few short launches instead of one
+// exchange round trip per column:
+//   1. one wave per 256 rows runs partial pivoting on them, in registers,
+//      and records the rows it picked (up to 32 candidates);
+//   2. while more than 8 candidate groups remain, one wave per 8 groups runs
+//      partial pivoting on the original values of their candidates and keeps
+//      32 of them;
+//   3. one wave does the same over the last (at most 8) groups: the rows it
+//      picks, in order, are the leaf's pivots and its first 32 rows after
+//      elimination are the factored top block L11\U11, written in place; it
+//      also saves the original rows 0..w-1, derives the transpositions,
+//      replays them into the permutation lists and lists where the displaced
 //      rows go;
-//   3. every other row is solved against U11 (l = a U11^-1) in parallel.
+//   4. every other row is solved against U11 (l = a U11^-1) in parallel.
 namespace {
 
-constexpr int kTslBlock = 256;  // rows (candidates) per workgroup
-constexpr int kTslFan = kTslBlock / LW;  // candidate groups merged per workgroup
+constexpr int kTslR = 4;                 // rows per lane
+constexpr int kTslBlock = 64 * kTslR;    // rows (candidates) per wave
+constexpr int kTslFan = kTslBlock / LW;  // candidate groups merged per wave
 
-// MODE 0: local selection over rows blockIdx.x*256 + tid, candidates to
-// cand_out.  MODE 2: merge of candidate groups [8 b, 8 b + 8) of cand_in into
-// group b of cand_out.  MODE 1: the final round over all G_in (<= 8) groups.
+// Maximum of a 32-bit key over the 64 lanes of a wave (lane pattern as in
+// wave_max).
+__device__ __forceinline__ unsigned wave_max_u32(unsigned k) {
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0xB1, 0xf, 0xf, false));
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x4E, 0xf, 0xf, false));
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x141, 0xf, 0xf, false));
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x140, 0xf, 0xf, false));
+  const unsigned a = (unsigned)readlane_i((int)k, 0), b = (unsigned)readlane_i((int)k, 16);
+  const unsigned c = (unsigned)readlane_i((int)k, 32), d = (unsigned)readlane_i((int)k, 48);
+  return max(max(a, b), max(c, d));
+}
+
+// u[c] = v[rs][c] of lane wl for c in [C0, C1); rs and wl are wave-uniform,
+// so this is a scalar branch over the row slot and reads into scalar
+// registers.
+template <int C0, int C1>
+__device__ __forceinline__ void fetch_row(float (&u)[LW], const float (&v)[kTslR][LW], int rs,
+                                          int wl) {
+  if (rs == 0) {
+#pragma unroll
+    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[0][c], wl);
+  } else if (rs == 1) {
+#pragma unroll
+    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[1][c], wl);
+  } else if (rs == 2) {
+#pragma unroll
+    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[2][c], wl);
+  } else {
+#pragma unroll
+    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[3][c], wl);
+  }
+}
+
+// One wave per block of up to 256 candidate rows, 4 rows per lane, all in
+// registers.  MODE 0: local selection over rows blockIdx.x*256 + [0, 256),
+// candidates to cand.  MODE 2: merge of candidate groups [8 b, 8 b + 8) of
+// cand_in into group b of cand.  MODE 1: the final round over all G_in
+// (<= 8) groups.
 template <int MODE>
-__global__ __launch_bounds__(kTslBlock) void tsl_kernel(
+__global__ __launch_bounds__(64) void tsl_kernel(
     float* __restrict__ P, int64_t ldp, int m, int w, const int* __restrict__ cand_in,
     const int* __restrict__ cand_n_in, int G_in, int* __restrict__ cand,
     int* __restrict__ cand_n, float* __restrict__ saved, int* __restrict__ moved,
@@ -616,185 +643,202 @@ __global__ __launch_bounds__(kTslBlock) void tsl_kernel(
     int* __restrict__ lsrc, int* __restrict__ pperm_n, int* __restrict__ ppos,
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
     int* __restrict__ fz_global) {
-  constexpr int BLOCK = kTslBlock, NWAVE = BLOCK / 64;
-  // per step (double-buffered by parity, one barrier per step): each wave's
-  // best key, its row and that row's original index
-  __shared__ u64 red_k[2][NWAVE];
-  __shared__ float candw[2][NWAVE][LW];
-  __shared__ int cand_orig[2][NWAVE];
-  __shared__ int s_sel[LW];
+  constexpr int R = kTslR;
+  constexpr int kNone = 0x7fffffff;  // position of an empty slot
   __shared__ int s_off[kTslFan + 1];
-  __shared__ int lPos[2 * LW], lCont[2 * LW], s_lnS;
-  __shared__ int pPos[512], pCont[512], s_pnS;
-  __shared__ int s_piv[LW];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int orig, nrows;
-  bool valid;
+  __shared__ int s_sel[LW];
+  __shared__ __align__(16) int s_pp[256 + 4];
+  const int lane = threadIdx.x;
+  // MODE 1: the original rows 0..w-1 (saved before the top block overwrites
+  // them), loaded first and stored last
+  float sv[LW * LW / 64];
+  if (MODE == 1) {
+#pragma unroll
+    for (int k = 0; k < LW * LW / 64; ++k) {
+      const int e = lane + 64 * k, r = e / LW, c = e % LW;
+      sv[k] = P[(int64_t)r * ldp + min(c, w - 1)];
+    }
+  }
+  int orig[R], pos[R];
+  int nrows;
   if (MODE == 0) {
-    orig = blockIdx.x * BLOCK + tid;
-    valid = orig < m;
-    nrows = min(BLOCK, m - (int)blockIdx.x * BLOCK);
+    const int base = blockIdx.x * kTslBlock;
+    nrows = min(kTslBlock, m - base);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = r * 64 + lane;
+      orig[r] = base + q;
+      pos[r] = q < nrows ? q : kNone;
+    }
   } else {
     const int g0 = MODE == 2 ? blockIdx.x * kTslFan : 0;
     const int ng = min(kTslFan, G_in - g0);
-    if (tid == 0) {
+    if (lane == 0) {
       int acc = 0;
       for (int g = 0; g < ng; ++g) { s_off[g] = acc; acc += cand_n_in[g0 + g]; }
       s_off[ng] = acc;
     }
     __syncthreads();
     nrows = s_off[ng];
-    valid = tid < nrows;
-    orig = 0;
-    if (valid) {
-      int g = 0;
-      while (s_off[g + 1] <= tid) ++g;
-      orig = cand_in[(g0 + g) * LW + tid - s_off[g]];
-    }
-  }
-  float v[LW];
-  {
-    const float* src = P + (int64_t)orig * ldp;
 #pragma unroll
-    for (int c = 0; c < LW; ++c) v[c] = (valid && c < w) ? src[c] : 0.f;
-  }
-  if (MODE == 1)  // the original rows 0..w-1, before the top block overwrites them
-    for (int e = tid; e < w * LW; e += BLOCK) {
-      const int r = e / LW, c = e - r * LW;
-      saved[e] = c < w ? P[(int64_t)r * ldp + c] : 0.f;
+    for (int r = 0; r < R; ++r) {
+      const int q = r * 64 + lane;
+      orig[r] = 0;
+      pos[r] = kNone;
+      if (q < nrows) {
+        int g = 0;
+        while (s_off[g + 1] <= q) ++g;
+        orig[r] = cand_in[(g0 + g) * LW + q - s_off[g]];
+        pos[r] = q;
+      }
     }
+  }
+  // An empty slot loads row 0 and a leaf narrower than 32 repeats its last
+  // column: such values are never keyed, selected or stored.
+  float v[R][LW];
+  if (w == LW) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const float* src = P + (int64_t)(pos[r] != kNone ? orig[r] : 0) * ldp;
+#pragma unroll
+      for (int c = 0; c < LW; ++c) v[r][c] = src[c];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const float* src = P + (int64_t)(pos[r] != kNone ? orig[r] : 0) * ldp;
+#pragma unroll
+      for (int c = 0; c < LW; ++c) v[r][c] = src[min(c, w - 1)];
+    }
+  }
   const int steps = min(w, nrows);
-  const int ngroups = (steps + 7) / 8;
   int first_zero = -1;
-  // Rows never move between threads: an interchange only relabels
-  // positions (pos), so a step costs one reduction and one barrier.  The
-  // reduction key orders by score, then by lower position (the first
-  // maximum in row order); a NaN scores only from position j, as infinity.
-  int pos = valid ? tid : 0x7fffffff;
-  // groups of 8 columns in slots 0..7 (see rotate8); slots [live, 32) hold
-  // columns finished in earlier groups and take no update
+  // Rows never move between lanes: an interchange only relabels positions
+  // (pos).  The pivot key is |a| with its 9 low mantissa bits replaced by the
+  // position (lower position wins, so exact ties go to the first row in
+  // order; scores within 2^-14 relative count as ties, which keeps every
+  // multiplier within 1 + 2^-14).  A NaN scores only from position j, as
+  // infinity.  Column j of group g8 is register slot jj (see rotate8); slots
+  // [live, 32) hold columns of earlier groups and are skipped by uniform
+  // branches on 8-slot chunks, so the update does exactly the triangle.
+  auto column = [&](auto jjc, int j, int live) __attribute__((always_inline)) {
+    constexpr int jj = decltype(jjc)::value;
+    unsigned key = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const unsigned a0 = __float_as_uint(v[r][jj]) & 0x7fffffffu;
+      const bool elig =
+          pos[r] >= j && pos[r] < kTslBlock && (a0 <= 0x7f800000u || pos[r] == j);
+      const unsigned kr = (min(a0, 0x7f800000u) & ~0x1ffu) | (0x1ffu - (unsigned)pos[r]);
+      key = max(key, elig ? kr : 0u);
+    }
+    // position j holds a row, so the winner exists
+    const unsigned wk = wave_max_u32(key);
+    const int bpos = 0x1ff - (int)(wk & 0x1ffu);
+    const u64 h0 = __ballot(pos[0] == bpos), h1 = __ballot(pos[1] == bpos),
+              h2 = __ballot(pos[2] == bpos), h3 = __ballot(pos[3] == bpos);
+    const int rs = h0 ? 0 : (h1 ? 1 : (h2 ? 2 : 3));
+    const int wl = __ffsll((long long)(h0 | h1 | h2 | h3)) - 1;
+    float u[LW];
+    fetch_row<jj, 8>(u, v, rs, wl);
+    const float pv = u[jj];
+    if (pv == 0.f && first_zero < 0) first_zero = j;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int p = pos[r];
+      pos[r] = p == bpos ? j : (p == j ? bpos : p);
+    }
+    const float rinv = 1.0f / pv;
+    float l[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool act = pv != 0.f && pos[r] > j && pos[r] < kTslBlock;
+      float x = act ? v[r][jj] * rinv : 0.f;
+      asm volatile("" : "+v"(x));  // keep the update unconditional
+      l[r] = x;
+      v[r][jj] = act ? x : v[r][jj];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = jj + 1; c < 8; ++c) v[r][c] -= l[r] * u[c];
+    if (live > 8) {
+      fetch_row<8, 16>(u, v, rs, wl);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 8; c < 16; ++c) v[r][c] -= l[r] * u[c];
+    }
+    if (live > 16) {
+      fetch_row<16, 24>(u, v, rs, wl);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 16; c < 24; ++c) v[r][c] -= l[r] * u[c];
+    }
+    if (live > 24) {
+      fetch_row<24, 32>(u, v, rs, wl);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 24; c < 32; ++c) v[r][c] -= l[r] * u[c];
+    }
+  };
+  const int ngroups = (steps + 7) / 8;
 #pragma unroll 1
   for (int g8 = 0; g8 < ngroups; ++g8) {
-    const int live = LW - 8 * g8;
+    const int live = LW - 8 * g8, j0 = 8 * g8;
+#define EAMD_COL(JJ) \
+    if (j0 + JJ < steps) column(std::integral_constant<int, JJ>{}, j0 + JJ, live);
+    EAMD_COL(0) EAMD_COL(1) EAMD_COL(2) EAMD_COL(3) EAMD_COL(4) EAMD_COL(5) EAMD_COL(6)
+    EAMD_COL(7)
+#undef EAMD_COL
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int j = g8 * 8 + jj;
-      if (j >= steps) break;
-      const int par = j & 1;
-      const float vj = v[jj];
-      u64 key = 0;
-      if (pos >= j && pos != 0x7fffffff) {
-        float a = fabsf(vj);
-        if (a != a) a = pos == j ? INFINITY : -1.0f;
-        if (a >= 0.0f) key = ((u64)__float_as_uint(a) << 32) | (0xffffffffu - (unsigned)pos);
-      }
-      const u64 wk = wave_max_u64(key);
-      const int wl = __ffsll((long long)__ballot(key == wk)) - 1;
-      if (lane == wl) {
-#pragma unroll
-        for (int c = 0; c < LW; ++c) candw[par][wave][c] = v[c];
-        cand_orig[par][wave] = orig;
-        red_k[par][wave] = wk;
-      }
-      __syncthreads();
-      u64 best = red_k[par][0];
-      int bq = 0;
-#pragma unroll
-      for (int q = 1; q < NWAVE; ++q) {
-        const u64 k2 = red_k[par][q];
-        if (k2 > best) { best = k2; bq = q; }
-      }
-      // position j holds a valid row, so the winner is set
-      const int bpos = (int)(0xffffffffu - (unsigned)best);
-      const float* up = candw[par][bq];
-      if (tid == 0) s_sel[j] = cand_orig[par][bq];
-      // interchange positions j and bpos
-      if (pos == bpos) pos = j;
-      else if (pos == j) pos = bpos;
-      // branch-free (a branch around the register row makes the compiler
-      // copy the whole row at the join)
-      const float pv = up[jj];
-      if (pv == 0.f && first_zero < 0) first_zero = j;
-      const bool act = pv != 0.f && pos > j && pos != 0x7fffffff;
-      const float qv = vj / pv;
-      float l = act ? qv : 0.f;
-      asm volatile("" : "+v"(l));  // keep the update unconditional
-      v[jj] = act ? l : v[jj];
-      const unsigned lm = 0u - (unsigned)(live > 8), lm16 = 0u - (unsigned)(live > 16),
-                     lm24 = 0u - (unsigned)(live > 24);
-#pragma unroll
-      for (int c = jj + 1; c < LW; ++c) {
-        // slot c is live iff c < live; an integer mask on the loaded value
-        // keeps the LDS reads unconditional
-        const unsigned m = c < 8 ? ~0u : (c < 16 ? lm : (c < 24 ? lm16 : lm24));
-        const float u = mask_sel(m, up[c], 0.f);
-        v[c] -= l * u;
-      }
-    }
-    rotate8(v);
+    for (int r = 0; r < R; ++r) rotate8(v[r]);
   }
-#pragma unroll 1
-  for (int r = ngroups; r < 4; ++r) rotate8(v);
-  __syncthreads();
   if (MODE != 1) {
-    if (tid < steps) cand[blockIdx.x * LW + tid] = s_sel[tid];
-    if (tid == 0) cand_n[blockIdx.x] = steps;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (pos[r] < steps) cand[blockIdx.x * LW + pos[r]] = orig[r];
+    if (lane == 0) cand_n[blockIdx.x] = steps;
     return;
   }
-  // ---- final round: the top block in place (rows 0..w-1)
-  if (pos < steps) {
-    float* dst = P + (int64_t)pos * ldp;
+#pragma unroll 1
+  for (int q = ngroups; q < 4; ++q)
 #pragma unroll
-    for (int c = 0; c < LW; ++c)
-      if (c < w) dst[c] = v[c];
+    for (int r = 0; r < R; ++r) rotate8(v[r]);
+#pragma unroll
+  for (int k = 0; k < LW * LW / 64; ++k) {
+    const int e = lane + 64 * k;
+    saved[e] = e % LW < w ? sv[k] : 0.f;
   }
-  // ---- transpositions from the selection order (wave 0): lane k holds the
-  // row now at touched position tpos_k; positions 0..w-1 are entries 0..w-1
-  if (wave == 0) {
-    int tpos = lane < w ? lane : -1, trow = lane < w ? lane : -1;
-    int nent = w;
-    for (int c = 0; c < w; ++c) {
-      const int p = s_sel[c];
-      const unsigned long long hit = __ballot(trow == p);
-      int k;
-      if (hit) {
-        k = __ffsll((long long)hit) - 1;
-      } else {  // untouched row p (>= w): it sits at position p
-        k = nent++;
-        if (lane == k) { tpos = p; trow = p; }
-      }
-      const int pos = __shfl(tpos, k);
-      if (lane == 0) s_piv[c] = pos;
-      const int rc = __shfl(trow, c), rk = __shfl(trow, k);
-      if (lane == c) trow = rk;
-      if (lane == k) trow = rc;
+  // ---- final round: the top block in place (rows 0..w-1) and the selection
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (pos[r] < steps) {
+      float* dst = P + (int64_t)pos[r] * ldp;
+#pragma unroll
+      for (int c = 0; c < LW; ++c)
+        if (c < w) dst[c] = v[r][c];
+      s_sel[pos[r]] = orig[r];
     }
-    // positions >= w now hold displaced original rows < w: list them
-    const bool mv = tpos >= w && lane < nent;
-    const unsigned long long mb = __ballot(mv);
-    const int idx = __popcll(mb & ((1ull << lane) - 1ull));
-    if (mv) {
-      moved[1 + 2 * idx] = tpos;
-      moved[2 + 2 * idx] = trow;
-    }
-    if (lane == 0) moved[0] = __popcll(mb);
-    // permutation tables, as the partial-pivoting leaf writes them
-    for (int e = lane; e < w; e += 64) { lPos[e] = e; lCont[e] = e; }
-    if (lane == 0) { s_lnS = w; s_pnS = *pperm_n; }
-    const int pn0 = *pperm_n;
-    for (int e = lane; e < pn0; e += 64) { pPos[e] = ppos[e]; pCont[e] = psrc[e]; }
-    replay_tables(s_piv, w, row0, pnb, lane, piv, piv_offset, lPos, lCont, &s_lnS, pPos, pCont,
-                  &s_pnS);
-    asm volatile("" ::: "memory");
-    const int ln = s_lnS, pn = s_pnS;
-    for (int e = lane; e < ln; e += 64) { lpos[e] = lPos[e]; lsrc[e] = lCont[e]; }
-    for (int e = lane; e < pn; e += 64) { ppos[e] = pPos[e]; psrc[e] = pCont[e]; }
-    if (lane == 0) {
-      *lperm_n = ln;
-      *pperm_n = pn;
-      *zero_col = first_zero;
-      if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
-    }
+  __syncthreads();
+  // ---- transpositions and permutation lists from the selection order
+  int tpos, trow, ln;
+  leaf_lists<true>(lane < w ? s_sel[lane] : 0, w, row0, pnb, lane, piv, piv_offset, lperm_n,
+                   lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, ln);
+  // positions >= w now hold displaced original rows < w: list them
+  const bool mv = tpos >= w && lane < ln;
+  const u64 mb = __ballot(mv);
+  const int idx = __popcll(mb & ((1ull << lane) - 1ull));
+  if (mv) {
+    moved[1 + 2 * idx] = tpos;
+    moved[2 + 2 * idx] = trow;
+  }
+  if (lane == 0) moved[0] = __popcll(mb);
+  if (lane == 0) {
+    *zero_col = first_zero;
+    if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
   }
 }
 
@@ -877,21 +921,21 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   int* moved = nB + G1;
   float* saved = reinterpret_cast<float*>(moved + 2 * LW + 8);
   hipStream_t s = h->stream;
-  hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(kTslBlock), 0, s, P, ldp, m, w, nullptr,
+  hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(64), 0, s, P, ldp, m, w, nullptr,
                      nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos,
                      psrc, row0, pnb, zero_col, piv_offset, fz_global);
   int G = G1;
   int *cin = candA, *nin = nA, *cout = candB, *nout = nB;
   while (G > kTslFan) {
     const int Gn = ceil_div(G, kTslFan);
-    hipLaunchKernelGGL(tsl_kernel<2>, dim3(Gn), dim3(kTslBlock), 0, s, P, ldp, m, w, cin, nin, G,
+    hipLaunchKernelGGL(tsl_kernel<2>, dim3(Gn), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
                        cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
                        row0, pnb, zero_col, piv_offset, fz_global);
     G = Gn;
     std::swap(cin, cout);
     std::swap(nin, nout);
   }
-  hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(kTslBlock), 0, s, P, ldp, m, w, cin, nin, G,
+  hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
                      cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
                      pnb, zero_col, piv_offset, fz_global);
   if (m > w) {
