@@ -596,16 +596,18 @@ constexpr int kTslR = 4;                 // rows per lane
 constexpr int kTslBlock = 64 * kTslR;    // rows (candidates) per wave
 constexpr int kTslFan = kTslBlock / LW;  // candidate groups merged per wave
 
-// Maximum of a 32-bit key over the 64 lanes of a wave (lane pattern as in
-// wave_max).
+// Maximum of a 32-bit key over the 64 lanes of a wave: xor-1 and xor-2
+// within quads, half-row and row mirrors within each 16-lane row, then the
+// row maxima are carried into lane 63 (row_bcast:15 into rows 1 and 3,
+// row_bcast:31 into rows 2 and 3).
 __device__ __forceinline__ unsigned wave_max_u32(unsigned k) {
   k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0xB1, 0xf, 0xf, false));
   k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x4E, 0xf, 0xf, false));
   k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x141, 0xf, 0xf, false));
   k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x140, 0xf, 0xf, false));
-  const unsigned a = (unsigned)readlane_i((int)k, 0), b = (unsigned)readlane_i((int)k, 16);
-  const unsigned c = (unsigned)readlane_i((int)k, 32), d = (unsigned)readlane_i((int)k, 48);
-  return max(max(a, b), max(c, d));
+  k = max(k, (unsigned)__builtin_amdgcn_update_dpp((int)k, (int)k, 0x142, 0xa, 0xf, false));
+  k = max(k, (unsigned)__builtin_amdgcn_update_dpp((int)k, (int)k, 0x143, 0xc, 0xf, false));
+  return (unsigned)readlane_i((int)k, 63);
 }
 
 // u[c] = v[rs][c] of lane wl for c in [C0, C1); rs and wl are wave-uniform,
@@ -644,7 +646,6 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
     int* __restrict__ fz_global) {
   constexpr int R = kTslR;
-  constexpr int kNone = 0x7fffffff;  // position of an empty slot
   __shared__ int s_off[kTslFan + 1];
   __shared__ int s_sel[LW];
   __shared__ __align__(16) int s_pp[256 + 4];
@@ -659,7 +660,8 @@ __global__ __launch_bounds__(64) void tsl_kernel(
       sv[k] = P[(int64_t)r * ldp + min(c, w - 1)];
     }
   }
-  int orig[R], pos[R];
+  int orig[R];
+  bool valid[R];
   int nrows;
   if (MODE == 0) {
     const int base = blockIdx.x * kTslBlock;
@@ -667,8 +669,8 @@ __global__ __launch_bounds__(64) void tsl_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int q = r * 64 + lane;
-      orig[r] = base + q;
-      pos[r] = q < nrows ? q : kNone;
+      valid[r] = q < nrows;
+      orig[r] = valid[r] ? base + q : 0;
     }
   } else {
     const int g0 = MODE == 2 ? blockIdx.x * kTslFan : 0;
@@ -684,12 +686,11 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     for (int r = 0; r < R; ++r) {
       const int q = r * 64 + lane;
       orig[r] = 0;
-      pos[r] = kNone;
-      if (q < nrows) {
+      valid[r] = q < nrows;
+      if (valid[r]) {
         int g = 0;
         while (s_off[g + 1] <= q) ++g;
         orig[r] = cand_in[(g0 + g) * LW + q - s_off[g]];
-        pos[r] = q;
       }
     }
   }
@@ -699,89 +700,94 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   if (w == LW) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const float* src = P + (int64_t)(pos[r] != kNone ? orig[r] : 0) * ldp;
+      const float* src = P + (int64_t)orig[r] * ldp;
 #pragma unroll
       for (int c = 0; c < LW; ++c) v[r][c] = src[c];
     }
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const float* src = P + (int64_t)(pos[r] != kNone ? orig[r] : 0) * ldp;
+      const float* src = P + (int64_t)orig[r] * ldp;
 #pragma unroll
       for (int c = 0; c < LW; ++c) v[r][c] = src[min(c, w - 1)];
     }
   }
   const int steps = min(w, nrows);
   int first_zero = -1;
-  // Rows never move between lanes: an interchange only relabels positions
-  // (pos).  The pivot key is |a| with its 9 low mantissa bits replaced by the
-  // position (lower position wins, so exact ties go to the first row in
-  // order; scores within 2^-14 relative count as ties, which keeps every
-  // multiplier within 1 + 2^-14).  A NaN scores only from position j, as
-  // infinity.  Column j of group g8 is register slot jj (see rotate8); slots
-  // [live, 32) hold columns of earlier groups and are skipped by uniform
-  // branches on 8-slot chunks, so the update does exactly the triangle.
+  // Rows never move between lanes.  Slot q = 64 r + lane holds a candidate
+  // row while em[r] is all ones (valid and not yet selected); st[r] is the
+  // step that selected it.  The pivot key of slot q is |a| with its 9 low
+  // mantissa bits replaced by 511 - q (km = em & 0x7ffffe00, kq = em &
+  // (511 - q); zero for a slot that is out), so the maximum names its slot,
+  // exact ties go to the lowest slot (the first row in order for the local
+  // round) and scores within 2^-14 relative count as ties, which keeps every
+  // multiplier within 1 + 2^-14.  Column j of group g8 is register slot jj
+  // (see rotate8); slots [live, 32) hold columns of earlier groups and are
+  // skipped by uniform branches on 8-slot chunks, so the update does exactly
+  // the triangle.  The multipliers use the reciprocal of the pivot.
+  unsigned em[R], km[R], kq[R];
+  int st[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    em[r] = valid[r] ? ~0u : 0u;
+    km[r] = em[r] & 0x7ffffe00u;
+    kq[r] = em[r] & (0x1ffu - (unsigned)(r * 64 + lane));
+    st[r] = -1;
+  }
   auto column = [&](auto jjc, int j, int live) __attribute__((always_inline)) {
     constexpr int jj = decltype(jjc)::value;
-    unsigned key = 0;
+    unsigned kr[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const unsigned a0 = __float_as_uint(v[r][jj]) & 0x7fffffffu;
-      const bool elig =
-          pos[r] >= j && pos[r] < kTslBlock && (a0 <= 0x7f800000u || pos[r] == j);
-      const unsigned kr = (min(a0, 0x7f800000u) & ~0x1ffu) | (0x1ffu - (unsigned)pos[r]);
-      key = max(key, elig ? kr : 0u);
-    }
-    // position j holds a row, so the winner exists
-    const unsigned wk = wave_max_u32(key);
-    const int bpos = 0x1ff - (int)(wk & 0x1ffu);
-    const u64 h0 = __ballot(pos[0] == bpos), h1 = __ballot(pos[1] == bpos),
-              h2 = __ballot(pos[2] == bpos), h3 = __ballot(pos[3] == bpos);
-    const int rs = h0 ? 0 : (h1 ? 1 : (h2 ? 2 : 3));
-    const int wl = __ffsll((long long)(h0 | h1 | h2 | h3)) - 1;
+    for (int r = 0; r < R; ++r) kr[r] = (__float_as_uint(v[r][jj]) & km[r]) | kq[r];
+    const unsigned key = max(max(kr[0], kr[1]), max(kr[2], kr[3]));
+    // a valid unselected slot exists (j < steps), so the winner does
+    const int qw = 0x1ff - (int)(wave_max_u32(key) & 0x1ffu);
+    const int rs = qw >> 6, wl = qw & 63;
+    const bool me = lane == wl;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (rs == r && me) {
+        em[r] = 0u;
+        km[r] = 0u;
+        kq[r] = 0u;
+        st[r] = j;
+      }
     float u[LW];
     fetch_row<jj, 8>(u, v, rs, wl);
     const float pv = u[jj];
     if (pv == 0.f && first_zero < 0) first_zero = j;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int p = pos[r];
-      pos[r] = p == bpos ? j : (p == j ? bpos : p);
-    }
-    const float rinv = 1.0f / pv;
+    const float rinv = pv != 0.f ? __builtin_amdgcn_rcpf(pv) : 0.f;
     float l[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const bool act = pv != 0.f && pos[r] > j && pos[r] < kTslBlock;
-      float x = act ? v[r][jj] * rinv : 0.f;
-      asm volatile("" : "+v"(x));  // keep the update unconditional
-      l[r] = x;
-      v[r][jj] = act ? x : v[r][jj];
+      const unsigned lb = __float_as_uint(v[r][jj] * rinv);
+      l[r] = __uint_as_float(lb & em[r]);
+      v[r][jj] = __uint_as_float((lb & em[r]) | (__float_as_uint(v[r][jj]) & ~em[r]));
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int c = jj + 1; c < 8; ++c) v[r][c] -= l[r] * u[c];
+      for (int c = jj + 1; c < 8; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     if (live > 8) {
       fetch_row<8, 16>(u, v, rs, wl);
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int c = 8; c < 16; ++c) v[r][c] -= l[r] * u[c];
+        for (int c = 8; c < 16; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     }
     if (live > 16) {
       fetch_row<16, 24>(u, v, rs, wl);
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int c = 16; c < 24; ++c) v[r][c] -= l[r] * u[c];
+        for (int c = 16; c < 24; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     }
     if (live > 24) {
       fetch_row<24, 32>(u, v, rs, wl);
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int c = 24; c < 32; ++c) v[r][c] -= l[r] * u[c];
+        for (int c = 24; c < 32; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     }
   };
   const int ngroups = (steps + 7) / 8;
@@ -799,7 +805,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   if (MODE != 1) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (pos[r] < steps) cand[blockIdx.x * LW + pos[r]] = orig[r];
+      if (st[r] >= 0) cand[blockIdx.x * LW + st[r]] = orig[r];
     if (lane == 0) cand_n[blockIdx.x] = steps;
     return;
   }
@@ -815,12 +821,12 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   // ---- final round: the top block in place (rows 0..w-1) and the selection
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (pos[r] < steps) {
-      float* dst = P + (int64_t)pos[r] * ldp;
+    if (st[r] >= 0) {
+      float* dst = P + (int64_t)st[r] * ldp;
 #pragma unroll
       for (int c = 0; c < LW; ++c)
         if (c < w) dst[c] = v[r][c];
-      s_sel[pos[r]] = orig[r];
+      s_sel[st[r]] = orig[r];
     }
   __syncthreads();
   // ---- transpositions and permutation lists from the selection order
