@@ -49,6 +49,37 @@ int Workspace::get(int slot, size_t need, void** out) {
   return kOk;
 }
 
+// The look-ahead streams and their ordering events, created on first use.
+// EAMD_PANEL_CUS > 0 asks for that many CUs (every k-th) reserved for the
+// panel stream and masked out of the update stream; where the runtime
+// refuses CU-masked streams both are plain streams.
+int env_int(const char* name, int dflt);
+int ensure_side_stream(eamd_handle* h) {
+  if (h->side) return kOk;
+  const int ncu = h->num_cus > 0 ? h->num_cus : 256;
+  const int want = env_int("EAMD_PANEL_CUS", 0);
+  if (want > 0 && want < ncu) {
+    std::vector<uint32_t> pm((ncu + 31) / 32, 0u), um((ncu + 31) / 32, 0u);
+    const int stride = ncu / want;
+    int taken = 0;
+    for (int c = 0; c < ncu; ++c) {
+      const bool panel = c % stride == 0 && taken < want;
+      taken += panel;
+      (panel ? pm : um)[c / 32] |= 1u << (c % 32);
+    }
+    if (hipExtStreamCreateWithCUMask(&h->pstream, (uint32_t)pm.size(), pm.data()) != hipSuccess ||
+        hipExtStreamCreateWithCUMask(&h->side, (uint32_t)um.size(), um.data()) != hipSuccess) {
+      (void)hipGetLastError();
+      if (h->pstream) (void)hipStreamDestroy(h->pstream);
+      if (h->side) (void)hipStreamDestroy(h->side);
+      h->pstream = h->side = nullptr;
+    }
+  }
+  if (!h->side) EAMD_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+  for (hipEvent_t& e : h->la_ev) EAMD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return kOk;
+}
+
 hipEvent_t PhaseTimer::take() {
   if (!pool.empty()) {
     hipEvent_t e = pool.back();
@@ -243,6 +274,16 @@ int eamd_create(eamd_handle_t* handle) {
 int eamd_destroy(eamd_handle_t h) {
   if (!h) return kOk;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    (void)hipStreamDestroy(h->side);
+    if (h->pstream) {
+      (void)hipStreamSynchronize(h->pstream);
+      (void)hipStreamDestroy(h->pstream);
+    }
+    for (hipEvent_t e : h->la_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   h->ws.release();
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

@@ -47,7 +47,7 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // A grow-only device arena owned by a handle.  Slots are named so that
 // concurrent users inside one call do not alias.
 struct Workspace {
-  static constexpr int kSlots = 16;
+  static constexpr int kSlots = 24;
   void* ptr[kSlots] = {};
   size_t bytes[kSlots] = {};
   int get(int slot, size_t need, void** out);
@@ -93,6 +93,13 @@ struct eamd_handle {
   int trsv_epoch = 0;   // monotonic epoch of the triangular-solve chain
   eamd::Workspace ws;
   eamd::PhaseTimer prof;
+  // look-ahead LU: a stream for the panels (on a reserved set of CUs) and
+  // one for the trailing update (the other CUs), both at the default
+  // priority, and the events that order them; pstream is null when no CUs
+  // are reserved (the panels then run on the handle's stream)
+  hipStream_t side = nullptr;
+  hipStream_t pstream = nullptr;
+  hipEvent_t la_ev[6] = {};
 };
 
 // Wraps one launch (or a short sequence) in a phase timer when enabled.
@@ -127,6 +134,8 @@ enum WsSlot : int {
   kWsAux2,
   kWsFactor32,     // FP32 factors kept for refinement
   kWsTsl,          // tournament-pivoting candidates and saved rows
+  kWsPackL,        // packed panel columns (A21) of two consecutive panels
+  kWsPackU,        // packed U rows (A12) of the next panel and of the rest
 };
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

@@ -84,6 +84,26 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
 // ---- batched plans (batched.hip) ----
 // Size sort of n[] (sizes 1..max_n <= 256) into a plan; the solvers then
 // fill in their per-size choices.
+// The handle's second stream (look-ahead LU) and its events (capi.hip).
+int ensure_side_stream(eamd_handle* h);
+
+// ---- mixed-precision trailing update (mixed.hip) ----
+// A packed operand: FP16/BF16 halves (hi, and lo for FP16x3) of K values per
+// row or column, padded to 128-row blocks, with one power-of-two scale per
+// block.
+struct MixedOperand {
+  unsigned short* hi = nullptr;
+  unsigned short* lo = nullptr;
+  float* scale = nullptr;
+};
+size_t mixed_operand_bytes(int len, int k, int precision);
+int mixed_pack(eamd_handle* h, bool rows, const float* S, int64_t ld, int len, int k,
+               int precision, void* buf, MixedOperand* out);
+// max_wgs > 0 caps the grid (rounded down to a multiple of 8; the
+// workgroups loop over the tiles).
+int mixed_update_packed(eamd_handle* h, const MixedOperand& a, int M, const MixedOperand& b, int N,
+                        int k, int precision, float* A22, int64_t ld, int max_wgs = 0);
+
 int batched_plan_sort(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
                       eamd_batched_plan** out);
 void batched_plan_destroy(eamd_batched_plan* p);

@@ -610,25 +610,21 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned k) {
   return (unsigned)readlane_i((int)k, 63);
 }
 
-// u[c] = v[rs][c] of lane wl for c in [C0, C1); rs and wl are wave-uniform,
-// so this is a scalar branch over the row slot and reads into scalar
-// registers.
-template <int C0, int C1>
-__device__ __forceinline__ void fetch_row(float (&u)[LW], const float (&v)[kTslR][LW], int rs,
-                                          int wl) {
-  if (rs == 0) {
-#pragma unroll
-    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[0][c], wl);
-  } else if (rs == 1) {
-#pragma unroll
-    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[1][c], wl);
-  } else if (rs == 2) {
-#pragma unroll
-    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[2][c], wl);
-  } else {
-#pragma unroll
-    for (int c = C0; c < C1; ++c) u[c] = readlane_f(v[3][c], wl);
+// Lane `me` stores its register row v[rs] (rs wave-uniform) to LDS as eight
+// 16-byte words: a scalar branch over the slot, then the one lane's stores.
+__device__ __forceinline__ void put_row(float* dst, const float (&v)[kTslR][LW], int rs,
+                                        bool me) {
+#define EAMD_PUT(R_)                                                                   \
+  if (rs == R_) {                                                                      \
+    if (me) {                                                                          \
+      _Pragma("unroll") for (int q = 0; q < LW / 4; ++q)                               \
+        reinterpret_cast<float4*>(dst)[q] =                                            \
+            make_float4(v[R_][4 * q], v[R_][4 * q + 1], v[R_][4 * q + 2], v[R_][4 * q + 3]); \
+    }                                                                                  \
+    asm volatile("; slot " #R_ ::: "memory");                                          \
   }
+  EAMD_PUT(0) else EAMD_PUT(1) else EAMD_PUT(2) else EAMD_PUT(3)
+#undef EAMD_PUT
 }
 
 // One wave per block of up to 256 candidate rows, 4 rows per lane, all in
@@ -649,6 +645,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   __shared__ int s_off[kTslFan + 1];
   __shared__ int s_sel[LW];
   __shared__ __align__(16) int s_pp[256 + 4];
+  __shared__ __align__(16) float s_u[LW];  // the pivot row of the current column
   const int lane = threadIdx.x;
   // MODE 1: the original rows 0..w-1 (saved before the top block overwrites
   // them), loaded first and stored last
@@ -744,6 +741,21 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     const int qw = 0x1ff - (int)(wave_max_u32(key) & 0x1ffu);
     const int rs = qw >> 6, wl = qw & 63;
     const bool me = lane == wl;
+    // the pivot row: stored by its lane, read back by all (a wave's LDS
+    // accesses execute in order; the barriers keep the compiler from moving
+    // the reads above the store)
+    put_row(s_u, v, rs, me);
+    asm volatile("" ::: "memory");
+    float u[LW];
+#pragma unroll
+    for (int q = 0; q < LW / 4; ++q) {
+      const float4 t = reinterpret_cast<const float4*>(s_u)[q];
+      u[4 * q] = t.x;
+      u[4 * q + 1] = t.y;
+      u[4 * q + 2] = t.z;
+      u[4 * q + 3] = t.w;
+    }
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (rs == r && me) {
@@ -752,8 +764,6 @@ __global__ __launch_bounds__(64) void tsl_kernel(
         kq[r] = 0u;
         st[r] = j;
       }
-    float u[LW];
-    fetch_row<jj, 8>(u, v, rs, wl);
     const float pv = u[jj];
     if (pv == 0.f && first_zero < 0) first_zero = j;
     const float rinv = pv != 0.f ? __builtin_amdgcn_rcpf(pv) : 0.f;
@@ -769,21 +779,18 @@ __global__ __launch_bounds__(64) void tsl_kernel(
 #pragma unroll
       for (int c = jj + 1; c < 8; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     if (live > 8) {
-      fetch_row<8, 16>(u, v, rs, wl);
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int c = 8; c < 16; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     }
     if (live > 16) {
-      fetch_row<16, 24>(u, v, rs, wl);
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int c = 16; c < 24; ++c) v[r][c] = fmaf(-l[r], u[c], v[r][c]);
     }
     if (live > 24) {
-      fetch_row<24, 32>(u, v, rs, wl);
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll

@@ -240,6 +240,64 @@ int panel_leaf(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* l
                     zero_col, piv_offset, fz_global);
 }
 
+// FP32 LU trailing update in the requested precision (defined in mixed.hip);
+// kPrecPanelFp32 marks the FP32 updates inside a panel.
+constexpr int kPrecPanelFp32 = -1;
+int sgemm_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
+                          int64_t ld, int mrows, int ncols, int k, int precision);
+
+// Factors the panel of columns [k, k+bs) over rows [k, k+m) of the
+// row-major W: 32-column leaves, each followed by its interchanges on the
+// other panel columns, the leaf's U rows and the update of the remaining
+// panel columns (`update`, within_panel = true).  The panel's composed row
+// permutation is left in pperm_n/ppos/psrc.
+template <typename T, typename Update>
+int factor_panel(eamd_handle* h, T* W, int64_t ldw, int k, int bs, int m, int* piv_dev,
+                 int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
+                 int* zero_col, int* fz_dev, Update& update) {
+  hipStream_t s = h->stream;
+  T* Pk = W + (int64_t)k * ldw + k;
+  PhaseScope ps(h, kPhPanel, 2.0 * sizeof(T) * (double)m * bs, (double)m * bs * bs);
+  EAMD_CHECK(panel_perm_init(s, pperm_n, ppos, psrc, bs));
+  for (int j0 = 0; j0 < bs; j0 += kLeaf) {
+    const int w = min(kLeaf, bs - j0);
+    T* Lk = Pk + (int64_t)j0 * ldw + j0;
+    // transpositions stored with the panel offset, zero pivot recorded
+    EAMD_CHECK(panel_leaf<T>(h, Lk, ldw, m - j0, w, piv_dev + k + j0, lperm_n, lpos, lsrc,
+                             pperm_n, ppos, psrc, j0, bs, zero_col, k + j0, fz_dev));
+    // the leaf's interchanges on the other columns of the panel
+    EAMD_CHECK(laswp_perm_rowmajor<T>(s, Pk + (int64_t)j0 * ldw, ldw, 0, lperm_n, lpos, lsrc, w,
+                                      j0, w, bs - w));
+    const int rest = bs - j0 - w;
+    if (rest > 0) {
+      T* X = Lk + w;  // leaf rows x remaining panel columns
+      EAMD_CHECK(trsm_llu_rowmajor<T>(s, Lk, ldw, w, X, ldw, rest));
+      if (m - j0 - w > 0)
+        EAMD_CHECK(update(X, Lk + (int64_t)w * ldw, Lk + (int64_t)w * ldw + w, m - j0 - w, rest,
+                          w, true));
+    }
+  }
+  return kOk;
+}
+
+// Reads back the first zero pivot and the panel exchange error flag.
+int finish_getrf(eamd_handle* h, const int* fz_dev, int64_t* first_zero) {
+  hipStream_t s = h->stream;
+  int host_fz = -1, host_err = 0;
+  EAMD_HIP(hipMemcpyAsync(&host_fz, fz_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (h->ws.ptr[kWsPanelSync])
+    EAMD_HIP(hipMemcpyAsync(&host_err, h->ws.ptr[kWsPanelSync], sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+  EAMD_HIP(hipStreamSynchronize(s));
+  *first_zero = host_fz;
+  if (host_err) {
+    EAMD_HIP(hipMemset(h->ws.ptr[kWsPanelSync], 0, sizeof(int)));
+    set_error("panel factorization: grid exchange timed out (workgroups not co-resident?)");
+    return kErrHip;
+  }
+  return kOk;
+}
+
 // Row-major blocked LU driver.  `update(A12, A21, A22, mrows, ncols, k,
 // within_panel)` performs A22 -= A21 * A12 (row-major, ld ldw).  ord
 // (nullable, device, n ints) receives the final row order: row i of the
@@ -270,28 +328,8 @@ int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int
     const int bs = min(nb, n - k);
     const int m = n - k;
     T* Pk = W + (int64_t)k * ldw + k;
-    {
-      PhaseScope ps(h, kPhPanel, 2.0 * sizeof(T) * (double)m * bs, (double)m * bs * bs);
-      EAMD_CHECK(panel_perm_init(s, pperm_n, ppos, psrc, bs));
-      for (int j0 = 0; j0 < bs; j0 += kLeaf) {
-        const int w = min(kLeaf, bs - j0);
-        T* Lk = Pk + (int64_t)j0 * ldw + j0;
-        // transpositions stored with the panel offset, zero pivot recorded
-        EAMD_CHECK(panel_leaf<T>(h, Lk, ldw, m - j0, w, piv_dev + k + j0, lperm_n, lpos, lsrc,
-                                 pperm_n, ppos, psrc, j0, bs, zero_col, k + j0, fz_dev));
-        // the leaf's interchanges on the other columns of the panel
-        EAMD_CHECK(laswp_perm_rowmajor<T>(s, Pk + (int64_t)j0 * ldw, ldw, 0, lperm_n, lpos, lsrc,
-                                          w, j0, w, bs - w));
-        const int rest = bs - j0 - w;
-        if (rest > 0) {
-          T* X = Lk + w;  // leaf rows x remaining panel columns
-          EAMD_CHECK(trsm_llu_rowmajor<T>(s, Lk, ldw, w, X, ldw, rest));
-          if (m - j0 - w > 0)
-            EAMD_CHECK(update(X, Lk + (int64_t)w * ldw, Lk + (int64_t)w * ldw + w, m - j0 - w,
-                              rest, w, true));
-        }
-      }
-    }
+    EAMD_CHECK(factor_panel<T>(h, W, ldw, k, bs, m, piv_dev, lperm_n, lpos, lsrc, pperm_n, ppos,
+                               psrc, zero_col, fz_dev, update));
     {
       // the panel's interchanges on the columns outside it: [0,k) and [k+bs,n)
       PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(T) * bs * (double)(n - bs), 0.0);
@@ -312,27 +350,151 @@ int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int
       EAMD_CHECK(update(A12, A21, A22, m - bs, tn, bs, false));
     }
   }
-  // read back the first zero pivot and the panel exchange error flag
-  int host_fz = -1, host_err = 0;
-  EAMD_HIP(hipMemcpyAsync(&host_fz, fz_dev, sizeof(int), hipMemcpyDeviceToHost, s));
-  if (h->ws.ptr[kWsPanelSync])
-    EAMD_HIP(hipMemcpyAsync(&host_err, h->ws.ptr[kWsPanelSync], sizeof(int),
-                            hipMemcpyDeviceToHost, s));
-  EAMD_HIP(hipStreamSynchronize(s));
-  *first_zero = host_fz;
-  if (host_err) {
-    EAMD_HIP(hipMemset(h->ws.ptr[kWsPanelSync], 0, sizeof(int)));
-    set_error("panel factorization: grid exchange timed out (workgroups not co-resident?)");
-    return kErrHip;
+  return finish_getrf(h, fz_dev, first_zero);
+}
+
+// Sets the handle's stream for the lifetime of the object (launchers and
+// phase timers use h->stream).
+struct StreamSwap {
+  eamd_handle* h;
+  hipStream_t saved;
+  StreamSwap(eamd_handle* h_, hipStream_t s) : h(h_), saved(h_->stream) { h->stream = s; }
+  ~StreamSwap() { h->stream = saved; }
+};
+
+// FP32 LU with a mixed-precision trailing update and a look-ahead of one
+// panel.  The handle's stream factors panel k, then (after the rest of panel
+// k-1's update) applies panel k's interchanges, U rows and update to the
+// next panel's columns only, and goes on to factor that panel; a second
+// stream applies panel k's interchanges to the columns left of it and
+// beyond the next panel and does their U rows and update, which so overlap
+// the next panel's factorization.  Packed panel columns and permutation
+// lists alternate between two buffers (panel k's are read by the second
+// stream while panel k+1 is factored).
+int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* W, int64_t ldw,
+                         int n, int* piv_dev, int* ord, int nb, int precision);
+int getrf_lookahead_mixed(eamd_handle* h, float* W, int64_t ldw, int n, int* piv_dev, int* ord,
+                          int64_t* first_zero, int nb, int precision) {
+  if (nb > 256) nb = 256;
+  if (nb < kLeaf) nb = kLeaf;
+  EAMD_CHECK(ensure_side_stream(h));
+  // the panels run on the reserved-CU stream when there is one
+  const hipStream_t user = h->stream;
+  const hipStream_t sa = h->pstream ? h->pstream : user, sb = h->side;
+  if (sa != user) {
+    EAMD_HIP(hipEventRecord(h->la_ev[4], user));
+    EAMD_HIP(hipStreamWaitEvent(sa, h->la_ev[4], 0));
+  }
+  int st = getrf_lookahead_body(h, sa, sb, W, ldw, n, piv_dev, ord, nb, precision);
+  if (sa != user) {
+    // join, also after a failure
+    EAMD_HIP(hipEventRecord(h->la_ev[5], sa));
+    EAMD_HIP(hipStreamWaitEvent(user, h->la_ev[5], 0));
+  }
+  EAMD_CHECK(st);
+  return finish_getrf(h, static_cast<int*>(h->ws.ptr[kWsPiv]) + 2, first_zero);
+}
+
+int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* W, int64_t ldw,
+                         int n, int* piv_dev, int* ord, int nb, int precision) {
+  StreamSwap swa(h, sa);
+  void* aux = nullptr;
+  // aux ints: lperm_n | zero_col | first_zero | pperm_n[2] | pad | lpos[2*32] |
+  //           lsrc[2*32] | (ppos[2nb], psrc[2nb]) x 2
+  EAMD_CHECK(h->ws.get(kWsPiv, sizeof(int) * (8 + 4 * kLeaf + 8 * (size_t)nb), &aux));
+  int* ai = static_cast<int*>(aux);
+  int* lperm_n = ai;
+  int* zero_col = ai + 1;
+  int* fz_dev = ai + 2;
+  int* lpos = ai + 8;
+  int* lsrc = lpos + 2 * kLeaf;
+  int* pn[2] = {ai + 3, ai + 4};
+  int* pp[2] = {lsrc + 2 * kLeaf, lsrc + 2 * kLeaf + 4 * nb};
+  int* psl[2] = {pp[0] + 2 * nb, pp[1] + 2 * nb};
+  // all buffers sized before the second stream has work (a growing
+  // workspace synchronizes the device)
+  const size_t opL = mixed_operand_bytes(n, nb, precision);
+  void* pl = nullptr;
+  void* pu = nullptr;
+  EAMD_CHECK(h->ws.get(kWsPackL, 2 * opL, &pl));
+  EAMD_CHECK(h->ws.get(kWsPackU, 2 * opL, &pu));
+  char* packL[2] = {static_cast<char*>(pl), static_cast<char*>(pl) + opL};
+  char* packU[2] = {static_cast<char*>(pu), static_cast<char*>(pu) + opL};
+  auto in_panel = [&](float* A12, float* A21, float* A22, int mrows, int ncols, int k,
+                      bool) -> int {
+    return sgemm_update_rowmajor(h, A12, A21, A22, ldw, mrows, ncols, k, kPrecPanelFp32);
+  };
+  hipEvent_t evP = h->la_ev[0], evB[2] = {h->la_ev[1], h->la_ev[2]};
+  // the concurrent update keeps to a grid of EAMD_UPDATE_WGS workgroups (one
+  // per CU at most, by its LDS), leaving the other CUs to the panel kernels
+  static const int update_wgs = env_int("EAMD_UPDATE_WGS", 0);
+  EAMD_HIP(hipMemsetAsync(fz_dev, 0xff, sizeof(int), sa));  // -1
+  if (ord) EAMD_CHECK(init_row_order(sa, ord, n));
+  bool side_used = false;
+  for (int k = 0, it = 0; k < n; k += nb, ++it) {
+    const int par = it & 1;
+    const int bs = min(nb, n - k), m = n - k;
+    float* Pk = W + (int64_t)k * ldw + k;
+    EAMD_CHECK(factor_panel<float>(h, W, ldw, k, bs, m, piv_dev, lperm_n, lpos, lsrc, pn[par],
+                                   pp[par], psl[par], zero_col, fz_dev, in_panel));
+    if (ord) EAMD_CHECK(update_row_order(sa, ord, k, pn[par], pp[par], psl[par], bs));
+    // the next panel's columns were last written by the previous panel's
+    // second-stream update
+    if (it > 0) EAMD_HIP(hipStreamWaitEvent(sa, evB[par ^ 1], 0));
+    const int tn = n - k - bs, n1 = min(nb, tn), n2 = tn - n1;
+    if (tn == 0) {  // last panel: its interchanges on the columns to its left
+      PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)k, 0.0);
+      EAMD_CHECK(laswp_perm_rowmajor<float>(sa, W + (int64_t)k * ldw, ldw, 0, pn[par], pp[par],
+                                            psl[par], bs, k, bs, k));
+      break;
+    }
+    float* A12n = W + (int64_t)k * ldw + (k + bs);
+    float* A21 = W + (int64_t)(k + bs) * ldw + k;
+    {
+      PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)n1, 0.0);
+      EAMD_CHECK(laswp_perm_rowmajor<float>(sa, A12n, ldw, 0, pn[par], pp[par], psl[par], bs, n1,
+                                            0, n1));
+    }
+    {
+      PhaseScope ps(h, kPhTrsm, 4.0 * (2.0 * bs * n1 + 0.5 * bs * bs), (double)bs * bs * n1);
+      EAMD_CHECK(trsm_llu_rowmajor<float>(sa, Pk, ldw, bs, A12n, ldw, n1));
+    }
+    MixedOperand L, Un, Ur;
+    EAMD_CHECK(mixed_pack(h, true, A21, ldw, m - bs, bs, precision, packL[par], &L));
+    EAMD_HIP(hipEventRecord(evP, sa));
+    EAMD_CHECK(mixed_pack(h, false, A12n, ldw, n1, bs, precision, packU[0], &Un));
+    EAMD_CHECK(mixed_update_packed(h, Un, n1, L, m - bs, bs, precision,
+                                   W + (int64_t)(k + bs) * ldw + (k + bs), ldw));
+    // second stream: the other columns
+    EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));
+    side_used = true;
+    {
+      StreamSwap sw(h, sb);
+      {
+        PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)(k + n2), 0.0);
+        EAMD_CHECK(laswp_perm_rowmajor<float>(sb, W + (int64_t)k * ldw, ldw, 0, pn[par], pp[par],
+                                              psl[par], bs, k, bs + n1, k + n2));
+      }
+      if (n2 > 0) {
+        float* A12r = A12n + n1;
+        {
+          PhaseScope ps(h, kPhTrsm, 4.0 * (2.0 * bs * n2 + 0.5 * bs * bs), (double)bs * bs * n2);
+          EAMD_CHECK(trsm_llu_rowmajor<float>(sb, Pk, ldw, bs, A12r, ldw, n2));
+        }
+        EAMD_CHECK(mixed_pack(h, false, A12r, ldw, n2, bs, precision, packU[1], &Ur));
+        EAMD_CHECK(mixed_update_packed(h, Ur, n2, L, m - bs, bs, precision,
+                                       W + (int64_t)(k + bs) * ldw + (k + bs + n1), ldw,
+                                       update_wgs));
+      }
+      EAMD_HIP(hipEventRecord(evB[par], sb));
+    }
+  }
+  if (side_used) {
+    EAMD_HIP(hipEventRecord(h->la_ev[3], sb));
+    EAMD_HIP(hipStreamWaitEvent(sa, h->la_ev[3], 0));
   }
   return kOk;
 }
-
-// FP32 LU trailing update in the requested precision (defined in mixed.hip);
-// kPrecPanelFp32 marks the FP32 updates inside a panel.
-constexpr int kPrecPanelFp32 = -1;
-int sgemm_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
-                          int64_t ld, int mrows, int ncols, int k, int precision);
 
 namespace {
 // row-major A22 -= A21*A12  <=>  column-major A22^T -= A12^T * A21^T
@@ -379,6 +541,10 @@ int getrf_rowmajor_order(eamd_handle* h, void* W, bool fp64, int64_t ldw, int n,
     return getrf_rowmajor_t<double>(h, Wd, ldw, n, piv, ord, first_zero, nb, upd);
   }
   float* Wf = static_cast<float*>(W);
+  static const bool lookahead = env_int("EAMD_LOOKAHEAD", 1) != 0;
+  if (lookahead && (precision == EAMD_PREC_FP16X3 || precision == EAMD_PREC_FP16 ||
+                    precision == EAMD_PREC_BF16))
+    return getrf_lookahead_mixed(h, Wf, ldw, n, piv, ord, first_zero, nb, precision);
   auto upd = [&](float* A12, float* A21, float* A22, int mrows, int ncols, int k,
                  bool within) -> int {
     return sgemm_update_rowmajor(h, A12, A21, A22, ldw, mrows, ncols, k,

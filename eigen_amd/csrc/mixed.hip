@@ -206,8 +206,12 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
   __shared__ __align__(16) unsigned short lds[2][NARR][MT * LROW];
 
   const int tiles_m = (M + MT - 1) / MT, tiles_n = (N + MT - 1) / MT;
+  const int ntiles = tiles_m * tiles_n;
+  // a grid smaller than the tile count loops (a multiple of 8 workgroups
+  // keeps each tile on the XCD the mapping assigns it)
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
   int tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  tile_coords(t, ntiles, tiles_m, tiles_n, tm, tn);
   const int m0 = tm * MT, n0 = tn * MT;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -320,6 +324,7 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
         C[(int64_t)gj * ldc + gi] = cv[i][j][r] - acc[i][j][r] * inv;
       }
   }
+  }
 }
 
 // FP32 update on the FP32 matrix cores (v_mfma_f32_16x16x4f32), operands
@@ -407,64 +412,106 @@ __global__ __launch_bounds__(256) void f32_update_kernel(int M, int N, int K,
 }  // namespace
 
 // row-major A22 (mrows x ncols) -= A21 (mrows x k) * A12 (k x ncols), ld
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+int mixed_mode(int precision) {
+  return precision == EAMD_PREC_FP16X3 ? kModeF16x3
+         : precision == EAMD_PREC_BF16 ? kModeBF16 : kModeF16;
+}
+}  // namespace
+
+size_t mixed_operand_bytes(int len, int k, int precision) {
+  const size_t KP = (size_t)ceil_div(k, KS) * KS, LB = (size_t)ceil_div(len, MT);
+  const size_t narr = mixed_mode(precision) == kModeF16x3 ? 2 : 1;
+  return narr * align256(sizeof(unsigned short) * LB * MT * KP) + align256(sizeof(float) * LB);
+}
+
+// Packs one operand of the mixed update into buf (mixed_operand_bytes):
+// rows = true takes `len` rows of K values (A21), rows = false takes `len`
+// columns of K rows (A12), both row-major with leading dimension ld.
+int mixed_pack(eamd_handle* h, bool rows, const float* S, int64_t ld, int len, int k,
+               int precision, void* buf, MixedOperand* out) {
+  if (len <= 0 || k <= 0) return kOk;
+  const int mode = mixed_mode(precision);
+  const int KP = ceil_div(k, KS) * KS, LB = ceil_div(len, MT);
+  const size_t arr = align256(sizeof(unsigned short) * (size_t)LB * MT * KP);
+  char* c = static_cast<char*>(buf);
+  out->hi = reinterpret_cast<unsigned short*>(c);
+  out->lo = mode == kModeF16x3 ? reinterpret_cast<unsigned short*>(c + arr) : out->hi;
+  out->scale = reinterpret_cast<float*>(c + (mode == kModeF16x3 ? 2 : 1) * arr);
+  const int narr = mode == kModeF16x3 ? 2 : 1;
+  PhaseScope ps(h, kPhPack, (4.0 + 2.0 * narr) * (double)k * len, 0.0);
+#define EAMD_PACK(MODE)                                                                    \
+  if (rows)                                                                                \
+    hipLaunchKernelGGL(pack_rows_kernel<MODE>, dim3(LB), dim3(256), 0, h->stream, S, ld, len, \
+                       k, KP, out->hi, out->lo, out->scale);                               \
+  else                                                                                     \
+    hipLaunchKernelGGL(pack_cols_kernel<MODE>, dim3(LB), dim3(256), 0, h->stream, S, ld, len, \
+                       k, KP, out->hi, out->lo, out->scale);
+  if (mode == kModeF16x3) {
+    EAMD_PACK(kModeF16x3)
+  } else if (mode == kModeBF16) {
+    EAMD_PACK(kModeBF16)
+  } else {
+    EAMD_PACK(kModeF16)
+  }
+#undef EAMD_PACK
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+// A22 (N rows x M columns, row-major, ld) -= A21 A12 from packed operands:
+// a = A12 (M columns), b = A21 (N rows), inner dimension k.
+int mixed_update_packed(eamd_handle* h, const MixedOperand& a, int M, const MixedOperand& b, int N,
+                        int k, int precision, float* A22, int64_t ld, int max_wgs) {
+  if (M <= 0 || N <= 0 || k <= 0) return kOk;
+  const int mode = mixed_mode(precision);
+  const int KP = ceil_div(k, KS) * KS;
+  int tiles = ceil_div(M, MT) * ceil_div(N, MT);
+  if (max_wgs > 0 && tiles > max_wgs) tiles = max_wgs & ~7;
+  const int narr = mode == kModeF16x3 ? 2 : 1;
+  // algorithmic traffic: FP32 C read + write, packed operands read once
+  PhaseScope ps(h, kPhUpdate, 8.0 * M * N + 2.0 * narr * (double)KP * (M + N),
+                2.0 * M * N * (double)k);
+#define EAMD_UPD(MODE)                                                                         \
+  hipLaunchKernelGGL(mixed_update_kernel<MODE>, dim3(tiles), dim3(256), 0, h->stream, M, N, KP, \
+                     a.hi, a.lo, b.hi, b.lo, a.scale, b.scale, A22, ld);
+  if (mode == kModeF16x3) {
+    EAMD_UPD(kModeF16x3)
+  } else if (mode == kModeBF16) {
+    EAMD_UPD(kModeBF16)
+  } else {
+    EAMD_UPD(kModeF16)
+  }
+#undef EAMD_UPD
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
 int sgemm_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
                           int64_t ld, int mrows, int ncols, int k, int precision) {
   if (mrows <= 0 || ncols <= 0 || k <= 0) return kOk;
   hipStream_t s = h->stream;
   const int M = ncols, N = mrows;
-  const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
   if (precision == EAMD_PREC_FP32 || precision == EAMD_PREC_FP64 || precision < 0) {
     // precision < 0: an update inside the panel (FP32, accounted to the panel)
     const bool in_panel = precision < 0;
     PhaseScope ps(h, in_panel ? kPhPanel : kPhUpdate,
                   in_panel ? 0.0 : 4.0 * (2.0 * M * N + (double)k * (M + N)),
                   in_panel ? 0.0 : 2.0 * M * N * k);
+    const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
     hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, s, M, N, k, A12, A21, ld,
                        A22);
     EAMD_HIP(hipGetLastError());
     return kOk;
   }
-  const int mode = precision == EAMD_PREC_FP16X3 ? kModeF16x3
-                   : precision == EAMD_PREC_BF16 ? kModeBF16 : kModeF16;
-  const int KP = ceil_div(k, KS) * KS;
-  const int MB = ceil_div(M, MT), NB = ceil_div(N, MT);
-  const size_t na = (size_t)MB * MT * KP, nbh = (size_t)NB * MT * KP;
-  const int narr = mode == kModeF16x3 ? 2 : 1;
-  size_t need = sizeof(unsigned short) * narr * (na + nbh) + sizeof(float) * (MB + NB) + 1024;
+  const size_t oa = mixed_operand_bytes(M, k, precision), ob = mixed_operand_bytes(N, k, precision);
   void* p = nullptr;
-  EAMD_CHECK(h->ws.get(kWsSplit, need, &p));
-  unsigned short* Pah = static_cast<unsigned short*>(p);
-  unsigned short* Pbh = Pah + na;
-  unsigned short* Pal = Pbh + nbh;
-  unsigned short* Pbl = mode == kModeF16x3 ? Pal + na : Pal;
-  float* sa = reinterpret_cast<float*>(Pah + (size_t)narr * (na + nbh));
-  float* sb = sa + MB;
-  // algorithmic traffic: FP32 C read + write, packed operands read once
-  const double upd_bytes = 8.0 * M * N + 2.0 * narr * (double)KP * (M + N);
-  const double pack_bytes = (4.0 + 2.0 * narr) * (double)k * (M + N);
-#define EAMD_MIXED(MODE)                                                                        \
-  {                                                                                             \
-    PhaseScope ps(h, kPhPack, pack_bytes, 0.0);                                                 \
-    hipLaunchKernelGGL(pack_cols_kernel<MODE>, dim3(MB), dim3(256), 0, s, A12, ld, M, k, KP,    \
-                       Pah, Pal, sa);                                                           \
-    hipLaunchKernelGGL(pack_rows_kernel<MODE>, dim3(NB), dim3(256), 0, s, A21, ld, N, k, KP,    \
-                       Pbh, Pbl, sb);                                                           \
-  }                                                                                             \
-  {                                                                                             \
-    PhaseScope ps(h, kPhUpdate, upd_bytes, 2.0 * M * N * (double)k);                            \
-    hipLaunchKernelGGL(mixed_update_kernel<MODE>, dim3(tiles), dim3(256), 0, s, M, N, KP, Pah,  \
-                       Pal, Pbh, Pbl, sa, sb, A22, ld);                                         \
-  }
-  if (mode == kModeF16x3) {
-    EAMD_MIXED(kModeF16x3);
-  } else if (mode == kModeBF16) {
-    EAMD_MIXED(kModeBF16);
-  } else {
-    EAMD_MIXED(kModeF16);
-  }
-#undef EAMD_MIXED
-  EAMD_HIP(hipGetLastError());
-  return kOk;
+  EAMD_CHECK(h->ws.get(kWsSplit, oa + ob, &p));
+  MixedOperand a, b;
+  EAMD_CHECK(mixed_pack(h, false, A12, ld, M, k, precision, p, &a));
+  EAMD_CHECK(mixed_pack(h, true, A21, ld, N, k, precision, static_cast<char*>(p) + oa, &b));
+  return mixed_update_packed(h, a, M, b, N, k, precision, A22, ld);
 }
 
 }  // namespace eamd
