@@ -243,12 +243,11 @@ __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int
 
 // All lists of a leaf from its interchanges (one wave): piv[] (plus
 // piv_offset), the leaf-level list lperm_n/lpos/lsrc and the panel-level one.
-// FROM_SEL as for leaf_perm; tpos/trow return the leaf list's entries.
+// FROM_SEL as for leaf_perm; tpos/trow/tsrc return the leaf list's entries.
 template <bool FROM_SEL>
 __device__ void leaf_lists(int x, int w, int row0, int pnb, int lane, int* piv, int piv_offset,
                            int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos,
-                           int* psrc, int* s_pp, int& tpos, int& trow, int& ln) {
-  int tsrc;
+                           int* psrc, int* s_pp, int& tpos, int& trow, int& tsrc, int& ln) {
   leaf_perm<FROM_SEL>(x, w, lane, tpos, trow, tsrc, ln);
   if (lane < w) piv[lane] = x + piv_offset;
   if (lane < ln) {
@@ -505,9 +504,9 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
   // permutation lists, replaying the interchanges in column order
   if (g == 0 && wave == 0) {
-    int tpos, trow, ln;
+    int tpos, trow, tsrc, ln;
     leaf_lists<false>(lane < w ? s_piv[lane] : 0, w, row0, pnb, lane, piv, piv_offset, lperm_n,
-                      lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, ln);
+                      lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, tsrc, ln);
   }
   if (valid) {
     T* dst = P + (int64_t)i * ldp;
@@ -640,7 +639,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     int* __restrict__ piv, int* __restrict__ lperm_n, int* __restrict__ lpos,
     int* __restrict__ lsrc, int* __restrict__ pperm_n, int* __restrict__ ppos,
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
-    int* __restrict__ fz_global) {
+    int* __restrict__ fz_global, int fuse, float* __restrict__ xtop) {
   constexpr int R = kTslR;
   __shared__ int s_off[kTslFan + 1];
   __shared__ int s_sel[LW];
@@ -816,6 +815,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     if (lane == 0) cand_n[blockIdx.x] = steps;
     return;
   }
+  if constexpr (MODE == 1) {
 #pragma unroll 1
   for (int q = ngroups; q < 4; ++q)
 #pragma unroll
@@ -837,9 +837,9 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     }
   __syncthreads();
   // ---- transpositions and permutation lists from the selection order
-  int tpos, trow, ln;
+  int tpos, trow, tsrc, ln;
   leaf_lists<true>(lane < w ? s_sel[lane] : 0, w, row0, pnb, lane, piv, piv_offset, lperm_n,
-                   lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, ln);
+                   lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, tsrc, ln);
   // positions >= w now hold displaced original rows < w: list them
   const bool mv = tpos >= w && lane < ln;
   const u64 mb = __ballot(mv);
@@ -849,9 +849,44 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     moved[2 + 2 * idx] = trow;
   }
   if (lane == 0) moved[0] = __popcll(mb);
+  if (fuse) {
+    // (w == 32) The leaf's interchanges on the other columns of the panel
+    // (row0 = j0 is the leaf's first panel column, pnb the panel width): the
+    // lane of each touched row moves its source row's values, 32 columns at
+    // a time (all loads of a chunk before its stores); the top rows' values
+    // right of the leaf go to xtop (LW x (pnb - j0 - LW)), from which the
+    // apply kernel solves U12.
+    const int srow = __shfl(tpos, tsrc);
+    const int j0 = row0, nc = pnb - LW, rest = pnb - j0 - LW;
+    float* base = P - j0;  // panel column 0 of leaf row 0
+    if (lane < ln) {
+      const float* sp = base + (int64_t)srow * ldp;
+      float* dp = base + (int64_t)tpos * ldp;
+      float* xp = tpos < LW ? xtop + tpos * rest - j0 : nullptr;
+      for (int c0 = 0; c0 < nc; c0 += 32) {
+        float x[32];
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+          const int c = min(c0 + t, nc - 1), col = c < j0 ? c : c + LW;
+          x[t] = sp[col];
+        }
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+          const int c = c0 + t, col = c < j0 ? c : c + LW;
+          if (c < nc) {
+            if (xp && c >= j0)
+              xp[c] = x[t];
+            else
+              dp[col] = x[t];
+          }
+        }
+      }
+    }
+  }
   if (lane == 0) {
     *zero_col = first_zero;
     if (fz_global && first_zero >= 0 && *fz_global < 0) *fz_global = first_zero + piv_offset;
+  }
   }
 }
 
@@ -905,6 +940,99 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
     if (FULL || c < w) dst[c] = a[c];
 }
 
+
+// As tsl_apply_kernel (w == 32), and the row's `rest` panel columns right of
+// the leaf take the update x -= l U12 (the rows' values already permuted).
+// Every workgroup solves U12 = L11^-1 xtop (the permuted top rows right of
+// the leaf) in LDS, one column per thread; workgroup 0 also stores it as the
+// top rows.  Four threads per row: each computes l and updates every fourth
+// column.
+constexpr int kApplyTpr = 4;
+__global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__ P, int64_t ldp,
+                                                             int m, const float* __restrict__ saved,
+                                                             const int* __restrict__ moved,
+                                                             const float* __restrict__ xtop,
+                                                             int rest) {
+  extern __shared__ float s_u12[];  // LW x rest
+  __shared__ float U[LW][LW + 1];
+  __shared__ float dinv[LW];
+  __shared__ int mpos[LW], msrc[LW];
+  __shared__ int s_nm;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < LW * LW; e += 256) {
+    const int r = e / LW, c = e - r * LW;
+    U[r][c] = P[(int64_t)r * ldp + c];
+  }
+  if (tid == 0) s_nm = moved[0];
+  if (tid < LW) {
+    mpos[tid] = moved[1 + 2 * tid];
+    msrc[tid] = moved[2 + 2 * tid];
+  }
+  __syncthreads();
+  if (tid < LW) {
+    const float d = U[tid][tid];
+    dinv[tid] = d != 0.f ? 1.0f / d : 1.0f;  // a zero pivot leaves its column unscaled
+  }
+  for (int c = tid; c < rest; c += 256) {
+    float u[LW];
+#pragma unroll
+    for (int i = 0; i < LW; ++i) {
+      float sum = xtop[i * rest + c];
+#pragma unroll
+      for (int k = 0; k < i; ++k) sum -= U[i][k] * u[k];
+      u[i] = sum;
+      s_u12[i * rest + c] = sum;
+    }
+    if (blockIdx.x == 0)
+#pragma unroll
+      for (int i = 0; i < LW; ++i) P[(int64_t)i * ldp + LW + c] = u[i];
+  }
+  __syncthreads();
+  const int q = tid % kApplyTpr;
+  const int i = LW + blockIdx.x * (256 / kApplyTpr) + tid / kApplyTpr;
+  if (i >= m) return;
+  int src = -1;
+  for (int k = 0; k < s_nm; ++k)
+    if (mpos[k] == i) src = msrc[k];
+  float a[LW];
+  const float* arow = src >= 0 ? saved + src * LW : P + (int64_t)i * ldp;
+#pragma unroll
+  for (int c = 0; c < LW; ++c) a[c] = arow[c];
+#pragma unroll
+  for (int k = 0; k < LW; ++k) {
+    const float x = a[k] * dinv[k];
+    a[k] = x;
+#pragma unroll
+    for (int c = k + 1; c < LW; ++c) a[c] -= x * U[k][c];
+  }
+  float* row = P + (int64_t)i * ldp;
+  if (q == 0) {
+#pragma unroll
+    for (int c = 0; c < LW; ++c) row[c] = a[c];
+  }
+  row += LW;
+  for (int c0 = q; c0 < rest; c0 += kApplyTpr * 8) {
+    float x[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int c = c0 + kApplyTpr * t;
+      x[t] = c < rest ? row[c] : 0.f;
+    }
+#pragma unroll 4
+    for (int k = 0; k < LW; ++k)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int c = min(c0 + kApplyTpr * t, rest - 1);
+        x[t] -= a[k] * s_u12[k * rest + c];
+      }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int c = c0 + kApplyTpr * t;
+      if (c < rest) row[c] = x[t];
+    }
+  }
+}
+
 }  // namespace
 
 // Same outputs as leaf_lu; FP32 only.  Panels of up to 256 rows take the
@@ -912,8 +1040,14 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
 // most 8 remain for the final round.
 int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
                        int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
-                       int row0, int pnb, int* zero_col, int piv_offset, int* fz_global) {
+                       int row0, int pnb, int* zero_col, int piv_offset, int* fz_global,
+                       bool* fused) {
   const int G1 = ceil_div(m, kTslBlock);
+  // the leaf's interchanges on the other panel columns, the U rows right of
+  // it and their update are done here too when w == 32 (the caller skips
+  // its own)
+  const bool fuse = fused && w == LW && G1 > 1;
+  if (fused) *fused = fuse;
   if (G1 <= 1)
     return leaf_lu<float>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
                           pnb, zero_col, piv_offset, fz_global);
@@ -924,8 +1058,8 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   void* base = nullptr;
   // two candidate buffers (ping-pong between merge levels), counts, moved
   // list and the saved rows
-  const size_t need =
-      sizeof(int) * (2 * (size_t)G1 * LW + 2 * (size_t)G1 + 2 * LW + 8) + sizeof(float) * LW * LW;
+  const size_t need = sizeof(int) * (2 * (size_t)G1 * LW + 2 * (size_t)G1 + 2 * LW + 8) +
+                      sizeof(float) * LW * LW + sizeof(float) * LW * 256;
   EAMD_CHECK(h->ws.get(kWsTsl, need, &base));
   int* candA = static_cast<int*>(base);
   int* candB = candA + (size_t)G1 * LW;
@@ -933,25 +1067,36 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   int* nB = nA + G1;
   int* moved = nB + G1;
   float* saved = reinterpret_cast<float*>(moved + 2 * LW + 8);
+  float* xtop = saved + LW * LW;  // permuted top rows right of the leaf (fused)
   hipStream_t s = h->stream;
   hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(64), 0, s, P, ldp, m, w, nullptr,
                      nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos,
-                     psrc, row0, pnb, zero_col, piv_offset, fz_global);
+                     psrc, row0, pnb, zero_col, piv_offset, fz_global, 0, xtop);
   int G = G1;
   int *cin = candA, *nin = nA, *cout = candB, *nout = nB;
   while (G > kTslFan) {
     const int Gn = ceil_div(G, kTslFan);
     hipLaunchKernelGGL(tsl_kernel<2>, dim3(Gn), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
                        cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
-                       row0, pnb, zero_col, piv_offset, fz_global);
+                       row0, pnb, zero_col, piv_offset, fz_global, 0, xtop);
     G = Gn;
     std::swap(cin, cout);
     std::swap(nin, nout);
   }
   hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
                      cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
-                     pnb, zero_col, piv_offset, fz_global);
-  if (m > w) {
+                     pnb, zero_col, piv_offset, fz_global, fuse ? 1 : 0, xtop);
+  const int rest = pnb - row0 - w;
+  if (fuse && rest > 0) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      EAMD_HIP(hipFuncSetAttribute((const void*)tsl_apply_rest_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(tsl_apply_rest_kernel, dim3(ceil_div(m - w, 256 / kApplyTpr)), dim3(256),
+                       sizeof(float) * LW * rest, s, P, ldp, m, saved, moved, xtop, rest);
+  } else if (m > w) {
     if (w == LW)
       hipLaunchKernelGGL(tsl_apply_kernel<true>, dim3(ceil_div(m - w, 256)), dim3(256), 0, s, P,
                          ldp, m, w, saved, moved);

@@ -218,23 +218,30 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
 
 int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
                        int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
-                       int row0, int pnb, int* zero_col, int piv_offset, int* fz_global);
+                       int row0, int pnb, int* zero_col, int piv_offset, int* fz_global,
+                       bool* fused);
 int env_int(const char* name, int dflt);
 
 constexpr int kLeaf = 32;
 
 // FP32 leaves use tournament pivoting (the mixed-precision solver judges
 // its factor by the refined backward error); EAMD_TOURNAMENT=0 selects
-// partial pivoting, as the FP64 factorization always uses.
+// partial pivoting, as the FP64 factorization always uses.  *fused (set to
+// false unless the tournament leaf also did them) reports whether the
+// leaf's interchanges on the other panel columns, the U rows right of it
+// and their update are done.
 template <typename T>
 int panel_leaf(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n,
                int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb,
-               int* zero_col, int piv_offset, int* fz_global) {
+               int* zero_col, int piv_offset, int* fz_global, bool* fused) {
+  *fused = false;
   if constexpr (std::is_same<T, float>::value) {
     static const bool tournament = env_int("EAMD_TOURNAMENT", 1) != 0;
+    static const bool fuse = env_int("EAMD_FUSE_PANEL", 1) != 0;
     if (tournament)
       return leaf_lu_tournament(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
-                                row0, pnb, zero_col, piv_offset, fz_global);
+                                row0, pnb, zero_col, piv_offset, fz_global,
+                                fuse ? fused : nullptr);
   }
   return leaf_lu<T>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb,
                     zero_col, piv_offset, fz_global);
@@ -263,8 +270,10 @@ int factor_panel(eamd_handle* h, T* W, int64_t ldw, int k, int bs, int m, int* p
     const int w = min(kLeaf, bs - j0);
     T* Lk = Pk + (int64_t)j0 * ldw + j0;
     // transpositions stored with the panel offset, zero pivot recorded
+    bool fused = false;
     EAMD_CHECK(panel_leaf<T>(h, Lk, ldw, m - j0, w, piv_dev + k + j0, lperm_n, lpos, lsrc,
-                             pperm_n, ppos, psrc, j0, bs, zero_col, k + j0, fz_dev));
+                             pperm_n, ppos, psrc, j0, bs, zero_col, k + j0, fz_dev, &fused));
+    if (fused) continue;
     // the leaf's interchanges on the other columns of the panel
     EAMD_CHECK(laswp_perm_rowmajor<T>(s, Pk + (int64_t)j0 * ldw, ldw, 0, lperm_n, lpos, lsrc, w,
                                       j0, w, bs - w));
