@@ -207,10 +207,20 @@ __device__ __forceinline__ void leaf_perm(int& x, int w, int lane, int& tpos, in
 // appended slots; slot s: row ppos[s] receives the content of slot psrc[s]).
 // All entries at once: a touched slot's new source is the old source of its
 // entry's source slot, and a newly appended slot's old source is itself.
-// One wave; s_pp is LDS scratch of at least pnb + 4 ints.
+// The panel's first leaf (row0 == 0) starts the list as the identity.  One
+// wave; s_pp is LDS scratch of at least pnb + 4 ints.
 __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int lane,
                               int* pperm_n, int* ppos, int* psrc, int* s_pp) {
-  const int pn0 = *pperm_n, na = pn0 - pnb, na4 = (na + 3) & ~3;
+  const bool first = row0 == 0;
+  if (first) {
+    for (int e = lane; e < pnb; e += 64) {
+      ppos[e] = e;
+      psrc[e] = e;
+    }
+    // complete before the touched slots are stored over them
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  }
+  const int pn0 = first ? pnb : *pperm_n, na = pn0 - pnb, na4 = (na + 3) & ~3;
   for (int e = lane; e < na4; e += 64) s_pp[e] = e < na ? ppos[pnb + e] : -1;
   asm volatile("" ::: "memory");  // LDS accesses of one wave execute in order
   const bool mine = lane < ln;
@@ -232,7 +242,7 @@ __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int
   const u64 nb = __ballot(need);
   if (need) slot = pn0 + __popcll(nb & ((1ull << lane) - 1ull));
   int old = slot;
-  if (mine && !need) old = psrc[slot];
+  if (mine && !need && !first) old = psrc[slot];
   const int src_old = __shfl(old, tsrc);
   if (mine) {
     psrc[slot] = src_old;
@@ -520,18 +530,6 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   }
 }
 
-namespace {
-__global__ void panel_perm_init_kernel(int* pn, int* ppos, int* psrc, int nb) {
-  for (int s = threadIdx.x; s < nb; s += blockDim.x) { ppos[s] = s; psrc[s] = s; }
-  if (threadIdx.x == 0) *pn = nb;
-}
-}  // namespace
-
-int panel_perm_init(hipStream_t s, int* pn, int* ppos, int* psrc, int nb) {
-  hipLaunchKernelGGL(panel_perm_init_kernel, dim3(1), dim3(256), 0, s, pn, ppos, psrc, nb);
-  EAMD_HIP(hipGetLastError());
-  return kOk;
-}
 
 // Launches one leaf over rows [0, m) of the sub-panel at P (row-major, ldp),
 // columns [0, w), w <= 32.

@@ -60,10 +60,16 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(T* __restrict__ W, int6
 // Per 32-row diagonal block: one thread per column solves the block from
 // registers (compile-time indices, L broadcast from LDS, no barrier inside),
 // then all threads update the rows below with 32-long dot products.
-template <typename T, int CB>
+// PERM: the panel's row permutation (pos/src/perm_n: row pos[s] receives
+// the content of slot src[s]; slots 0..nb-1 are rows 0..nb-1) is applied to
+// these columns first: all touched rows are staged in LDS, the rows below
+// the top nb are written back permuted and the top nb enter the solve.
+template <typename T, int CB, bool PERM>
 __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, int64_t ldl,
                                                        int nb, T* __restrict__ X, int64_t ldx,
-                                                       int ncols) {
+                                                       int ncols, const int* __restrict__ perm_n,
+                                                       const int* __restrict__ pos,
+                                                       const int* __restrict__ src) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int XP = CB + 1;
   T* Xs = reinterpret_cast<T*>(smem_raw);  // nb x XP
@@ -72,9 +78,27 @@ __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, 
   const int c0 = blockIdx.x * CB;
   const int cw = min(CB, ncols - c0);
   const int nbp = (nb + 31) & ~31;  // rows padded to whole blocks, padding zero
-  for (int e = tid; e < nbp * CB; e += 256) {
-    const int i = e / CB, c = e - i * CB;
-    Xs[i * XP + c] = (c < cw && i < nb) ? X[(int64_t)i * ldx + c0 + c] : T(0);
+  if (PERM) {
+    T* buf = Xs + nbp * XP;  // touched rows x CB
+    const int nS = *perm_n;
+    for (int e = tid; e < nS * CB; e += 256) {
+      const int sl = e / CB, c = e - sl * CB;
+      buf[e] = c < cw ? X[(int64_t)pos[sl] * ldx + c0 + c] : T(0);
+    }
+    __syncthreads();
+    for (int e = tid; e < nbp * CB; e += 256) {
+      const int i = e / CB, c = e - i * CB;
+      Xs[i * XP + c] = i < nb ? buf[src[i] * CB + c] : T(0);
+    }
+    for (int e = nb * CB + tid; e < nS * CB; e += 256) {
+      const int sl = e / CB, c = e - sl * CB;
+      if (c < cw) X[(int64_t)pos[sl] * ldx + c0 + c] = buf[src[sl] * CB + c];
+    }
+  } else {
+    for (int e = tid; e < nbp * CB; e += 256) {
+      const int i = e / CB, c = e - i * CB;
+      Xs[i * XP + c] = (c < cw && i < nb) ? X[(int64_t)i * ldx + c0 + c] : T(0);
+    }
   }
   for (int b = 0; b < nb; b += 32) {
     const int bs = min(32, nb - b);
@@ -174,7 +198,7 @@ int trsm_llu_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int6
   if (ncols <= 0 || nb <= 0) return kOk;
   constexpr int CB = 32;
   size_t lds = sizeof(T) * (size_t)((nb + 31) & ~31) * (CB + 1);
-  auto kern = trsm_llu_kernel<T, CB>;
+  auto kern = trsm_llu_kernel<T, CB, false>;
   static bool attr_set = false;
   if (!attr_set) {
     EAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -182,7 +206,30 @@ int trsm_llu_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int6
     attr_set = true;
   }
   hipLaunchKernelGGL(kern, dim3(ceil_div(ncols, CB)), dim3(256), lds, s, L, ldl, nb, X, ldx,
-                     ncols);
+                     ncols, nullptr, nullptr, nullptr);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+// The panel's row permutation on ncols columns starting at X (row 0 = the
+// panel's first row), then X[0..nb) <- L^{-1} X[0..nb): the interchanges and
+// the U rows of those columns in one launch.  nb <= 256.
+template <typename T>
+int laswp_trsm_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int64_t ldx,
+                        int ncols, const int* perm_n, const int* pos, const int* src) {
+  if (ncols <= 0 || nb <= 0) return kOk;
+  constexpr int CB = 32;
+  const size_t nbp = (size_t)((nb + 31) & ~31);
+  size_t lds = sizeof(T) * (nbp * (CB + 1) + 2 * (size_t)nb * CB);
+  auto kern = trsm_llu_kernel<T, CB, true>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    EAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 150 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(ceil_div(ncols, CB)), dim3(256), lds, s, L, ldl, nb, X, ldx,
+                     ncols, perm_n, pos, src);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
@@ -210,7 +257,6 @@ template int transpose_copy<float, float>(hipStream_t, const float*, int64_t, in
 int update_row_order(hipStream_t s, int* ord, int k, const int* perm_n, const int* pos,
                      const int* src, int nb);
 int init_row_order(hipStream_t s, int* ord, int n);
-int panel_perm_init(hipStream_t s, int* pn, int* ppos, int* psrc, int nb);
 template <typename T>
 int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lperm_n, int* lpos,
             int* lsrc, int* pperm_n, int* ppos, int* psrc, int row0, int pnb, int* zero_col,
@@ -265,7 +311,7 @@ int factor_panel(eamd_handle* h, T* W, int64_t ldw, int k, int bs, int m, int* p
   hipStream_t s = h->stream;
   T* Pk = W + (int64_t)k * ldw + k;
   PhaseScope ps(h, kPhPanel, 2.0 * sizeof(T) * (double)m * bs, (double)m * bs * bs);
-  EAMD_CHECK(panel_perm_init(s, pperm_n, ppos, psrc, bs));
+  // (the panel's first leaf starts the panel permutation list)
   for (int j0 = 0; j0 < bs; j0 += kLeaf) {
     const int w = min(kLeaf, bs - j0);
     T* Lk = Pk + (int64_t)j0 * ldw + j0;
@@ -434,24 +480,22 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
     return sgemm_update_rowmajor(h, A12, A21, A22, ldw, mrows, ncols, k, kPrecPanelFp32);
   };
   hipEvent_t evP = h->la_ev[0], evB[2] = {h->la_ev[1], h->la_ev[2]};
-  // the concurrent update keeps to a grid of EAMD_UPDATE_WGS workgroups (one
-  // per CU at most, by its LDS), leaving the other CUs to the panel kernels
-  static const int update_wgs = env_int("EAMD_UPDATE_WGS", 0);
   EAMD_HIP(hipMemsetAsync(fz_dev, 0xff, sizeof(int), sa));  // -1
   if (ord) EAMD_CHECK(init_row_order(sa, ord, n));
   bool side_used = false;
   for (int k = 0, it = 0; k < n; k += nb, ++it) {
     const int par = it & 1;
     const int bs = min(nb, n - k), m = n - k;
+    const int tn = n - k - bs, n1 = min(nb, tn), n2 = tn - n1;
     float* Pk = W + (int64_t)k * ldw + k;
     EAMD_CHECK(factor_panel<float>(h, W, ldw, k, bs, m, piv_dev, lperm_n, lpos, lsrc, pn[par],
                                    pp[par], psl[par], zero_col, fz_dev, in_panel));
-    if (ord) EAMD_CHECK(update_row_order(sa, ord, k, pn[par], pp[par], psl[par], bs));
+    if (tn > 0) EAMD_HIP(hipEventRecord(evP, sa));  // the panel is factored
     // the next panel's columns were last written by the previous panel's
     // second-stream update
     if (it > 0) EAMD_HIP(hipStreamWaitEvent(sa, evB[par ^ 1], 0));
-    const int tn = n - k - bs, n1 = min(nb, tn), n2 = tn - n1;
     if (tn == 0) {  // last panel: its interchanges on the columns to its left
+      if (ord) EAMD_CHECK(update_row_order(sa, ord, k, pn[par], pp[par], psl[par], bs));
       PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)k, 0.0);
       EAMD_CHECK(laswp_perm_rowmajor<float>(sa, W + (int64_t)k * ldw, ldw, 0, pn[par], pp[par],
                                             psl[par], bs, k, bs, k));
@@ -460,40 +504,38 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
     float* A12n = W + (int64_t)k * ldw + (k + bs);
     float* A21 = W + (int64_t)(k + bs) * ldw + k;
     {
-      PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)n1, 0.0);
-      EAMD_CHECK(laswp_perm_rowmajor<float>(sa, A12n, ldw, 0, pn[par], pp[par], psl[par], bs, n1,
-                                            0, n1));
+      // the next panel's columns: interchanges and U rows in one launch,
+      // then their update in FP32 (no packing on this path)
+      PhaseScope ps(h, kPhTrsm, 4.0 * (4.0 * bs * n1 + 0.5 * bs * bs), (double)bs * bs * n1);
+      EAMD_CHECK(laswp_trsm_rowmajor<float>(sa, Pk, ldw, bs, A12n, ldw, n1, pn[par], pp[par],
+                                            psl[par]));
     }
-    {
-      PhaseScope ps(h, kPhTrsm, 4.0 * (2.0 * bs * n1 + 0.5 * bs * bs), (double)bs * bs * n1);
-      EAMD_CHECK(trsm_llu_rowmajor<float>(sa, Pk, ldw, bs, A12n, ldw, n1));
-    }
-    MixedOperand L, Un, Ur;
-    EAMD_CHECK(mixed_pack(h, true, A21, ldw, m - bs, bs, precision, packL[par], &L));
-    EAMD_HIP(hipEventRecord(evP, sa));
-    EAMD_CHECK(mixed_pack(h, false, A12n, ldw, n1, bs, precision, packU[0], &Un));
-    EAMD_CHECK(mixed_update_packed(h, Un, n1, L, m - bs, bs, precision,
-                                   W + (int64_t)(k + bs) * ldw + (k + bs), ldw));
-    // second stream: the other columns
+    EAMD_CHECK(sgemm_update_rowmajor(h, A12n, A21, W + (int64_t)(k + bs) * ldw + (k + bs), ldw,
+                                     m - bs, n1, bs, EAMD_PREC_FP32));
+    // second stream: the row order, the columns left of the panel and
+    // beyond the next panel
     EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));
     side_used = true;
     {
       StreamSwap sw(h, sb);
-      {
-        PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)(k + n2), 0.0);
+      if (ord) EAMD_CHECK(update_row_order(sb, ord, k, pn[par], pp[par], psl[par], bs));
+      if (k > 0) {
+        PhaseScope ps(h, kPhLaswp, 4.0 * sizeof(float) * bs * (double)k, 0.0);
         EAMD_CHECK(laswp_perm_rowmajor<float>(sb, W + (int64_t)k * ldw, ldw, 0, pn[par], pp[par],
-                                              psl[par], bs, k, bs + n1, k + n2));
+                                              psl[par], bs, k, 0, k));
       }
       if (n2 > 0) {
         float* A12r = A12n + n1;
         {
-          PhaseScope ps(h, kPhTrsm, 4.0 * (2.0 * bs * n2 + 0.5 * bs * bs), (double)bs * bs * n2);
-          EAMD_CHECK(trsm_llu_rowmajor<float>(sb, Pk, ldw, bs, A12r, ldw, n2));
+          PhaseScope ps(h, kPhTrsm, 4.0 * (4.0 * bs * n2 + 0.5 * bs * bs), (double)bs * bs * n2);
+          EAMD_CHECK(laswp_trsm_rowmajor<float>(sb, Pk, ldw, bs, A12r, ldw, n2, pn[par], pp[par],
+                                                psl[par]));
         }
+        MixedOperand L, Ur;
+        EAMD_CHECK(mixed_pack(h, true, A21, ldw, m - bs, bs, precision, packL[par], &L));
         EAMD_CHECK(mixed_pack(h, false, A12r, ldw, n2, bs, precision, packU[1], &Ur));
         EAMD_CHECK(mixed_update_packed(h, Ur, n2, L, m - bs, bs, precision,
-                                       W + (int64_t)(k + bs) * ldw + (k + bs + n1), ldw,
-                                       update_wgs));
+                                       W + (int64_t)(k + bs) * ldw + (k + bs + n1), ldw));
       }
       EAMD_HIP(hipEventRecord(evB[par], sb));
     }
