@@ -41,18 +41,36 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(T* __restrict__ W, int6
                                                          int c_split, int skip, int ncols_eff) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* buf = reinterpret_cast<T*>(smem_raw);
+  __shared__ int s_pos[512], s_src[512];
   const int nS = *perm_n;
+  for (int e = threadIdx.x; e < nS; e += 256) {
+    s_pos[e] = pos[e];
+    s_src[e] = src[e];
+  }
+  __syncthreads();
   const int cbase = blockIdx.x * CW;
   const int tc = threadIdx.x % CW, tr = threadIdx.x / CW;
-  const int RS = 256 / CW;
-  int ce = cbase + tc;
-  bool ok = ce < ncols_eff;
-  int c = ce < c_split ? ce : ce + skip;
-  for (int s = tr; s < nS; s += RS)
-    if (ok) buf[s * CW + tc] = W[(int64_t)(r0 + pos[s]) * ldw + c];
+  constexpr int RS = 256 / CW, U = 8;
+  const int ce = cbase + tc;
+  const bool ok = ce < ncols_eff;
+  const int c = ce < c_split ? ce : ce + skip;
+  // U rows in flight per thread before their values are staged
+  for (int s0 = tr; s0 < nS; s0 += RS * U) {
+    T v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int s2 = s0 + RS * j;
+      v[j] = (ok && s2 < nS) ? W[(int64_t)(r0 + s_pos[s2]) * ldw + c] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int s2 = s0 + RS * j;
+      if (s2 < nS) buf[s2 * CW + tc] = v[j];
+    }
+  }
   __syncthreads();
-  for (int s = tr; s < nS; s += RS)
-    if (ok) W[(int64_t)(r0 + pos[s]) * ldw + c] = buf[src[s] * CW + tc];
+  for (int s2 = tr; s2 < nS; s2 += RS)
+    if (ok) W[(int64_t)(r0 + s_pos[s2]) * ldw + c] = buf[s_src[s2] * CW + tc];
 }
 
 // X <- L^{-1} X, L unit lower nb x nb (row-major, ld ldl), X nb x ncols
@@ -60,16 +78,10 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(T* __restrict__ W, int6
 // Per 32-row diagonal block: one thread per column solves the block from
 // registers (compile-time indices, L broadcast from LDS, no barrier inside),
 // then all threads update the rows below with 32-long dot products.
-// PERM: the panel's row permutation (pos/src/perm_n: row pos[s] receives
-// the content of slot src[s]; slots 0..nb-1 are rows 0..nb-1) is applied to
-// these columns first: all touched rows are staged in LDS, the rows below
-// the top nb are written back permuted and the top nb enter the solve.
-template <typename T, int CB, bool PERM>
+template <typename T, int CB>
 __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, int64_t ldl,
                                                        int nb, T* __restrict__ X, int64_t ldx,
-                                                       int ncols, const int* __restrict__ perm_n,
-                                                       const int* __restrict__ pos,
-                                                       const int* __restrict__ src) {
+                                                       int ncols) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int XP = CB + 1;
   T* Xs = reinterpret_cast<T*>(smem_raw);  // nb x XP
@@ -78,27 +90,9 @@ __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, 
   const int c0 = blockIdx.x * CB;
   const int cw = min(CB, ncols - c0);
   const int nbp = (nb + 31) & ~31;  // rows padded to whole blocks, padding zero
-  if (PERM) {
-    T* buf = Xs + nbp * XP;  // touched rows x CB
-    const int nS = *perm_n;
-    for (int e = tid; e < nS * CB; e += 256) {
-      const int sl = e / CB, c = e - sl * CB;
-      buf[e] = c < cw ? X[(int64_t)pos[sl] * ldx + c0 + c] : T(0);
-    }
-    __syncthreads();
-    for (int e = tid; e < nbp * CB; e += 256) {
-      const int i = e / CB, c = e - i * CB;
-      Xs[i * XP + c] = i < nb ? buf[src[i] * CB + c] : T(0);
-    }
-    for (int e = nb * CB + tid; e < nS * CB; e += 256) {
-      const int sl = e / CB, c = e - sl * CB;
-      if (c < cw) X[(int64_t)pos[sl] * ldx + c0 + c] = buf[src[sl] * CB + c];
-    }
-  } else {
-    for (int e = tid; e < nbp * CB; e += 256) {
-      const int i = e / CB, c = e - i * CB;
-      Xs[i * XP + c] = (c < cw && i < nb) ? X[(int64_t)i * ldx + c0 + c] : T(0);
-    }
+  for (int e = tid; e < nbp * CB; e += 256) {
+    const int i = e / CB, c = e - i * CB;
+    Xs[i * XP + c] = (c < cw && i < nb) ? X[(int64_t)i * ldx + c0 + c] : T(0);
   }
   for (int b = 0; b < nb; b += 32) {
     const int bs = min(32, nb - b);
@@ -147,6 +141,106 @@ __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, 
   }
 }
 
+// The panel's row permutation (row pos[s] receives the content of slot
+// src[s]; slots 0..nb-1 are rows 0..nb-1) on CB columns of X, then
+// X[0..nb) <- L^{-1} X[0..nb) for those columns (FP32, nb <= 128).  The whole
+// L (unit lower, nb x nb) and every touched row are staged in LDS first, with
+// their loads in flight together; the rows below the top nb are written back
+// permuted and the top nb are solved in LDS, 32-row blocks: one thread per
+// column solves the diagonal block from registers, then all threads update
+// the rows below.
+template <int CB>
+__global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict__ L, int64_t ldl,
+                                                         int nb, float* __restrict__ X,
+                                                         int64_t ldx, int ncols,
+                                                         const int* __restrict__ perm_n,
+                                                         const int* __restrict__ pos,
+                                                         const int* __restrict__ src) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int XP = CB + 1, U = 8;
+  const int nbp = (nb + 31) & ~31, LP = nbp + 1;
+  float* Lf = reinterpret_cast<float*>(smem_raw);  // nbp x LP
+  float* Xs = Lf + nbp * LP;                       // nbp x XP
+  float* buf = Xs + nbp * XP;                      // touched rows x CB
+  int* s_pos = reinterpret_cast<int*>(buf + 2 * nb * CB);
+  int* s_src = s_pos + 2 * nb;
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * CB;
+  const int cw = min(CB, ncols - c0);
+  const int nS = *perm_n;
+  for (int e = tid; e < nS; e += 256) {
+    s_pos[e] = pos[e];
+    s_src[e] = src[e];
+  }
+  __syncthreads();
+  // L (strictly lower part; zero elsewhere) and the touched rows
+  for (int e0 = tid; e0 < nbp * nbp; e0 += 256 * U) {
+    float v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int e = e0 + 256 * j, r = e / nbp, q = e - r * nbp;
+      v[j] = (e < nbp * nbp && q < r && r < nb) ? L[(int64_t)r * ldl + q] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int e = e0 + 256 * j, r = e / nbp, q = e - r * nbp;
+      if (e < nbp * nbp) Lf[r * LP + q] = v[j];
+    }
+  }
+  for (int e0 = tid; e0 < nS * CB; e0 += 256 * U) {
+    float v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int e = e0 + 256 * j, sl = e / CB, c = e - sl * CB;
+      v[j] = (e < nS * CB && c < cw) ? X[(int64_t)s_pos[sl] * ldx + c0 + c] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int e = e0 + 256 * j;
+      if (e < nS * CB) buf[e] = v[j];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nbp * CB; e += 256) {
+    const int i = e / CB, c = e - i * CB;
+    Xs[i * XP + c] = i < nb ? buf[s_src[i] * CB + c] : 0.f;
+  }
+  for (int e = nb * CB + tid; e < nS * CB; e += 256) {
+    const int sl = e / CB, c = e - sl * CB;
+    if (c < cw) X[(int64_t)s_pos[sl] * ldx + c0 + c] = buf[s_src[sl] * CB + c];
+  }
+  __syncthreads();
+  for (int b = 0; b < nb; b += 32) {
+    if (tid < CB) {
+      float x[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) x[i] = Xs[(b + i) * XP + tid];
+#pragma unroll
+      for (int i = 1; i < 32; ++i) {
+        float acc = x[i];
+#pragma unroll
+        for (int q = 0; q < i; ++q) acc -= Lf[(b + i) * LP + b + q] * x[q];
+        x[i] = acc;
+      }
+#pragma unroll
+      for (int i = 0; i < 32; ++i) Xs[(b + i) * XP + tid] = x[i];
+    }
+    __syncthreads();
+    for (int e = tid; e < (nbp - b - 32) * CB; e += 256) {
+      const int i = b + 32 + e / CB, c = e % CB;
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < 32; ++q) acc += Lf[i * LP + b + q] * Xs[(b + q) * XP + c];
+      Xs[i * XP + c] -= acc;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * CB; e += 256) {
+    const int i = e / CB, c = e - i * CB;
+    if (c < cw) X[(int64_t)i * ldx + c0 + c] = Xs[i * XP + c];
+  }
+}
+
 // B[i*ldb + j] = A[i + j*lda] for i < rows, j < cols: a column-major matrix
 // into the row-major layout (or back: a row-major matrix is the
 // column-major storage of its transpose).
@@ -183,7 +277,7 @@ int laswp_perm_rowmajor(hipStream_t s, T* W, int64_t ldw, int r0, const int* per
   static bool attr_set = false;
   if (!attr_set) {
     EAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 160 * 1024));
+                                 150 * 1024));
     attr_set = true;
   }
   hipLaunchKernelGGL(kern, dim3(ceil_div(ncols_eff, CW)), dim3(256), lds, s, W, ldw, r0, perm_n,
@@ -198,7 +292,7 @@ int trsm_llu_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int6
   if (ncols <= 0 || nb <= 0) return kOk;
   constexpr int CB = 32;
   size_t lds = sizeof(T) * (size_t)((nb + 31) & ~31) * (CB + 1);
-  auto kern = trsm_llu_kernel<T, CB, false>;
+  auto kern = trsm_llu_kernel<T, CB>;
   static bool attr_set = false;
   if (!attr_set) {
     EAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -206,30 +300,34 @@ int trsm_llu_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int6
     attr_set = true;
   }
   hipLaunchKernelGGL(kern, dim3(ceil_div(ncols, CB)), dim3(256), lds, s, L, ldl, nb, X, ldx,
-                     ncols, nullptr, nullptr, nullptr);
+                     ncols);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
 
 // The panel's row permutation on ncols columns starting at X (row 0 = the
 // panel's first row), then X[0..nb) <- L^{-1} X[0..nb): the interchanges and
-// the U rows of those columns in one launch.  nb <= 256.
-template <typename T>
-int laswp_trsm_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int64_t ldx,
+// the U rows of those columns in one launch (FP32, nb <= 128).
+int laswp_trsm_rowmajor(hipStream_t s, const float* L, int64_t ldl, int nb, float* X, int64_t ldx,
                         int ncols, const int* perm_n, const int* pos, const int* src) {
   if (ncols <= 0 || nb <= 0) return kOk;
+  if (nb > 128) {
+    set_error("laswp_trsm: panel width %d > 128", nb);
+    return kErrUnsupported;
+  }
   constexpr int CB = 32;
   const size_t nbp = (size_t)((nb + 31) & ~31);
-  size_t lds = sizeof(T) * (nbp * (CB + 1) + 2 * (size_t)nb * CB);
-  auto kern = trsm_llu_kernel<T, CB, true>;
+  const size_t lds = sizeof(float) * (nbp * (nbp + 1) + nbp * (CB + 1) + 2 * (size_t)nb * CB) +
+                     sizeof(int) * 4 * (size_t)nb;
+  auto kern = laswp_trsm_kernel<CB>;
   static bool attr_set = false;
   if (!attr_set) {
     EAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(ceil_div(ncols, CB)), dim3(256), lds, s, L, ldl, nb, X, ldx,
-                     ncols, perm_n, pos, src);
+  hipLaunchKernelGGL(kern, dim3(ceil_div(ncols, CB)), dim3(256), lds, s, L, ldl, nb, X, ldx, ncols,
+                     perm_n, pos, src);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
@@ -430,7 +528,7 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
                          int n, int* piv_dev, int* ord, int nb, int precision);
 int getrf_lookahead_mixed(eamd_handle* h, float* W, int64_t ldw, int n, int* piv_dev, int* ord,
                           int64_t* first_zero, int nb, int precision) {
-  if (nb > 256) nb = 256;
+  if (nb > 128) nb = 128;  // the next panel's interchanges and U rows keep L in LDS
   if (nb < kLeaf) nb = kLeaf;
   EAMD_CHECK(ensure_side_stream(h));
   // the panels run on the reserved-CU stream when there is one
@@ -507,7 +605,7 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
       // the next panel's columns: interchanges and U rows in one launch,
       // then their update in FP32 (no packing on this path)
       PhaseScope ps(h, kPhTrsm, 4.0 * (4.0 * bs * n1 + 0.5 * bs * bs), (double)bs * bs * n1);
-      EAMD_CHECK(laswp_trsm_rowmajor<float>(sa, Pk, ldw, bs, A12n, ldw, n1, pn[par], pp[par],
+      EAMD_CHECK(laswp_trsm_rowmajor(sa, Pk, ldw, bs, A12n, ldw, n1, pn[par], pp[par],
                                             psl[par]));
     }
     EAMD_CHECK(sgemm_update_rowmajor(h, A12n, A21, W + (int64_t)(k + bs) * ldw + (k + bs), ldw,
@@ -528,7 +626,7 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
         float* A12r = A12n + n1;
         {
           PhaseScope ps(h, kPhTrsm, 4.0 * (4.0 * bs * n2 + 0.5 * bs * bs), (double)bs * bs * n2);
-          EAMD_CHECK(laswp_trsm_rowmajor<float>(sb, Pk, ldw, bs, A12r, ldw, n2, pn[par], pp[par],
+          EAMD_CHECK(laswp_trsm_rowmajor(sb, Pk, ldw, bs, A12r, ldw, n2, pn[par], pp[par],
                                                 psl[par]));
         }
         MixedOperand L, Ur;
