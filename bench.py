@@ -241,7 +241,7 @@ def main():
     upd_ms, upd_bytes, upd_flops, upd_cnt = profile_phase(ea, 3)
     phases = {}
     for name, ph in (("panel", 0), ("laswp", 1), ("trsm", 2), ("update", 3), ("pack", 4),
-                     ("trsv", 5), ("residual", 6)):
+                     ("trsv", 5), ("residual", 6), ("next_panel", 10)):
         ms, by, fl, cnt = profile_phase(ea, ph)
         phases[name] = round(ms / args.steps, 3)
     ea.lib().eamd_profile_enable(h, 0)

@@ -61,7 +61,7 @@ namespace eamd {
 // launches on the handle's stream (enabled per handle; off by default).
 enum Phase : int {
   kPhPanel = 0, kPhLaswp, kPhTrsm, kPhUpdate, kPhPack, kPhTrsv, kPhResidual, kPhGemm,
-  kPhPotf2, kPhBatched, kPhCount
+  kPhPotf2, kPhBatched, kPhNext, kPhCount
 };
 struct PhaseTimer {
   bool enabled = false;

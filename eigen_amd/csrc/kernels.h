@@ -99,6 +99,9 @@ struct MixedOperand {
 size_t mixed_operand_bytes(int len, int k, int precision);
 int mixed_pack(eamd_handle* h, bool rows, const float* S, int64_t ld, int len, int k,
                int precision, void* buf, MixedOperand* out);
+// FP32 update A22 -= A21 A12 (row-major, ld), timed under `phase`.
+int f32_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
+                        int64_t ld, int mrows, int ncols, int k, int phase);
 // max_wgs > 0 caps the grid (rounded down to a multiple of 8; the
 // workgroups loop over the tiles).
 int mixed_update_packed(eamd_handle* h, const MixedOperand& a, int M, const MixedOperand& b, int N,

@@ -946,6 +946,7 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
 // top rows.  Four threads per row: each computes l and updates every fourth
 // column.
 constexpr int kApplyTpr = 4;
+constexpr int kFuseRestMax = 96;  // panel columns right of a fused leaf (panels <= 128)
 __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__ P, int64_t ldp,
                                                              int m, const float* __restrict__ saved,
                                                              const int* __restrict__ moved,
@@ -1044,7 +1045,7 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   // the leaf's interchanges on the other panel columns, the U rows right of
   // it and their update are done here too when w == 32 (the caller skips
   // its own)
-  const bool fuse = fused && w == LW && G1 > 1;
+  const bool fuse = fused && w == LW && G1 > 1 && pnb - row0 - w <= kFuseRestMax;
   if (fused) *fused = fuse;
   if (G1 <= 1)
     return leaf_lu<float>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,

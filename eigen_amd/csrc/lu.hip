@@ -608,8 +608,8 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
       EAMD_CHECK(laswp_trsm_rowmajor(sa, Pk, ldw, bs, A12n, ldw, n1, pn[par], pp[par],
                                             psl[par]));
     }
-    EAMD_CHECK(sgemm_update_rowmajor(h, A12n, A21, W + (int64_t)(k + bs) * ldw + (k + bs), ldw,
-                                     m - bs, n1, bs, EAMD_PREC_FP32));
+    EAMD_CHECK(f32_update_rowmajor(h, A12n, A21, W + (int64_t)(k + bs) * ldw + (k + bs), ldw,
+                                   m - bs, n1, bs, kPhNext));
     // second stream: the row order, the columns left of the panel and
     // beyond the next panel
     EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));

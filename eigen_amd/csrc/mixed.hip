@@ -488,23 +488,30 @@ int mixed_update_packed(eamd_handle* h, const MixedOperand& a, int M, const Mixe
   return kOk;
 }
 
+// FP32 update A22 -= A21 A12 (row-major, ld) on the FP32 matrix cores,
+// timed under `phase` (the panel's own updates add no bytes or flops).
+int f32_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
+                        int64_t ld, int mrows, int ncols, int k, int phase) {
+  if (mrows <= 0 || ncols <= 0 || k <= 0) return kOk;
+  const int M = ncols, N = mrows;
+  const bool in_panel = phase == kPhPanel;
+  PhaseScope ps(h, phase, in_panel ? 0.0 : 4.0 * (2.0 * M * N + (double)k * (M + N)),
+                in_panel ? 0.0 : 2.0 * M * N * k);
+  const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
+  hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, h->stream, M, N, k, A12, A21,
+                     ld, A22);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
 int sgemm_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, float* A22,
                           int64_t ld, int mrows, int ncols, int k, int precision) {
   if (mrows <= 0 || ncols <= 0 || k <= 0) return kOk;
-  hipStream_t s = h->stream;
   const int M = ncols, N = mrows;
-  if (precision == EAMD_PREC_FP32 || precision == EAMD_PREC_FP64 || precision < 0) {
+  if (precision == EAMD_PREC_FP32 || precision == EAMD_PREC_FP64 || precision < 0)
     // precision < 0: an update inside the panel (FP32, accounted to the panel)
-    const bool in_panel = precision < 0;
-    PhaseScope ps(h, in_panel ? kPhPanel : kPhUpdate,
-                  in_panel ? 0.0 : 4.0 * (2.0 * M * N + (double)k * (M + N)),
-                  in_panel ? 0.0 : 2.0 * M * N * k);
-    const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
-    hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, s, M, N, k, A12, A21, ld,
-                       A22);
-    EAMD_HIP(hipGetLastError());
-    return kOk;
-  }
+    return f32_update_rowmajor(h, A12, A21, A22, ld, mrows, ncols, k,
+                               precision < 0 ? kPhPanel : kPhUpdate);
   const size_t oa = mixed_operand_bytes(M, k, precision), ob = mixed_operand_bytes(N, k, precision);
   void* p = nullptr;
   EAMD_CHECK(h->ws.get(kWsSplit, oa + ob, &p));

@@ -67,6 +67,7 @@ int eamd_release_workspace(eamd_handle_t handle);
 #define EAMD_PHASE_GEMM 7      /* eamd_dgemm */
 #define EAMD_PHASE_POTF2 8     /* Cholesky diagonal blocks */
 #define EAMD_PHASE_BATCHED 9   /* batched solves */
+#define EAMD_PHASE_NEXT 10     /* look-ahead LU: the next panel's FP32 update */
 int eamd_profile_enable(eamd_handle_t handle, int on);
 int eamd_profile_read(eamd_handle_t handle, int phase, double* ms, double* bytes, double* flops,
                       int64_t* count);
