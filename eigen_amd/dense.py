@@ -3,9 +3,10 @@
 Mirrors the reference's decomposition classes:
 
 * ``PartialPivLU``  -- ``compute() / solve() / matrixLU() / permutationP() /
-  determinant() / reconstructedMatrix()`` (Eigen/src/LU/PartialPivLU.h:98-221)
+  determinant() / inverse() / rcond() / reconstructedMatrix()``
+  (Eigen/src/LU/PartialPivLU.h:98-221)
 * ``LLT``           -- ``compute() / solve() / solveInPlace() / matrixL() /
-  matrixLLT() / info()`` (Eigen/src/Cholesky/LLT.h:84-203)
+  matrixLLT() / info() / rcond()`` (Eigen/src/Cholesky/LLT.h:84-203)
 * ``gemm`` (``C.noalias() = A*B``), ``triangular_solve``
   (``TriangularView::solveInPlace``), ``rank_update`` (``SelfAdjointView::
   rankUpdate``).
@@ -191,6 +192,19 @@ class PartialPivLU:
         sign = -1.0 if self._nbt % 2 else 1.0
         return sign * float(torch.prod(torch.diagonal(self._lu)))
 
+    def inverse(self):
+        """A^{-1} from the factors: the solve of the identity (PartialPivLU.h:197-202)."""
+        self._check()
+        n = self._lu.shape[0]
+        return self.solve(torch.eye(n, dtype=torch.float64, device=self._lu.device))
+
+    def rcond(self):
+        """Estimate of 1 / (||A||_1 ||A^{-1}||_1) (PartialPivLU.h:184-188)."""
+        self._check()
+        n = self._lu.shape[0]
+        return _rcond_estimate(self._l1_norm, n, self.solve,
+                               lambda v: self.solve(v, trans="T"), self._lu.device)
+
     def reconstructedMatrix(self):
         """P^{-1} L U (PartialPivLU.h:538-566)."""
         self._check()
@@ -227,6 +241,11 @@ class LLT:
         m = colmajor(matrix, torch.float64).clone()
         m = colmajor(m)
         n = m.shape[0]
+        # ||A||_1 of the selfadjoint matrix read from the stored triangle
+        # (LLT.h:434-445)
+        tri = torch.tril(m) if self.uplo == "L" else torch.triu(m)
+        sym = tri + tri.t() - torch.diag(torch.diagonal(tri))
+        self._l1_norm = float(sym.abs().sum(dim=0).max()) if n else 0.0
         fc = C.c_int64(-1)
         check("eamd_dpotrf", lib().eamd_dpotrf(handle(), self.uplo.encode(), n, _p(m), _ld(m),
                                                C.byref(fc)))
@@ -262,6 +281,52 @@ class LLT:
         x = colmajor(x)
         self.solveInPlace(x)
         return x.reshape(-1) if vec else x
+
+    def rcond(self):
+        """Estimate of 1 / (||A||_1 ||A^{-1}||_1) (LLT.h:161-166); the
+        factorization must have succeeded."""
+        if self._info != Success:
+            raise RuntimeError("LLT is not initialized or the matrix is not positive definite.")
+        n = self._m.shape[0]
+        return _rcond_estimate(self._l1_norm, n, self.solve, self.solve, self._m.device)
+
+
+def _rcond_estimate(norm1, n, solve, solve_adj, device):
+    """1 / (norm1 * est(||A^{-1}||_1)) from solves with A and A^T: Hager's
+    one-norm estimator with Higham's refinements (N. J. Higham, ACM TOMS 14,
+    1988, Algorithms 4.1 and 5.1), as rcond_estimate_helper and
+    rcond_invmatrix_L1_norm_estimate (Eigen/src/Core/ConditionEstimator.h:
+    56-172) use it."""
+    if n == 0:
+        return 1.0
+    if norm1 == 0.0:
+        return 0.0
+    if n == 1:
+        return 1.0
+    f64 = dict(dtype=torch.float64, device=device)
+    v = solve(torch.full((n,), 1.0 / n, **f64))
+    lower = float(v.abs().sum())
+    old_lower, old_sign, old_idx = lower, None, -1
+    for k in range(4):
+        sign = torch.where(v < 0, -1.0, 1.0).to(torch.float64)
+        if k > 0 and torch.equal(sign, old_sign):
+            break
+        z = solve_adj(sign)
+        idx = int(torch.argmax(z.abs()))  # the first maximum
+        if idx == old_idx:
+            break
+        e = torch.zeros(n, **f64)
+        e[idx] = 1.0
+        v = solve(e)
+        lower = float(v.abs().sum())
+        if lower <= old_lower:
+            break
+        old_sign, old_idx, old_lower = sign, idx, lower
+    i = torch.arange(n, **f64)
+    alt = torch.where(torch.arange(n, device=device) % 2 == 0, 1.0, -1.0).to(torch.float64)
+    alt = alt * (1.0 + i / (n - 1))
+    est = max(lower, 2.0 * float(solve(alt).abs().sum()) / (3.0 * n))
+    return 0.0 if est == 0.0 else (1.0 / est) / norm1
 
 
 def backward_error(A, X, B):

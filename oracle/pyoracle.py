@@ -89,6 +89,11 @@ def ref():
                                             C.POINTER(C.c_double)]
         lib.eref_llt_d.argtypes = [L, _dp, L, L, _dp, L, _dp, _dp, L]
         lib.eref_llt_f.argtypes = [L, _fp, L, L, _fp, L, _fp, _fp, L]
+        lib.eref_lu_rcond_d.argtypes = [L, _dp, L]
+        lib.eref_lu_rcond_d.restype = C.c_double
+        lib.eref_lu_inverse_d.argtypes = [L, _dp, L, _dp]
+        lib.eref_llt_rcond_d.argtypes = [L, _dp, L]
+        lib.eref_llt_rcond_d.restype = C.c_double
         lib.eref_llt_blocked_d.argtypes = [L, _dp, L]
         lib.eref_llt_blocked_d.restype = L
         lib.eref_gemm_d.argtypes = [L, L, L, _dp, L, _dp, L, _dp, L]
@@ -209,3 +214,96 @@ def batched_lu_solve_d(offA, n, A, offb, nrhs, B):
     info = np.zeros(len(n), dtype=np.int32)
     orc().orc_batched_lu_solve_d(len(n), offA, n, A, offb, nrhs, B, info)
     return B, info
+
+
+# ------------------------------------------------------- condition numbers --
+def inv_l1_norm_estimate(n, solve, solve_adj):
+    """Estimate of ||A^-1||_1 from solves with A and A^T (Hager's method with
+    Higham's refinements, N. J. Higham, "FORTRAN codes for estimating the
+    one-norm of a real or complex matrix", ACM TOMS 14, 1988, Algorithms 4.1
+    and 5.1; restated from rcond_invmatrix_L1_norm_estimate,
+    Eigen/src/Core/ConditionEstimator.h:56-141).  `solve(v)` returns A^-1 v,
+    `solve_adj(v)` returns A^-T v (1-D float64 arrays)."""
+    if n == 0:
+        return 0.0
+    v = solve(np.full(n, 1.0 / n))
+    lower = float(np.abs(v).sum())
+    if n == 1:
+        return lower
+    old_lower, old_sign, old_idx = lower, None, -1
+    for k in range(4):
+        sign = np.where(v < 0, -1.0, 1.0)  # rcond_compute_sign, real case (:21-27)
+        if k > 0 and np.array_equal(sign, old_sign):
+            break
+        z = solve_adj(sign)
+        idx = int(np.argmax(np.abs(z)))  # first maximum, as maxCoeff
+        if idx == old_idx:
+            break
+        e = np.zeros(n)
+        e[idx] = 1.0
+        v = solve(e)
+        lower = float(np.abs(v).sum())
+        if lower <= old_lower:
+            break
+        old_sign, old_idx, old_lower = sign, idx, lower
+    # Higham's alternating-sign vector (:126-137)
+    i = np.arange(n, dtype=np.float64)
+    alt = np.where(np.arange(n) % 2 == 0, 1.0, -1.0) * (1.0 + i / (n - 1))
+    alt_lower = 2.0 * float(np.abs(solve(alt)).sum()) / (3.0 * n)
+    return max(lower, alt_lower)
+
+
+def rcond_from(norm1, n, solve, solve_adj):
+    """1 / (||A||_1 ||A^-1||_1 estimate) (ConditionEstimator.h:157-172)."""
+    if n == 0:
+        return 1.0
+    if norm1 == 0.0:
+        return 0.0
+    if n == 1:
+        return 1.0
+    inv = inv_l1_norm_estimate(n, solve, solve_adj)
+    return 0.0 if inv == 0.0 else (1.0 / inv) / norm1
+
+
+def lu_rcond(a):
+    """PartialPivLU::rcond on the oracle's factorization (PartialPivLU.h:184-188):
+    ||A||_1 is the maximum absolute column sum of A (:522)."""
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[0]
+    lu_, t, _, _ = lu(a)
+    L = np.tril(lu_, -1) + np.eye(n)
+    U = np.triu(lu_)
+    perm = perm_from_transpositions(t)  # P A = L U, (P b)[perm[i]] = b[i]
+    import scipy.linalg as sl
+
+    def solve(b):
+        pb = np.empty(n)
+        pb[perm] = b
+        return sl.solve_triangular(U, sl.solve_triangular(L, pb, lower=True, unit_diagonal=True))
+
+    def solve_adj(b):
+        y = sl.solve_triangular(L.T, sl.solve_triangular(U.T, b, lower=True), lower=False,
+                                unit_diagonal=True)
+        return y[perm]
+
+    return rcond_from(float(np.abs(a).sum(axis=0).max()), n, solve, solve_adj)
+
+
+def llt_rcond(a):
+    """LLT::rcond (LLT.h:161-166): ||A||_1 of the selfadjoint matrix read from
+    its lower triangle (LLT.h:434-445)."""
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[0]
+    l_, fail = llt(a)
+    if fail >= 0:
+        return -1.0
+    L = np.tril(l_)
+    low = np.tril(a)
+    sym = low + np.tril(a, -1).T
+    import scipy.linalg as sl
+
+    def solve(b):
+        return sl.solve_triangular(L.T, sl.solve_triangular(L, b, lower=True), lower=False)
+
+    return rcond_from(float(np.abs(sym).sum(axis=0).max()), n, solve, solve)
+

@@ -93,6 +93,25 @@ int eref_llt_f(long n, const float* A, long lda, long nrhs, const float* B,
   return llt.info() == Success ? 0 : 1;
 }
 
+// PartialPivLU<MatrixXd>::rcond() (PartialPivLU.h:184-188,
+// ConditionEstimator.h:157-172) and inverse() (PartialPivLU.h:197-202)
+double eref_lu_rcond_d(long n, const double* A, long lda) {
+  MatrixXd Am = MapD(const_cast<double*>(A), n, n, OuterStride<>(lda));
+  PartialPivLU<MatrixXd> lu(Am);
+  return lu.rcond();
+}
+void eref_lu_inverse_d(long n, const double* A, long lda, double* Ainv) {
+  MatrixXd Am = MapD(const_cast<double*>(A), n, n, OuterStride<>(lda));
+  PartialPivLU<MatrixXd> lu(Am);
+  Map<MatrixXd>(Ainv, n, n) = lu.inverse();
+}
+// LLT<MatrixXd,Lower>::rcond() (LLT.h:161-166): -1 when not SPD
+double eref_llt_rcond_d(long n, const double* A, long lda) {
+  MatrixXd Am = MapD(const_cast<double*>(A), n, n, OuterStride<>(lda));
+  LLT<MatrixXd> llt(Am);
+  return llt.info() == Success ? llt.rcond() : -1.0;
+}
+
 // internal::llt_inplace<double,Lower>::blocked (LLT.h:330): failing col or -1
 long eref_llt_blocked_d(long n, double* a, long lda) {
   MapD A(a, n, n, OuterStride<>(lda));

@@ -17,6 +17,9 @@ these fixtures are Eigen's outputs on seeded inputs (tests/inputs.py):
             small ones; row sums, column sums and 256 sampled entries else).
   batched.npz  a 1,000-system sample of the batched LLT (FP32) and batched LU
             (FP64, 4 RHS, 48 systems) workloads with Eigen's per-system solutions.
+  rcond.npz PartialPivLU::rcond / inverse (PartialPivLU.h:184-202) and
+            LLT::rcond (LLT.h:161-166) on the lu.npz / llt.npz inputs
+            (`make_golden.py rcond` writes this file alone).
 Inputs are not stored (they are regenerated from the seeds); their SHA-256
 is, so a diverging generator is detected.
 """
@@ -154,5 +157,33 @@ def main():
     print("fixtures written to", HERE)
 
 
+RCOND_LU_SIZES = [1, 2, 3, 5, 16, 33, 64, 129, 257]
+RCOND_LLT_SIZES = [1, 6, 24, 33, 129]
+INV_SIZES = [1, 5, 33, 64]
+
+
+def rcond_main():
+    ref = po.ref()
+    out = {}
+    for n in RCOND_LU_SIZES:
+        a = inputs.normal(1000 + n, n, n)
+        out[f"lu{n}_sha"] = np.array(inputs.sha256(a))
+        out[f"lu{n}_rcond"] = np.array(ref.eref_lu_rcond_d(n, a, n))
+    for n in RCOND_LLT_SIZES:
+        a = inputs.spd(3000 + n, n)
+        out[f"llt{n}_sha"] = np.array(inputs.sha256(a))
+        out[f"llt{n}_rcond"] = np.array(ref.eref_llt_rcond_d(n, a, n))
+    for n in INV_SIZES:
+        a = inputs.normal(1000 + n, n, n)
+        inv = np.zeros((n, n), order="F")
+        ref.eref_lu_inverse_d(n, a, n, inv)
+        out[f"inv{n}"] = inv
+    np.savez_compressed(os.path.join(HERE, "rcond.npz"), **out)
+    print("rcond fixtures written to", HERE)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "rcond":
+        rcond_main()
+    else:
+        main()

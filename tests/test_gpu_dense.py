@@ -238,3 +238,39 @@ def test_trsm_all_variants(gpu, side, uplo, trans, diag):
     ea.triangular_solve(dev(t), out, side=side, uplo=uplo, trans=trans, diag=diag, alpha=2.0)
     ref = po.trsm(side, uplo, trans, diag, t, b, alpha=2.0)
     np.testing.assert_allclose(host(out), ref, rtol=1e-10, atol=1e-11)
+
+
+# ---------------------------------------------------------- rcond / inverse --
+@pytest.fixture(scope="module")
+def grc():
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "rcond.npz"))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 33, 64, 129, 257])
+def test_lu_rcond_against_reference_fixture(gpu, grc, n):
+    a = inputs.normal(1000 + n, n, n)
+    lu = ea.dense.PartialPivLU(torch.from_numpy(a).cuda())
+    np.testing.assert_allclose(lu.rcond(), float(grc[f"lu{n}_rcond"]), rtol=1e-9)
+
+
+@pytest.mark.parametrize("n", [1, 6, 24, 33, 129])
+def test_llt_rcond_against_reference_fixture(gpu, grc, n):
+    a = inputs.spd(3000 + n, n)
+    llt = ea.dense.LLT(torch.from_numpy(a).cuda())
+    np.testing.assert_allclose(llt.rcond(), float(grc[f"llt{n}_rcond"]), rtol=1e-9)
+
+
+@pytest.mark.parametrize("n", [1, 5, 33, 64])
+def test_lu_inverse_against_reference_fixture(gpu, grc, n):
+    a = inputs.normal(1000 + n, n, n)
+    inv = ea.dense.PartialPivLU(torch.from_numpy(a).cuda()).inverse().cpu().numpy()
+    ref = grc[f"inv{n}"]
+    np.testing.assert_allclose(inv, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
+def test_rcond_closed_forms_gpu(gpu):
+    d = torch.tensor([3.0, -0.5, 2.0, 8.0, -1.0], dtype=torch.float64)
+    assert ea.dense.PartialPivLU(torch.diag(d).cuda()).rcond() == pytest.approx(1 / 16, rel=1e-13)
+    assert ea.dense.LLT(torch.diag(d.abs()).cuda()).rcond() == pytest.approx(1 / 16, rel=1e-13)
+    z = torch.zeros(4, 4, dtype=torch.float64)
+    assert ea.dense.PartialPivLU(z.cuda()).rcond() == 0.0

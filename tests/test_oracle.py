@@ -289,3 +289,43 @@ def test_batched_equals_single_system_routines():
         L, f = po.llt(a, dtype=np.float32)
         xi = po.llt_solve(L, b[offb[i]:offb[i] + k])[:, 0]
         np.testing.assert_array_equal(xi, x[offb[i]:offb[i] + k])
+
+
+# ---------------------------------------------------------- rcond / inverse --
+@pytest.fixture(scope="module")
+def grc():
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "rcond.npz"))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 33, 64, 129, 257])
+def test_lu_rcond_matches_reference_fixture(grc, n):
+    a = inputs.normal(1000 + n, n, n)
+    assert inputs.sha256(a) == str(grc[f"lu{n}_sha"])
+    np.testing.assert_allclose(po.lu_rcond(a), float(grc[f"lu{n}_rcond"]), rtol=1e-10)
+
+
+@pytest.mark.parametrize("n", [1, 6, 24, 33, 129])
+def test_llt_rcond_matches_reference_fixture(grc, n):
+    a = inputs.spd(3000 + n, n)
+    assert inputs.sha256(a) == str(grc[f"llt{n}_sha"])
+    np.testing.assert_allclose(po.llt_rcond(a), float(grc[f"llt{n}_rcond"]), rtol=1e-10)
+
+
+def test_rcond_closed_forms():
+    # identity: 1; diagonal: min|d| / max|d| (the estimator finds e_j exactly);
+    # a zero matrix: 0
+    assert po.lu_rcond(np.eye(7)) == pytest.approx(1.0)
+    d = np.array([3.0, -0.5, 2.0, 8.0, -1.0])
+    assert po.lu_rcond(np.diag(d)) == pytest.approx(0.5 / 8.0, rel=1e-14)
+    assert po.llt_rcond(np.diag(np.abs(d))) == pytest.approx(0.5 / 8.0, rel=1e-14)
+    assert po.rcond_from(0.0, 3, None, None) == 0.0
+    # a 2x2 case by hand: A = [[1, 2], [3, 4]], ||A||_1 = 6, A^-1 = [[-2, 1], [1.5, -0.5]],
+    # ||A^-1||_1 = 3.5
+    assert po.lu_rcond(np.array([[1.0, 2.0], [3.0, 4.0]])) == pytest.approx(1.0 / 21.0, rel=1e-14)
+
+
+def test_reference_inverse_fixture_is_an_inverse(grc):
+    for n in (1, 5, 33, 64):
+        a = inputs.normal(1000 + n, n, n)
+        inv = grc[f"inv{n}"]
+        assert np.abs(a @ inv - np.eye(n)).max() < 1e-10 * max(1.0, np.abs(inv).max() * n)
