@@ -104,7 +104,7 @@ __device__ __forceinline__ void store_slab(double* As, double* Bs, const double 
 // MODE 0: full C.  MODE 1: lower triangle only (i >= j), tiles strictly above
 // the diagonal are skipped.  MODE 2: upper triangle only (i <= j).
 template <bool TA, bool TB, int MODE>
-__global__ __launch_bounds__(256) void dgemm_kernel(int m, int n, int k, double alpha,
+__global__ __launch_bounds__(256, 2) void dgemm_kernel(int m, int n, int k, double alpha,
                                                     const double* __restrict__ A, int64_t lda,
                                                     const double* __restrict__ B, int64_t ldb,
                                                     double beta, double* __restrict__ C,
