@@ -46,11 +46,11 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
                                                          int64_t ldx, int c0, int nrhs,
                                                          u64* __restrict__ xw, unsigned epoch,
                                                          int* __restrict__ err) {
-  static_assert(BS == 64 && NRC == 4, "four threads per row of a 64-row block, 4 RHS");
+  static_assert(BS == 64 && (NRC == 4 || NRC == 1), "four threads per row of a 64-row block");
   constexpr int SEG = BS / 4;  // columns of a predecessor block per thread
   constexpr int NW = XWords<T>::n;
-  constexpr int CHUNK = BS * NRC * NW;  // words of one block's solution
-  constexpr int WPT = CHUNK / 256;      // words staged per thread
+  constexpr int CHUNK = BS * NRC * NW;          // words of one block's solution
+  constexpr int WPT = (CHUNK + 255) / 256;      // words staged per thread
   __shared__ T Dinv[BS][BS + 1];
   __shared__ T Tb[BS][BS + 1];
   __shared__ T accs[BS][NRC];
@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {
       const int widx = tid + k * 256;
+      if (widx >= CHUNK) break;
       const int row = jb * BS + widx / (NRC * NW);
       const int c = (widx / NW) % NRC;
       unsigned payload = 0;
@@ -247,6 +248,23 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
   }
   PhaseScope ps(h, kPhTrsv, sizeof(T) * (0.5 * (double)n * n + 2.0 * (double)n * nrhs),
                 (double)n * n * nrhs);
+  if (nrhs == 1) {
+    // one right-hand side: one exchange word (per 32 payload bits) per row
+    const unsigned epoch = (unsigned)++h->trsv_epoch;
+#define EAMD_TRSV1(RM, LO, UN)                                                                 \
+  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, 1>), dim3(nblk), dim3(256), 0,      \
+                     h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err)
+    if (rowmajor) {
+      if (lower) { if (unit) EAMD_TRSV1(true, true, true); else EAMD_TRSV1(true, true, false); }
+      else { if (unit) EAMD_TRSV1(true, false, true); else EAMD_TRSV1(true, false, false); }
+    } else {
+      if (lower) { if (unit) EAMD_TRSV1(false, true, true); else EAMD_TRSV1(false, true, false); }
+      else { if (unit) EAMD_TRSV1(false, false, true); else EAMD_TRSV1(false, false, false); }
+    }
+#undef EAMD_TRSV1
+    EAMD_HIP(hipGetLastError());
+    return kOk;
+  }
   for (int c0 = 0; c0 < nrhs; c0 += NRC) {
     const unsigned epoch = (unsigned)++h->trsv_epoch;
 #define EAMD_TRSV(RM, LO, UN)                                                                   \
