@@ -210,7 +210,8 @@ __device__ __forceinline__ void leaf_perm(int& x, int w, int lane, int& tpos, in
 // The panel's first leaf (row0 == 0) starts the list as the identity.  One
 // wave; s_pp is LDS scratch of at least pnb + 4 ints.
 __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int lane,
-                              int* pperm_n, int* ppos, int* psrc, int* s_pp) {
+                              int* pperm_n, int* ppos, int* psrc, int* s_pp, const int* s_ps,
+                              int pn0_pre) {
   const bool first = row0 == 0;
   if (first) {
     for (int e = lane; e < pnb; e += 64) {
@@ -220,8 +221,12 @@ __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int
     // complete before the touched slots are stored over them
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   }
-  const int pn0 = first ? pnb : *pperm_n, na = pn0 - pnb, na4 = (na + 3) & ~3;
-  for (int e = lane; e < na4; e += 64) s_pp[e] = e < na ? ppos[pnb + e] : -1;
+  // pn0_pre >= 0: the incoming list was prefetched (count pn0_pre, appended
+  // rows in s_pp padded with -1 to a multiple of 4, sources in s_ps)
+  const bool pre = pn0_pre >= 0;
+  const int pn0 = first ? pnb : (pre ? pn0_pre : *pperm_n), na = pn0 - pnb, na4 = (na + 3) & ~3;
+  if (!pre)
+    for (int e = lane; e < na4; e += 64) s_pp[e] = e < na ? ppos[pnb + e] : -1;
   asm volatile("" ::: "memory");  // LDS accesses of one wave execute in order
   const bool mine = lane < ln;
   const int prow = row0 + tpos;
@@ -242,7 +247,7 @@ __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int
   const u64 nb = __ballot(need);
   if (need) slot = pn0 + __popcll(nb & ((1ull << lane) - 1ull));
   int old = slot;
-  if (mine && !need && !first) old = psrc[slot];
+  if (mine && !need && !first) old = pre ? s_ps[slot] : psrc[slot];
   const int src_old = __shfl(old, tsrc);
   if (mine) {
     psrc[slot] = src_old;
@@ -257,7 +262,8 @@ __device__ void panel_compose(int tpos, int tsrc, int ln, int row0, int pnb, int
 template <bool FROM_SEL>
 __device__ void leaf_lists(int x, int w, int row0, int pnb, int lane, int* piv, int piv_offset,
                            int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos,
-                           int* psrc, int* s_pp, int& tpos, int& trow, int& tsrc, int& ln) {
+                           int* psrc, int* s_pp, int& tpos, int& trow, int& tsrc, int& ln,
+                           const int* s_ps = nullptr, int pn0_pre = -1) {
   leaf_perm<FROM_SEL>(x, w, lane, tpos, trow, tsrc, ln);
   if (lane < w) piv[lane] = x + piv_offset;
   if (lane < ln) {
@@ -265,7 +271,7 @@ __device__ void leaf_lists(int x, int w, int row0, int pnb, int lane, int* piv, 
     lsrc[lane] = tsrc;
   }
   if (lane == 0) *lperm_n = ln;
-  panel_compose(tpos, tsrc, ln, row0, pnb, lane, pperm_n, ppos, psrc, s_pp);
+  panel_compose(tpos, tsrc, ln, row0, pnb, lane, pperm_n, ppos, psrc, s_pp, s_ps, pn0_pre);
 }
 
 }  // namespace
@@ -592,6 +598,7 @@ namespace {
 constexpr int kTslR = 4;                 // rows per lane
 constexpr int kTslBlock = 64 * kTslR;    // rows (candidates) per wave
 constexpr int kTslFan = kTslBlock / LW;  // candidate groups merged per wave
+constexpr int kMoveCols = 96;            // panel columns moved per pass in the final round
 
 // Maximum of a 32-bit key over the 64 lanes of a wave: xor-1 and xor-2
 // within quads, half-row and row mirrors within each 16-lane row, then the
@@ -644,6 +651,10 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   __shared__ __align__(16) int s_pp[256 + 4];
   __shared__ __align__(16) float s_u[LW];  // the pivot row of the current column
   const int lane = threadIdx.x;
+  // MODE 1: the incoming panel list's count (the lists themselves are
+  // prefetched below, while the columns are eliminated)
+  int pn0_pre = -1;
+  if (MODE == 1) pn0_pre = row0 == 0 ? pnb : *pperm_n;
   // MODE 1: the original rows 0..w-1 (saved before the top block overwrites
   // them), loaded first and stored last
   float sv[LW * LW / 64];
@@ -704,6 +715,20 @@ __global__ __launch_bounds__(64) void tsl_kernel(
       const float* src = P + (int64_t)orig[r] * ldp;
 #pragma unroll
       for (int c = 0; c < LW; ++c) v[r][c] = src[min(c, w - 1)];
+    }
+  }
+  int ppv[4], psv[8];
+  if (MODE == 1) {
+    const bool has = row0 != 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = 64 * q + lane;
+      ppv[q] = (has && e < pn0_pre - pnb) ? ppos[pnb + e] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = 64 * q + lane;
+      psv[q] = (has && e < pn0_pre) ? psrc[e] : 0;
     }
   }
   const int steps = min(w, nrows);
@@ -835,9 +860,15 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     }
   __syncthreads();
   // ---- transpositions and permutation lists from the selection order
+  __shared__ int s_ps[512];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) s_pp[64 * q + lane] = ppv[q];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s_ps[64 * q + lane] = psv[q];
+  asm volatile("" ::: "memory");  // a wave's LDS accesses execute in order
   int tpos, trow, tsrc, ln;
   leaf_lists<true>(lane < w ? s_sel[lane] : 0, w, row0, pnb, lane, piv, piv_offset, lperm_n,
-                   lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, tsrc, ln);
+                   lpos, lsrc, pperm_n, ppos, psrc, s_pp, tpos, trow, tsrc, ln, s_ps, pn0_pre);
   // positions >= w now hold displaced original rows < w: list them
   const bool mv = tpos >= w && lane < ln;
   const u64 mb = __ballot(mv);
@@ -850,33 +881,33 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   if (fuse) {
     // (w == 32) The leaf's interchanges on the other columns of the panel
     // (row0 = j0 is the leaf's first panel column, pnb the panel width): the
-    // lane of each touched row moves its source row's values, 32 columns at
+    // lane of each touched row moves its source row's values, 96 columns at
     // a time (all loads of a chunk before its stores); the top rows' values
     // right of the leaf go to xtop (LW x (pnb - j0 - LW)), from which the
     // apply kernel solves U12.
     const int srow = __shfl(tpos, tsrc);
     const int j0 = row0, nc = pnb - LW, rest = pnb - j0 - LW;
     float* base = P - j0;  // panel column 0 of leaf row 0
+    // 16-byte words: the host fuses only panels whose rows start 16-byte
+    // aligned with pnb and ldp multiples of 4 (a word never straddles the
+    // leaf, whose columns start at a multiple of 32)
     if (lane < ln) {
-      const float* sp = base + (int64_t)srow * ldp;
-      float* dp = base + (int64_t)tpos * ldp;
-      float* xp = tpos < LW ? xtop + tpos * rest - j0 : nullptr;
-      for (int c0 = 0; c0 < nc; c0 += 32) {
-        float x[32];
+      const v4f* sp = reinterpret_cast<const v4f*>(base + (int64_t)srow * ldp);
+      v4f* dp = reinterpret_cast<v4f*>(base + (int64_t)tpos * ldp);
+      v4f* xp = tpos < LW ? reinterpret_cast<v4f*>(xtop + tpos * rest) : nullptr;
+      const int nq = nc / 4;
+      for (int q0 = 0; q0 < nq; q0 += kMoveCols / 4) {
+        v4f x[kMoveCols / 4];
 #pragma unroll
-        for (int t = 0; t < 32; ++t) {
-          const int c = min(c0 + t, nc - 1), col = c < j0 ? c : c + LW;
-          x[t] = sp[col];
+        for (int t = 0; t < kMoveCols / 4; ++t) {
+          const int c = 4 * min(q0 + t, nq - 1);
+          x[t] = sp[(c < j0 ? c : c + LW) / 4];
         }
 #pragma unroll
-        for (int t = 0; t < 32; ++t) {
-          const int c = c0 + t, col = c < j0 ? c : c + LW;
-          if (c < nc) {
-            if (xp && c >= j0)
-              xp[c] = x[t];
-            else
-              dp[col] = x[t];
-          }
+        for (int t = 0; t < kMoveCols / 4; ++t) {
+          const int c = 4 * (q0 + t);
+          v4f* d = xp && c >= j0 ? xp + (c - j0) / 4 : dp + (c < j0 ? c : c + LW) / 4;
+          if (q0 + t < nq) *d = x[t];
         }
       }
     }
@@ -1045,7 +1076,8 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   // the leaf's interchanges on the other panel columns, the U rows right of
   // it and their update are done here too when w == 32 (the caller skips
   // its own)
-  const bool fuse = fused && w == LW && G1 > 1 && pnb - row0 - w <= kFuseRestMax;
+  const bool fuse = fused && w == LW && G1 > 1 && pnb - row0 - w <= kFuseRestMax &&
+                    ((uintptr_t)(P - row0) & 15) == 0 && ldp % 4 == 0 && pnb % 4 == 0;
   if (fused) *fused = fuse;
   if (G1 <= 1)
     return leaf_lu<float>(h, P, ldp, m, w, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
