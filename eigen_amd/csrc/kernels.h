@@ -25,6 +25,9 @@ struct eamd_batched_plan {
 
 namespace eamd {
 
+// integer environment setting, dflt when unset or empty
+int env_int(const char* name, int dflt);
+
 // ---- level 3 (gemm_f64.hip) ----
 // C = alpha*op(A)*op(B) + beta*C, column-major.  tri: 0 full C, 1 only
 // i >= j (lower SYRK), 2 only i <= j (upper).  abort_flag (nullable, device):

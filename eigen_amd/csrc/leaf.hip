@@ -635,9 +635,12 @@ __device__ __forceinline__ void put_row(float* dst, const float (&v)[kTslR][LW],
 // registers.  MODE 0: local selection over rows blockIdx.x*256 + [0, 256),
 // candidates to cand.  MODE 2: merge of candidate groups [8 b, 8 b + 8) of
 // cand_in into group b of cand.  MODE 1: the final round over all G_in
-// (<= 8) groups.
-template <int MODE>
-__global__ __launch_bounds__(64) void tsl_kernel(
+// (<= 8) groups.  With NW > 1 waves a workgroup covers NW such blocks (MODE
+// 0: NW*256 rows; MODE 1: up to 8*NW groups, wave q holding groups
+// [8q, 8q + 8)): every column's pivot is the maximum over the waves' own
+// maxima, exchanged through LDS under one barrier.
+template <int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void tsl_kernel(
     float* __restrict__ P, int64_t ldp, int m, int w, const int* __restrict__ cand_in,
     const int* __restrict__ cand_n_in, int G_in, int* __restrict__ cand,
     int* __restrict__ cand_n, float* __restrict__ saved, int* __restrict__ moved,
@@ -646,11 +649,19 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
     int* __restrict__ fz_global, int fuse, float* __restrict__ xtop) {
   constexpr int R = kTslR;
-  __shared__ int s_off[kTslFan + 1];
+  static_assert(MODE != 2 || NW == 1, "merge rounds run one wave per workgroup");
+  // slot ids in the key's low KB bits: 2^KB = 2 * NW * 256 keeps every
+  // id's code (2^KB - 1 - slot) nonzero
+  constexpr int KB = NW == 1 ? 9 : (NW == 2 ? 10 : 11);
+  constexpr unsigned kSlotMask = (1u << KB) - 1u;
+  static_assert((1 << KB) == 2 * NW * kTslBlock, "slot code width");
   __shared__ int s_sel[LW];
   __shared__ __align__(16) int s_pp[256 + 4];
   __shared__ __align__(16) float s_u[LW];  // the pivot row of the current column
-  const int lane = threadIdx.x;
+  __shared__ __align__(16) float s_rows[NW > 1 ? 2 : 1][NW][LW];  // each wave's best row
+  __shared__ unsigned s_key[NW > 1 ? 2 : 1][NW];                  // and its key
+  const int lane = threadIdx.x & 63;
+  const int wave = NW > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
   // MODE 1: the incoming panel list's count (the lists themselves are
   // prefetched below, while the columns are eliminated)
   int pn0_pre = -1;
@@ -658,7 +669,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   // MODE 1: the original rows 0..w-1 (saved before the top block overwrites
   // them), loaded first and stored last
   float sv[LW * LW / 64];
-  if (MODE == 1) {
+  if (MODE == 1 && wave == 0) {
 #pragma unroll
     for (int k = 0; k < LW * LW / 64; ++k) {
       const int e = lane + 64 * k, r = e / LW, c = e % LW;
@@ -667,36 +678,51 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   }
   int orig[R];
   bool valid[R];
-  int nrows;
+  int nrows;  // candidates of the workgroup
   if (MODE == 0) {
-    const int base = blockIdx.x * kTslBlock;
-    nrows = min(kTslBlock, m - base);
+    const int wg0 = blockIdx.x * (NW * kTslBlock), base = wg0 + wave * kTslBlock;
+    nrows = min(NW * kTslBlock, m - wg0);
+    const int nw = min(kTslBlock, m - base);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int q = r * 64 + lane;
-      valid[r] = q < nrows;
+      valid[r] = q < nw;
       orig[r] = valid[r] ? base + q : 0;
     }
   } else {
-    const int g0 = MODE == 2 ? blockIdx.x * kTslFan : 0;
-    const int ng = min(kTslFan, G_in - g0);
-    if (lane == 0) {
-      int acc = 0;
-      for (int g = 0; g < ng; ++g) { s_off[g] = acc; acc += cand_n_in[g0 + g]; }
-      s_off[ng] = acc;
+    // the workgroup's group counts, one per lane, and their inclusive
+    // prefix sums; wave q's groups are [8q, 8q + 8) of the workgroup's
+    constexpr int NG = NW * kTslFan;
+    const int gb = MODE == 2 ? blockIdx.x * kTslFan : 0;
+    const int nga = max(0, min(NG, G_in - gb));
+    int inc = lane < nga ? cand_n_in[gb + lane] : 0;
+#pragma unroll
+    for (int d = 1; d < NG; d <<= 1) {
+      const int t = __shfl_up(inc, d);
+      if (lane >= d) inc += t;
     }
-    __syncthreads();
-    nrows = s_off[ng];
+    nrows = readlane_i(inc, NG - 1);
+    const int wg = wave * kTslFan;
+    const int pre = wave > 0 ? readlane_i(inc, wg - 1) : 0;
+    int off[kTslFan + 1];  // wave-uniform offsets of the wave's groups
+    off[0] = 0;
+#pragma unroll
+    for (int g = 0; g < kTslFan; ++g) off[g + 1] = readlane_i(inc, wg + g) - pre;
+    const int nw = off[kTslFan];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int q = r * 64 + lane;
-      orig[r] = 0;
-      valid[r] = q < nrows;
-      if (valid[r]) {
-        int g = 0;
-        while (s_off[g + 1] <= q) ++g;
-        orig[r] = cand_in[(g0 + g) * LW + q - s_off[g]];
+      valid[r] = q < nw;
+      // group of slot q: the last g whose offset is <= q (empty groups
+      // share their successor's offset)
+      int gs = 0, o = 0;
+#pragma unroll
+      for (int g = 1; g < kTslFan; ++g) {
+        const bool ge = q >= off[g];
+        gs = ge ? g : gs;
+        o = ge ? off[g] : o;
       }
+      orig[r] = valid[r] ? cand_in[(gb + wg + gs) * LW + q - o] : 0;
     }
   }
   // An empty slot loads row 0 and a leaf narrower than 32 repeats its last
@@ -719,7 +745,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   }
   int ppv[4], psv[8];
   if (MODE == 1) {
-    const bool has = row0 != 0;
+    const bool has = row0 != 0 && wave == 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = 64 * q + lane;
@@ -733,14 +759,15 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   }
   const int steps = min(w, nrows);
   int first_zero = -1;
-  // Rows never move between lanes.  Slot q = 64 r + lane holds a candidate
-  // row while em[r] is all ones (valid and not yet selected); st[r] is the
-  // step that selected it.  The pivot key of slot q is |a| with its 9 low
-  // mantissa bits replaced by 511 - q (km = em & 0x7ffffe00, kq = em &
-  // (511 - q); zero for a slot that is out), so the maximum names its slot,
-  // exact ties go to the lowest slot (the first row in order for the local
-  // round) and scores within 2^-14 relative count as ties, which keeps every
-  // multiplier within 1 + 2^-14.  Column j of group g8 is register slot jj
+  // Rows never move between lanes.  Slot q = 256 wave + 64 r + lane holds a
+  // candidate row while em[r] is all ones (valid and not yet selected);
+  // st[r] is the step that selected it.  The pivot key of slot q is |a| with
+  // its KB low mantissa bits replaced by 2^KB - 1 - q (km = em & 0x7fffffff
+  // with those bits clear, kq = em & (2^KB - 1 - q); zero for a slot that is
+  // out), so the maximum names its slot, exact ties go to the lowest slot
+  // (the first row in order for the local round) and scores within
+  // 2^(KB-23) relative count as ties, which keeps every multiplier within
+  // 1 + 2^(KB-23) (2^-14 for one wave, 2^-12 for four).  Column j of group g8 is register slot jj
   // (see rotate8); slots [live, 32) hold columns of earlier groups and are
   // skipped by uniform branches on 8-slot chunks, so the update does exactly
   // the triangle.  The multipliers use the reciprocal of the pivot.
@@ -749,8 +776,8 @@ __global__ __launch_bounds__(64) void tsl_kernel(
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     em[r] = valid[r] ? ~0u : 0u;
-    km[r] = em[r] & 0x7ffffe00u;
-    kq[r] = em[r] & (0x1ffu - (unsigned)(r * 64 + lane));
+    km[r] = em[r] & (0x7fffffffu & ~kSlotMask);
+    kq[r] = em[r] & (kSlotMask - (unsigned)(wave * kTslBlock + r * 64 + lane));
     st[r] = -1;
   }
   auto column = [&](auto jjc, int j, int live) __attribute__((always_inline)) {
@@ -760,18 +787,38 @@ __global__ __launch_bounds__(64) void tsl_kernel(
     for (int r = 0; r < R; ++r) kr[r] = (__float_as_uint(v[r][jj]) & km[r]) | kq[r];
     const unsigned key = max(max(kr[0], kr[1]), max(kr[2], kr[3]));
     // a valid unselected slot exists (j < steps), so the winner does
-    const int qw = 0x1ff - (int)(wave_max_u32(key) & 0x1ffu);
-    const int rs = qw >> 6, wl = qw & 63;
-    const bool me = lane == wl;
-    // the pivot row: stored by its lane, read back by all (a wave's LDS
-    // accesses execute in order; the barriers keep the compiler from moving
-    // the reads above the store)
-    put_row(s_u, v, rs, me);
-    asm volatile("" ::: "memory");
+    const unsigned wkey = wave_max_u32(key);
+    const int ql = (int)(kSlotMask - (wkey & kSlotMask)) - wave * kTslBlock;
+    int rs = ql >> 6, wl = ql & 63;
+    bool me = lane == wl;
+    const float* ub = s_u;
+    if constexpr (NW == 1) {
+      // the pivot row: stored by its lane, read back by all (a wave's LDS
+      // accesses execute in order; the barriers keep the compiler from
+      // moving the reads above the store)
+      put_row(s_u, v, rs, me);
+      asm volatile("" ::: "memory");
+    } else {
+      // every wave stores its best row and key (a wave without candidates
+      // has key 0 and decodes a slot outside its 256, so stores nothing),
+      // then all take the largest key: its wave's row is the pivot row.
+      // Column parity double-buffers the exchange: the stores of column
+      // j + 2 follow the barrier of column j + 1, after every read of j.
+      const int par = j & 1;
+      put_row(s_rows[par][wave], v, rs, me);
+      if (lane == 0) s_key[par][wave] = wkey;
+      __syncthreads();
+      unsigned gkey = s_key[par][0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q) gkey = max(gkey, s_key[par][q]);
+      const int gw = (int)(kSlotMask - (gkey & kSlotMask)) >> 8;
+      me = me && gw == wave;
+      ub = s_rows[par][gw];
+    }
     float u[LW];
 #pragma unroll
     for (int q = 0; q < LW / 4; ++q) {
-      const float4 t = reinterpret_cast<const float4*>(s_u)[q];
+      const float4 t = reinterpret_cast<const float4*>(ub)[q];
       u[4 * q] = t.x;
       u[4 * q + 1] = t.y;
       u[4 * q + 2] = t.z;
@@ -835,7 +882,7 @@ __global__ __launch_bounds__(64) void tsl_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (st[r] >= 0) cand[blockIdx.x * LW + st[r]] = orig[r];
-    if (lane == 0) cand_n[blockIdx.x] = steps;
+    if (threadIdx.x == 0) cand_n[blockIdx.x] = steps;
     return;
   }
   if constexpr (MODE == 1) {
@@ -843,10 +890,12 @@ __global__ __launch_bounds__(64) void tsl_kernel(
   for (int q = ngroups; q < 4; ++q)
 #pragma unroll
     for (int r = 0; r < R; ++r) rotate8(v[r]);
+  if (wave == 0) {
 #pragma unroll
-  for (int k = 0; k < LW * LW / 64; ++k) {
-    const int e = lane + 64 * k;
-    saved[e] = e % LW < w ? sv[k] : 0.f;
+    for (int k = 0; k < LW * LW / 64; ++k) {
+      const int e = lane + 64 * k;
+      saved[e] = e % LW < w ? sv[k] : 0.f;
+    }
   }
   // ---- final round: the top block in place (rows 0..w-1) and the selection
 #pragma unroll
@@ -859,6 +908,8 @@ __global__ __launch_bounds__(64) void tsl_kernel(
       s_sel[st[r]] = orig[r];
     }
   __syncthreads();
+  // the rest is one wave's (no barrier follows)
+  if (NW > 1 && wave != 0) return;
   // ---- transpositions and permutation lists from the selection order
   __shared__ int s_ps[512];
 #pragma unroll
@@ -1100,23 +1151,42 @@ int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int*
   float* saved = reinterpret_cast<float*>(moved + 2 * LW + 8);
   float* xtop = saved + LW * LW;  // permuted top rows right of the leaf (fused)
   hipStream_t s = h->stream;
-  hipLaunchKernelGGL(tsl_kernel<0>, dim3(G1), dim3(64), 0, s, P, ldp, m, w, nullptr,
-                     nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos,
-                     psrc, row0, pnb, zero_col, piv_offset, fz_global, 0, xtop);
-  int G = G1;
+  // Rounds: one wave per 256 rows, or per 512 rows in pairs when more than
+  // 32 groups would follow; merges of 8 groups while more than 32 remain;
+  // the final round in one workgroup (one wave for up to 8 groups, four
+  // waves for up to 32).
+  static const bool wide = env_int("EAMD_TSL_WIDE", 1) != 0;
+  const int fan1 = wide ? 4 * kTslFan : kTslFan;
+  int G;
+  if (wide && G1 > fan1) {
+    G = ceil_div(m, 2 * kTslBlock);
+    hipLaunchKernelGGL((tsl_kernel<0, 2>), dim3(G), dim3(128), 0, s, P, ldp, m, w, nullptr,
+                       nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n,
+                       ppos, psrc, row0, pnb, zero_col, piv_offset, fz_global, 0, xtop);
+  } else {
+    G = G1;
+    hipLaunchKernelGGL((tsl_kernel<0, 1>), dim3(G), dim3(64), 0, s, P, ldp, m, w, nullptr,
+                       nullptr, 0, candA, nA, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n,
+                       ppos, psrc, row0, pnb, zero_col, piv_offset, fz_global, 0, xtop);
+  }
   int *cin = candA, *nin = nA, *cout = candB, *nout = nB;
-  while (G > kTslFan) {
+  while (G > fan1) {
     const int Gn = ceil_div(G, kTslFan);
-    hipLaunchKernelGGL(tsl_kernel<2>, dim3(Gn), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
+    hipLaunchKernelGGL((tsl_kernel<2, 1>), dim3(Gn), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
                        cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
                        row0, pnb, zero_col, piv_offset, fz_global, 0, xtop);
     G = Gn;
     std::swap(cin, cout);
     std::swap(nin, nout);
   }
-  hipLaunchKernelGGL(tsl_kernel<1>, dim3(1), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
-                     cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0,
-                     pnb, zero_col, piv_offset, fz_global, fuse ? 1 : 0, xtop);
+  if (G > kTslFan)
+    hipLaunchKernelGGL((tsl_kernel<1, 4>), dim3(1), dim3(256), 0, s, P, ldp, m, w, cin, nin, G,
+                       cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
+                       row0, pnb, zero_col, piv_offset, fz_global, fuse ? 1 : 0, xtop);
+  else
+    hipLaunchKernelGGL((tsl_kernel<1, 1>), dim3(1), dim3(64), 0, s, P, ldp, m, w, cin, nin, G,
+                       cout, nout, saved, moved, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc,
+                       row0, pnb, zero_col, piv_offset, fz_global, fuse ? 1 : 0, xtop);
   const int rest = pnb - row0 - w;
   if (fuse && rest > 0) {
     static bool attr_set = false;

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
                                                          const int* __restrict__ pos,
                                                          const int* __restrict__ src) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr int XP = CB + 1, U = 8;
+  constexpr int XP = CB + 1;
   const int nbp = (nb + 31) & ~31, LP = nbp + 1;
   float* Lf = reinterpret_cast<float*>(smem_raw);  // nbp x LP
   float* Xs = Lf + nbp * LP;                       // nbp x XP
@@ -167,38 +167,36 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CB;
   const int cw = min(CB, ncols - c0);
+  // L (strictly lower part; zero elsewhere) with the permutation lists, then
+  // the touched rows: every load in flight before the first LDS store (nb <=
+  // 128: at most 64 L words and 2 nb CB / 256 row words per thread)
+  constexpr int NL = 128 * 128 / 256, NR = 2 * 128 * CB / 256;
+  float lv[NL], rv[NR];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int e = tid + 256 * j, r = e / nbp, q = e - r * nbp;
+    lv[j] = (e < nbp * nbp && q < r && r < nb) ? L[(int64_t)r * ldl + q] : 0.f;
+  }
   const int nS = *perm_n;
   for (int e = tid; e < nS; e += 256) {
     s_pos[e] = pos[e];
     s_src[e] = src[e];
   }
   __syncthreads();
-  // L (strictly lower part; zero elsewhere) and the touched rows
-  for (int e0 = tid; e0 < nbp * nbp; e0 += 256 * U) {
-    float v[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int e = e0 + 256 * j, r = e / nbp, q = e - r * nbp;
-      v[j] = (e < nbp * nbp && q < r && r < nb) ? L[(int64_t)r * ldl + q] : 0.f;
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int e = e0 + 256 * j, r = e / nbp, q = e - r * nbp;
-      if (e < nbp * nbp) Lf[r * LP + q] = v[j];
-    }
+  for (int j = 0; j < NR; ++j) {
+    const int e = tid + 256 * j, sl = e / CB, c = e - sl * CB;
+    rv[j] = (e < nS * CB && c < cw) ? X[(int64_t)s_pos[sl] * ldx + c0 + c] : 0.f;
   }
-  for (int e0 = tid; e0 < nS * CB; e0 += 256 * U) {
-    float v[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int e = e0 + 256 * j, sl = e / CB, c = e - sl * CB;
-      v[j] = (e < nS * CB && c < cw) ? X[(int64_t)s_pos[sl] * ldx + c0 + c] : 0.f;
-    }
+  for (int j = 0; j < NL; ++j) {
+    const int e = tid + 256 * j, r = e / nbp, q = e - r * nbp;
+    if (e < nbp * nbp) Lf[r * LP + q] = lv[j];
+  }
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const int e = e0 + 256 * j;
-      if (e < nS * CB) buf[e] = v[j];
-    }
+  for (int j = 0; j < NR; ++j) {
+    const int e = tid + 256 * j;
+    if (e < nS * CB) buf[e] = rv[j];
   }
   __syncthreads();
   for (int e = tid; e < nbp * CB; e += 256) {

@@ -58,6 +58,19 @@ def test_ir_ill_conditioned_householder(gpu, prec):
     assert po.backward_error(a, x.cpu().numpy(), b) <= 1e-14
 
 
+@pytest.mark.parametrize("n", [3000, 9000])
+def test_ir_tall_panels(gpu, n):
+    """Panels taller than 2048 rows take the four-wave final tournament round
+    (up to 32 candidate groups), and taller than 8192 rows the two-wave
+    first round: the refined solution still meets the FP64 bound."""
+    a = inputs.normal(40 + n, n, n)
+    b = inputs.normal(41 + n, n)
+    x, rep = mixed.gesv_ir(dev(a), dev(b).reshape(-1), precision="fp16x3")
+    assert rep["converged"] == 1 and rep["used_gmres"] == 0, rep
+    assert rep["ir_iterations"] <= 6, rep
+    assert po.backward_error(a, x.cpu().numpy(), b) <= 1e-14
+
+
 def test_ir_multiple_rhs(gpu):
     n = 700
     a = inputs.normal(7, n, n)
