@@ -728,7 +728,22 @@ __global__ __launch_bounds__(64 * NW) void tsl_kernel(
   // An empty slot loads row 0 and a leaf narrower than 32 repeats its last
   // column: such values are never keyed, selected or stored.
   float v[R][LW];
-  if (w == LW) {
+  if (w == LW && ((uintptr_t)P & 15) == 0 && (ldp & 3) == 0) {
+    // 16-byte loads: a quarter of the address work of word loads, which
+    // otherwise saturates the CU's address unit when several waves load
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const v4f* src = reinterpret_cast<const v4f*>(P + (int64_t)orig[r] * ldp);
+#pragma unroll
+      for (int q = 0; q < LW / 4; ++q) {
+        const v4f t = src[q];
+        v[r][4 * q] = t.x;
+        v[r][4 * q + 1] = t.y;
+        v[r][4 * q + 2] = t.z;
+        v[r][4 * q + 3] = t.w;
+      }
+    }
+  } else if (w == LW) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const float* src = P + (int64_t)orig[r] * ldp;
@@ -902,9 +917,16 @@ __global__ __launch_bounds__(64 * NW) void tsl_kernel(
   for (int r = 0; r < R; ++r)
     if (st[r] >= 0) {
       float* dst = P + (int64_t)st[r] * ldp;
+      if (w == LW && ((uintptr_t)P & 15) == 0 && (ldp & 3) == 0) {
 #pragma unroll
-      for (int c = 0; c < LW; ++c)
-        if (c < w) dst[c] = v[r][c];
+        for (int q = 0; q < LW / 4; ++q)
+          reinterpret_cast<v4f*>(dst)[q] =
+              v4f{v[r][4 * q], v[r][4 * q + 1], v[r][4 * q + 2], v[r][4 * q + 3]};
+      } else {
+#pragma unroll
+        for (int c = 0; c < LW; ++c)
+          if (c < w) dst[c] = v[r][c];
+      }
       s_sel[st[r]] = orig[r];
     }
   __syncthreads();
@@ -998,9 +1020,19 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
   for (int k = 0; k < s_nm; ++k)
     if (mpos[k] == i) src = msrc[k];
   float a[LW];
-  if (src >= 0) {
+  // 16-byte words when the rows allow (saved rows always do)
+  const bool vec = FULL && ((uintptr_t)P & 15) == 0 && (ldp & 3) == 0;
+  if (src >= 0 || vec) {
+    const v4f* ar = reinterpret_cast<const v4f*>(src >= 0 ? saved + src * LW
+                                                          : P + (int64_t)i * ldp);
 #pragma unroll
-    for (int c = 0; c < LW; ++c) a[c] = saved[src * LW + c];
+    for (int q = 0; q < LW / 4; ++q) {
+      const v4f t = ar[q];
+      a[4 * q] = t.x;
+      a[4 * q + 1] = t.y;
+      a[4 * q + 2] = t.z;
+      a[4 * q + 3] = t.w;
+    }
   } else {
     const float* row = P + (int64_t)i * ldp;
 #pragma unroll
@@ -1015,9 +1047,15 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
     for (int c = k + 1; c < LW; ++c) a[c] -= x * U[k][c];
   }
   float* dst = P + (int64_t)i * ldp;
+  if (vec) {
 #pragma unroll
-  for (int c = 0; c < LW; ++c)
-    if (FULL || c < w) dst[c] = a[c];
+    for (int q = 0; q < LW / 4; ++q)
+      reinterpret_cast<v4f*>(dst)[q] = v4f{a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};
+  } else {
+#pragma unroll
+    for (int c = 0; c < LW; ++c)
+      if (FULL || c < w) dst[c] = a[c];
+  }
 }
 
 
@@ -1075,10 +1113,19 @@ __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__
   int src = -1;
   for (int k = 0; k < s_nm; ++k)
     if (mpos[k] == i) src = msrc[k];
+  // 16-byte words throughout: the host fuses only panels whose rows start
+  // 16-byte aligned, with ldp and rest multiples of 4
   float a[LW];
-  const float* arow = src >= 0 ? saved + src * LW : P + (int64_t)i * ldp;
+  const v4f* arow = reinterpret_cast<const v4f*>(src >= 0 ? saved + src * LW
+                                                          : P + (int64_t)i * ldp);
 #pragma unroll
-  for (int c = 0; c < LW; ++c) a[c] = arow[c];
+  for (int c = 0; c < LW / 4; ++c) {
+    const v4f t = arow[c];
+    a[4 * c] = t.x;
+    a[4 * c + 1] = t.y;
+    a[4 * c + 2] = t.z;
+    a[4 * c + 3] = t.w;
+  }
 #pragma unroll
   for (int k = 0; k < LW; ++k) {
     const float x = a[k] * dinv[k];
@@ -1089,7 +1136,8 @@ __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__
   float* row = P + (int64_t)i * ldp;
   if (q == 0) {
 #pragma unroll
-    for (int c = 0; c < LW; ++c) row[c] = a[c];
+    for (int c = 0; c < LW / 4; ++c)
+      reinterpret_cast<v4f*>(row)[c] = v4f{a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]};
   }
   row += LW;
   for (int c0 = q; c0 < rest; c0 += kApplyTpr * 8) {
