@@ -409,6 +409,96 @@ __global__ __launch_bounds__(256) void f32_update_kernel(int M, int N, int K,
   }
 }
 
+// FP32 update of at most 128 columns with an inner dimension of at most 128
+// (the next panel's columns in the look-ahead LU, a leaf's panel columns):
+// C (N rows x M columns) -= A21 (N x K) A12 (K x M), row-major, ld.  One
+// workgroup per 32 rows, each wave 32 of the columns.  Every operand word is
+// loaded up front (the whole A12, the workgroup's A21 rows and its C tile),
+// so a single memory round trip precedes the FP32 MFMA work; K passes
+// through LDS in two halves of 64.  M, K and ld are multiples of 4 and the
+// operands 16-byte aligned (checked by the caller).
+__global__ __launch_bounds__(256) void f32_update_narrow_kernel(int M, int N, int K,
+                                                                const float* __restrict__ A12,
+                                                                const float* __restrict__ A21,
+                                                                int64_t ld, float* __restrict__ C) {
+  constexpr int NT = 32, KH = 64;
+  __shared__ float As[KH][128 + 4];  // A12: k x column
+  __shared__ float Bs[KH][NT + 4];   // A21 transposed: k x row
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n0 = blockIdx.x * NT;
+  const int l16 = lane & 15, lk = lane >> 4, wm = wave * 32;
+  // A12 rows k = tid/32 + 8t, words (tid%32); A21 row tid/8, words (tid%8) + 8t
+  v4f ra[16], rb[4];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int k = (tid >> 5) + 8 * t, i = (tid & 31) * 4;
+    ra[t] = (k < K && i < M) ? *reinterpret_cast<const v4f*>(A12 + (int64_t)k * ld + i)
+                             : v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  {
+    const int j = n0 + (tid >> 3);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = (tid & 7) * 4 + 32 * t;
+      rb[t] = (j < N && k < K) ? *reinterpret_cast<const v4f*>(A21 + (int64_t)j * ld + k)
+                               : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  }
+  // the C tile in the MFMA result layout: row n0 + 16 j + 4 lk + r, column
+  // wm + 16 i + l16
+  float cv[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + j * 16 + lk * 4 + r, gi = wm + i * 16 + l16;
+        cv[i][j][r] = (gj < N && gi < M) ? C[(int64_t)gj * ld + gi] : 0.f;
+      }
+  v4f acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();  // every read of the first half is done
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      *reinterpret_cast<v4f*>(&As[(tid >> 5) + 8 * t][(tid & 31) * 4]) = ra[8 * h + t];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int kk = (tid & 7) * 4 + 32 * t;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) Bs[kk + e][tid >> 3] = rb[2 * h + t][e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < KH / 4; ++k4) {
+      float af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = As[k4 * 4 + lk][wm + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[k4 * 4 + lk][j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + j * 16 + lk * 4 + r, gi = wm + i * 16 + l16;
+        if (gj < N && gi < M) C[(int64_t)gj * ld + gi] = cv[i][j][r] - acc[i][j][r];
+      }
+}
+
 }  // namespace
 
 // row-major A22 (mrows x ncols) -= A21 (mrows x k) * A12 (k x ncols), ld
@@ -497,9 +587,16 @@ int f32_update_rowmajor(eamd_handle* h, const float* A12, const float* A21, floa
   const bool in_panel = phase == kPhPanel;
   PhaseScope ps(h, phase, in_panel ? 0.0 : 4.0 * (2.0 * M * N + (double)k * (M + N)),
                 in_panel ? 0.0 : 2.0 * M * N * k);
-  const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
-  hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, h->stream, M, N, k, A12, A21,
-                     ld, A22);
+  const bool narrow = M <= 128 && k <= 128 && M % 4 == 0 && k % 4 == 0 && ld % 4 == 0 &&
+                      ((uintptr_t)A12 & 15) == 0 && ((uintptr_t)A21 & 15) == 0;
+  if (narrow) {
+    hipLaunchKernelGGL(f32_update_narrow_kernel, dim3(ceil_div(N, 32)), dim3(256), 0, h->stream,
+                       M, N, k, A12, A21, ld, A22);
+  } else {
+    const int tiles = ceil_div(M, MT) * ceil_div(N, MT);
+    hipLaunchKernelGGL(f32_update_kernel, dim3(tiles), dim3(256), 0, h->stream, M, N, k, A12,
+                       A21, ld, A22);
+  }
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
