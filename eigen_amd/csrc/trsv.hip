@@ -41,16 +41,18 @@ struct XWords {
 };
 
 template <typename T, bool ROWMAJOR, bool LOWER, bool UNIT, int BS, int NRC>
-__global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restrict__ A,
+__global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __restrict__ A,
                                                          int64_t lda, T* __restrict__ X,
                                                          int64_t ldx, int c0, int nrhs,
                                                          u64* __restrict__ xw, unsigned epoch,
                                                          int* __restrict__ err) {
-  static_assert(BS == 64 && (NRC == 4 || NRC == 1), "four threads per row of a 64-row block");
+  static_assert((BS == 64 || BS == 128) && (NRC == 4 || NRC == 1),
+                "four threads per row of a 64- or 128-row block");
+  constexpr int NT = 4 * BS;   // threads
   constexpr int SEG = BS / 4;  // columns of a predecessor block per thread
   constexpr int NW = XWords<T>::n;
   constexpr int CHUNK = BS * NRC * NW;          // words of one block's solution
-  constexpr int WPT = (CHUNK + 255) / 256;      // words staged per thread
+  constexpr int WPT = (CHUNK + NT - 1) / NT;    // words staged per thread
   __shared__ T Dinv[BS][BS + 1];
   __shared__ T Tb[BS][BS + 1];
   __shared__ T accs[BS][NRC];
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
 
   // ---- diagonal block into LDS and its inverse (independent of the
   // predecessors: overlaps their solves)
-  for (int e = tid; e < BS * BS; e += 256) {
+  for (int e = tid; e < BS * BS; e += NT) {
     const int rr = e / BS, cc = e - rr * BS;
     Tb[rr][cc] = (rr < bs && cc < bs) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + rr, rb0 + cc) : T(0);
     Dinv[rr][cc] = T(0);  // the padding of a partial last block stays exactly zero
@@ -107,12 +109,36 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
   for (int c = 0; c < NRC; ++c) acc[c] = T(0);
   const int ndep = LOWER ? b : nblk - 1 - b;
   const bool rvalid = r < bs;
+  // a row-major segment is contiguous: 16-byte loads when aligned (a quarter
+  // or half of the address work of element loads, which the CU's address
+  // unit otherwise limits)
+  constexpr int PW = 16 / sizeof(T);  // values per 16-byte word
+  const bool vec = ROWMAJOR && ((uintptr_t)A & 15) == 0 && lda % PW == 0;
   auto load_seg = [&](int d, T (&tv)[SEG]) {
     const int jb = LOWER ? d : nblk - 1 - d;
     const int cb = jb * BS + q * SEG;
+    if (vec && cb + SEG <= n) {
+      const T* p = tptr<T, ROWMAJOR>(A, lda, rb0 + r, cb);
 #pragma unroll
-    for (int i = 0; i < SEG; ++i)
-      tv[i] = (rvalid && cb + i < n) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + r, cb + i) : T(0);
+      for (int i = 0; i < SEG; i += PW) {
+        if (sizeof(T) == 4) {
+          const v4f w = rvalid ? *reinterpret_cast<const v4f*>(p + i) : v4f{0.f, 0.f, 0.f, 0.f};
+          tv[i] = (T)w.x;
+          tv[i + 1] = (T)w.y;
+          tv[i + 2] = (T)w.z;
+          tv[i + 3] = (T)w.w;
+        } else {
+          typedef double v2d __attribute__((ext_vector_type(2)));
+          const v2d w = rvalid ? *reinterpret_cast<const v2d*>(p + i) : v2d{0.0, 0.0};
+          tv[i] = (T)w.x;
+          tv[i + 1] = (T)w.y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < SEG; ++i)
+        tv[i] = (rvalid && cb + i < n) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + r, cb + i) : T(0);
+    }
   };
   T tcur[SEG], tnext[SEG];
   if (ndep > 0) load_seg(0, tcur);
@@ -124,7 +150,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
     // my words of the predecessor's solution: row jr, column c, half h
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {
-      const int widx = tid + k * 256;
+      const int widx = tid + k * NT;
       if (widx >= CHUNK) break;
       const int row = jb * BS + widx / (NRC * NW);
       const int c = (widx / NW) % NRC;
@@ -250,9 +276,11 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
                 (double)n * n * nrhs);
   if (nrhs == 1) {
     // one right-hand side: one exchange word (per 32 payload bits) per row
+    // (128-row blocks measured slower: 6.5 vs 5.8 ms of solves per step at
+    // n = 16384)
     const unsigned epoch = (unsigned)++h->trsv_epoch;
 #define EAMD_TRSV1(RM, LO, UN)                                                                 \
-  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, 1>), dim3(nblk), dim3(256), 0,      \
+  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, 1>), dim3(nblk), dim3(4 * BS), 0,   \
                      h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV1(true, true, true); else EAMD_TRSV1(true, true, false); }
