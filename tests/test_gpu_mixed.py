@@ -71,6 +71,23 @@ def test_ir_tall_panels(gpu, n):
     assert po.backward_error(a, x.cpu().numpy(), b) <= 1e-14
 
 
+@pytest.mark.parametrize("n", [3000, 9000])
+def test_mixed_factor_matches_fp32_factor(gpu, n):
+    """The look-ahead mixed factorization (FP16x3 trailing updates, fused
+    tournament leaves) is as good a preconditioner as the plain FP32 one:
+    after one refinement step its backward error stays within a small factor
+    of the FP32 factor's (a wrong update would still converge under
+    refinement, only slower, so the final bound alone does not show it)."""
+    a = inputs.normal(60 + n, n, n)
+    b = inputs.normal(61 + n, n)
+    one = dict(check_converged=False, max_iter=1, gmres=0)
+    _, r16 = mixed.gesv_ir(dev(a), dev(b).reshape(-1), precision="fp16x3", **one)
+    _, r32 = mixed.gesv_ir(dev(a), dev(b).reshape(-1), precision="fp32", **one)
+    print(f"n={n}: one step fp16x3 {r16['backward_error']:.3e} fp32 {r32['backward_error']:.3e}")
+    assert r16["ir_iterations"] == 1 and r32["ir_iterations"] == 1
+    assert r16["backward_error"] <= 20 * r32["backward_error"] + 1e-16, (r16, r32)
+
+
 def test_ir_multiple_rhs(gpu):
     n = 700
     a = inputs.normal(7, n, n)
