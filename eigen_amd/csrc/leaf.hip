@@ -1147,9 +1147,7 @@ __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__
       const int c = c0 + kApplyTpr * t;
       x[t] = c < rest ? row[c] : 0.f;
     }
-    // a[k] with compile-time k: a runtime index into the register array
-    // would go through indexed register moves
-#pragma unroll
+#pragma unroll 4
     for (int k = 0; k < LW; ++k)
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
