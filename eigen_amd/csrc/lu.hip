@@ -149,7 +149,9 @@ __global__ __launch_bounds__(256) void trsm_llu_kernel(const T* __restrict__ L, 
 // permuted and the top nb are solved in LDS, 32-row blocks: one thread per
 // column solves the diagonal block from registers, then all threads update
 // the rows below.
-template <int CB>
+// LBLK: L kept as its lower 32 x 32 blocks only (under 70 KB of LDS with CB
+// = 16, so the launch fits on a CU beside a trailing-update workgroup).
+template <int CB, bool LBLK>
 __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict__ L, int64_t ldl,
                                                          int nb, float* __restrict__ X,
                                                          int64_t ldx, int ncols,
@@ -158,9 +160,17 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
                                                          const int* __restrict__ src) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int XP = CB + 1;
-  const int nbp = (nb + 31) & ~31, LP = nbp + 1;
-  float* Lf = reinterpret_cast<float*>(smem_raw);  // nbp x LP
-  float* Xs = Lf + nbp * LP;                       // nbp x XP
+  const int nbp = (nb + 31) & ~31, LP = nbp + 1, nbb = nbp / 32;
+  // full: nbp x LP; blocks: (bi, bj), bi >= bj, 32 x 33 each at bi(bi+1)/2 + bj
+  float* Lf = reinterpret_cast<float*>(smem_raw);
+  float* Xs = Lf + (LBLK ? nbb * (nbb + 1) / 2 * 32 * 33 : nbp * LP);  // nbp x XP
+  auto lat = [&](int r, int q) -> float& {
+    if (LBLK) {
+      const int bi = r >> 5, bj = q >> 5;
+      return Lf[((bi * (bi + 1) / 2 + bj) * 32 + (r & 31)) * 33 + (q & 31)];
+    }
+    return Lf[r * LP + q];
+  };
   float* buf = Xs + nbp * XP;                      // touched rows x CB
   int* s_pos = reinterpret_cast<int*>(buf + 2 * nb * CB);
   int* s_src = s_pos + 2 * nb;
@@ -191,7 +201,7 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
 #pragma unroll
   for (int j = 0; j < NL; ++j) {
     const int e = tid + 256 * j, r = e / nbp, q = e - r * nbp;
-    if (e < nbp * nbp) Lf[r * LP + q] = lv[j];
+    if (e < nbp * nbp && (!LBLK || (q >> 5) <= (r >> 5))) lat(r, q) = lv[j];
   }
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
       for (int i = 1; i < 32; ++i) {
         float acc = x[i];
 #pragma unroll
-        for (int q = 0; q < i; ++q) acc -= Lf[(b + i) * LP + b + q] * x[q];
+        for (int q = 0; q < i; ++q) acc -= lat(b + i, b + q) * x[q];
         x[i] = acc;
       }
 #pragma unroll
@@ -228,7 +238,7 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
       const int i = b + 32 + e / CB, c = e % CB;
       float acc = 0.f;
 #pragma unroll
-      for (int q = 0; q < 32; ++q) acc += Lf[i * LP + b + q] * Xs[(b + q) * XP + c];
+      for (int q = 0; q < 32; ++q) acc += lat(i, b + q) * Xs[(b + q) * XP + c];
       Xs[i * XP + c] -= acc;
     }
     __syncthreads();
@@ -307,25 +317,33 @@ int trsm_llu_rowmajor(hipStream_t s, const T* L, int64_t ldl, int nb, T* X, int6
 // panel's first row), then X[0..nb) <- L^{-1} X[0..nb): the interchanges and
 // the U rows of those columns in one launch (FP32, nb <= 128).
 int laswp_trsm_rowmajor(hipStream_t s, const float* L, int64_t ldl, int nb, float* X, int64_t ldx,
-                        int ncols, const int* perm_n, const int* pos, const int* src) {
+                        int ncols, const int* perm_n, const int* pos, const int* src,
+                        bool compact) {
   if (ncols <= 0 || nb <= 0) return kOk;
   if (nb > 128) {
     set_error("laswp_trsm: panel width %d > 128", nb);
     return kErrUnsupported;
   }
-  constexpr int CB = 32;
-  const size_t nbp = (size_t)((nb + 31) & ~31);
-  const size_t lds = sizeof(float) * (nbp * (nbp + 1) + nbp * (CB + 1) + 2 * (size_t)nb * CB) +
-                     sizeof(int) * 4 * (size_t)nb;
-  auto kern = laswp_trsm_kernel<CB>;
+  const size_t nbp = (size_t)((nb + 31) & ~31), nbb = nbp / 32;
+  auto lds_of = [&](int cb, bool lblk) {
+    return sizeof(float) * ((lblk ? nbb * (nbb + 1) / 2 * 32 * 33 : nbp * (nbp + 1)) +
+                            nbp * (cb + 1) + 2 * (size_t)nb * cb) +
+           sizeof(int) * 4 * (size_t)nb;
+  };
   static bool attr_set = false;
   if (!attr_set) {
-    EAMD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 150 * 1024));
+    EAMD_HIP(hipFuncSetAttribute((const void*)laswp_trsm_kernel<32, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    EAMD_HIP(hipFuncSetAttribute((const void*)laswp_trsm_kernel<16, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(ceil_div(ncols, CB)), dim3(256), lds, s, L, ldl, nb, X, ldx, ncols,
-                     perm_n, pos, src);
+  if (compact)
+    hipLaunchKernelGGL((laswp_trsm_kernel<16, true>), dim3(ceil_div(ncols, 16)), dim3(256),
+                       lds_of(16, true), s, L, ldl, nb, X, ldx, ncols, perm_n, pos, src);
+  else
+    hipLaunchKernelGGL((laswp_trsm_kernel<32, false>), dim3(ceil_div(ncols, 32)), dim3(256),
+                       lds_of(32, false), s, L, ldl, nb, X, ldx, ncols, perm_n, pos, src);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
@@ -604,7 +622,7 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
       // then their update in FP32 (no packing on this path)
       PhaseScope ps(h, kPhTrsm, 4.0 * (4.0 * bs * n1 + 0.5 * bs * bs), (double)bs * bs * n1);
       EAMD_CHECK(laswp_trsm_rowmajor(sa, Pk, ldw, bs, A12n, ldw, n1, pn[par], pp[par],
-                                            psl[par]));
+                                            psl[par], true));
     }
     EAMD_CHECK(f32_update_rowmajor(h, A12n, A21, W + (int64_t)(k + bs) * ldw + (k + bs), ldw,
                                    m - bs, n1, bs, kPhNext));
@@ -625,7 +643,7 @@ int getrf_lookahead_body(eamd_handle* h, hipStream_t sa, hipStream_t sb, float* 
         {
           PhaseScope ps(h, kPhTrsm, 4.0 * (4.0 * bs * n2 + 0.5 * bs * bs), (double)bs * bs * n2);
           EAMD_CHECK(laswp_trsm_rowmajor(sb, Pk, ldw, bs, A12r, ldw, n2, pn[par], pp[par],
-                                                psl[par]));
+                                                psl[par], false));
         }
         MixedOperand L, Ur;
         EAMD_CHECK(mixed_pack(h, true, A21, ldw, m - bs, bs, precision, packL[par], &L));
