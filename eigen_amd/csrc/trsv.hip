@@ -46,7 +46,7 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
                                                          int64_t ldx, int c0, int nrhs,
                                                          u64* __restrict__ xw, unsigned epoch,
                                                          int* __restrict__ err) {
-  static_assert((BS == 64 || BS == 128) && (NRC == 4 || NRC == 1),
+  static_assert((BS == 64 || BS == 128) && (NRC == 16 || NRC == 4 || NRC == 1),
                 "four threads per row of a 64- or 128-row block");
   constexpr int NT = 4 * BS;   // threads
   constexpr int SEG = BS / 4;  // columns of a predecessor block per thread
@@ -147,31 +147,49 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
     if (d + 1 < ndep) load_seg(d + 1, tnext);
     const int jb = LOWER ? d : nblk - 1 - d;
     const int par = d & 1;
-    // my words of the predecessor's solution: row jr, column c, half h
+    // my words of the predecessor's solution (row jr, column c, half h): all
+    // loads in flight together, then only the words not yet published are
+    // read again
+    const u64* wsrc[WPT];
+    u64 wv[WPT];
+    bool need[WPT];
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {
       const int widx = tid + k * NT;
-      if (widx >= CHUNK) break;
       const int row = jb * BS + widx / (NRC * NW);
       const int c = (widx / NW) % NRC;
-      unsigned payload = 0;
-      if (row < n && c < nc) {
-        const u64* src = xw + (size_t)row * NRC * NW + (widx % (NRC * NW));
-        int spins = 0;
-        for (;;) {
-          const u64 wv = __hip_atomic_load(const_cast<u64*>(src), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-          if ((unsigned)(wv >> 32) == epoch) { payload = (unsigned)wv; break; }
-          if ((++spins & 255) == 0 &&
-              (spins > kSpin ||
-               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            bad = 1;
-            break;
+      need[k] = widx < CHUNK && row < n && c < nc;
+      wsrc[k] = xw + (size_t)row * NRC * NW + (widx % (NRC * NW));
+      wv[k] = need[k] ? __hip_atomic_load(const_cast<u64*>(wsrc[k]), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                      : 0ull;
+      if (widx < CHUNK && !need[k]) xsw[par][widx] = 0u;
+    }
+    for (int spins = 0;;) {
+      bool all = true;
+#pragma unroll
+      for (int k = 0; k < WPT; ++k)
+        if (need[k]) {
+          if ((unsigned)(wv[k] >> 32) == epoch) {
+            xsw[par][tid + k * NT] = (unsigned)wv[k];
+            need[k] = false;
+          } else {
+            all = false;
           }
-          __builtin_amdgcn_s_sleep(1);
         }
+      if (all) break;
+      if ((++spins & 255) == 0 &&
+          (spins > kSpin ||
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        bad = 1;
+        break;
       }
-      xsw[par][widx] = payload;
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int k = 0; k < WPT; ++k)
+        if (need[k])
+          wv[k] = __hip_atomic_load(const_cast<u64*>(wsrc[k]), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
     }
     if (bad) s_abort = 1;
     __syncthreads();
@@ -257,10 +275,10 @@ template <typename T>
 int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, bool lower,
                 bool unit, T* X, int64_t ldx, int nrhs) {
   if (n <= 0 || nrhs <= 0) return kOk;
-  constexpr int BS = 64, NRC = 4;
+  constexpr int BS = 64, NRC = 4, NRC16 = 16;
   const int nblk = ceil_div(n, BS);
-  // error word | exchange words (NRC values per row, 32 payload bits each)
-  const size_t nwords = (size_t)nblk * BS * NRC * (sizeof(T) / 4);
+  // error word | exchange words (up to 16 values per row, 32 payload bits each)
+  const size_t nwords = (size_t)nblk * BS * NRC16 * (sizeof(T) / 4);
   size_t need = 256 + sizeof(unsigned long long) * nwords;
   void* base = nullptr;
   bool fresh = h->ws.bytes[kWsStatus] < need;
@@ -293,11 +311,20 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     EAMD_HIP(hipGetLastError());
     return kOk;
   }
-  for (int c0 = 0; c0 < nrhs; c0 += NRC) {
+  // more than four right-hand sides: sixteen per chain (each chain step's
+  // latency is paid once for all of them), four for the remainder
+  for (int c0 = 0; c0 < nrhs;) {
     const unsigned epoch = (unsigned)++h->trsv_epoch;
+    const bool wide = nrhs - c0 > NRC;
 #define EAMD_TRSV(RM, LO, UN)                                                                   \
-  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC>), dim3(nblk), dim3(256), 0,     \
-                     h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err)
+  do {                                                                                          \
+    if (wide)                                                                                   \
+      hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC16>), dim3(nblk), dim3(256),  \
+                         0, h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err);            \
+    else                                                                                        \
+      hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC>), dim3(nblk), dim3(256), 0, \
+                         h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err);               \
+  } while (0)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV(true, true, true); else EAMD_TRSV(true, true, false); }
       else { if (unit) EAMD_TRSV(true, false, true); else EAMD_TRSV(true, false, false); }
@@ -307,6 +334,7 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     }
 #undef EAMD_TRSV
     EAMD_HIP(hipGetLastError());
+    c0 += wide ? NRC16 : NRC;
   }
   return kOk;
 }

@@ -202,6 +202,26 @@ def test_llt_against_oracle(gpu, n):
     assert po.backward_error(a, x, b) <= 1e-14
 
 
+@pytest.mark.parametrize("nrhs", [5, 16, 21])
+def test_solves_many_rhs(gpu, nrhs):
+    """More than four right-hand sides take the sixteen-column triangular
+    chain (21 = 16 + 5 also takes the four-column one): LU (both
+    transpositions) and LLT solves meet the FP64 bound column by column."""
+    n = 700
+    a = inputs.normal(5000 + nrhs, n, n)
+    b = inputs.normal(5100 + nrhs, n, nrhs)
+    lu = ea.PartialPivLU(dev(a))
+    x = host(lu.solve(dev(b)))
+    assert po.backward_error(a, x, b) <= 1e-14
+    xt = host(lu.solve(dev(b), trans="T"))
+    assert po.backward_error(np.asfortranarray(a.T), xt, b) <= 1e-14
+    s = inputs.spd(5200 + nrhs, n)
+    llt = ea.LLT(dev(s))
+    y = host(llt.solve(dev(b)))
+    assert po.backward_error(s, y, b) <= 1e-14
+    np.testing.assert_allclose(y, np.linalg.solve(s, b), rtol=1e-9, atol=1e-12)
+
+
 def test_llt_upper(gpu):
     n = 200
     a = inputs.spd(9, n)
