@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int6
   const int bsp = (bs + 31) & ~31;
   const int ld = bsp + 1;
   __shared__ int s_fail;
+  __shared__ double s_dinv[32];  // 1 / L_cc of the current diagonal sub-block
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (*fail >= 0) return;
   // padding: identity on the diagonal, zero elsewhere (factors to itself)
@@ -74,6 +75,9 @@ __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int6
 #pragma unroll
         for (int c = 0; c < 32; ++c)
           if (c <= lane) S[i + (kb + c) * ld] = x[c];
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+          if (c == lane) s_dinv[c] = 1.0 / x[c];
       }
       if (lane == 0 && f >= 0) s_fail = kb + f;
     }
@@ -92,23 +96,46 @@ __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int6
         double v = y[c];
 #pragma unroll
         for (int q = 0; q < c; ++q) v -= y[q] * S[kb + c + (kb + q) * ld];
-        y[c] = v / S[kb + c + (kb + c) * ld];
+        y[c] = v * s_dinv[c];
       }
 #pragma unroll
       for (int c = 0; c < 32; ++c) S[i + (kb + c) * ld] = y[c];
     }
     __syncthreads();
-    // ---- trailing lower update: S[i][j] -= sum_q S[i][kb+q] S[j][kb+q]
-    const int t0 = kb + 32;
-    for (int e = tid; e < below * below; e += 256) {
-      const int jj = e / below, ii = e - jj * below;
-      if (ii >= jj) {
-        const int i = t0 + ii, j = t0 + jj;
-        double acc = 0.0;
-#pragma unroll 8
-        for (int q = 0; q < 32; ++q) acc += S[i + (kb + q) * ld] * S[j + (kb + q) * ld];
-        S[i + j * ld] -= acc;
+    // ---- trailing lower update: S[i][j] -= sum_q S[i][kb+q] S[j][kb+q], in
+    // 4 x 4 blocks (bi >= bj) so each loaded value serves four products
+    const int t0 = kb + 32, nb4 = below / 4;
+    for (int e = tid; e < nb4 * (nb4 + 1) / 2; e += 256) {
+      // e -> (bi, bj) with bi >= bj: bi(bi+1)/2 <= e < (bi+1)(bi+2)/2
+      int bi = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      if ((bi + 1) * (bi + 2) / 2 <= e) ++bi;
+      if (bi * (bi + 1) / 2 > e) --bi;
+      const int bj = e - bi * (bi + 1) / 2;
+      const int i0 = t0 + 4 * bi, j0 = t0 + 4 * bj;
+      double acc[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+#pragma unroll 4
+      for (int q = 0; q < 32; ++q) {
+        const double* col = S + (kb + q) * ld;
+        double a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          a[r] = col[i0 + r];
+          b[r] = col[j0 + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (bi > bj || r >= c) S[i0 + r + (j0 + c) * ld] -= acc[r][c];
     }
     __syncthreads();
   }
@@ -208,7 +235,22 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attr = true;
   }
-  for (int k = 0; k < n; k += nb) {
+  // One-panel look-ahead on two streams (no pivoting, so nothing waits on
+  // a grid-wide exchange): the handle's stream factors panel k (POTF2 of the
+  // diagonal block, then A21 L11^-T), updates the next panel's columns only
+  // (rows below the panel, lower part of their diagonal block) and goes on
+  // to panel k+1; the second stream, started by an event after panel k,
+  // does the SYRK of the rest of the trailing matrix.  The handle's stream
+  // waits for it only before updating the next panel's columns again.
+  static const bool lookahead = env_int("EAMD_LOOKAHEAD", 1) != 0;
+  hipStream_t sb = nullptr;
+  if (lookahead && n > nb) {
+    EAMD_CHECK(ensure_side_stream(h));
+    sb = h->side;
+  }
+  hipEvent_t evP = h->la_ev[0], evB[2] = {h->la_ev[1], h->la_ev[2]};
+  bool side_used = false;
+  for (int k = 0, it = 0; k < n; k += nb, ++it) {
     const int bs = min(nb, n - k);
     const int rs = n - k - bs;
     double* A11 = A + k + (int64_t)k * lda;
@@ -221,9 +263,33 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
       hipLaunchKernelGGL(trsm_rlt_kernel, dim3(ceil_div(rs, 32)), dim3(256), lds2, s, A11, lda,
                          bs, A21, lda, rs, fail);
       double* A22 = A21 + (int64_t)bs * lda;
-      EAMD_CHECK(dgemm_device(s, false, true, rs, rs, bs, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
-                              1, fail));
+      if (!sb) {
+        EAMD_CHECK(dgemm_device(s, false, true, rs, rs, bs, -1.0, A21, lda, A21, lda, 1.0, A22,
+                                lda, 1, fail));
+        continue;
+      }
+      const int par = it & 1, n1 = min(nb, rs), n2 = rs - n1;
+      EAMD_HIP(hipEventRecord(evP, s));  // the panel is factored
+      // the next panel's columns were last written by the previous panel's
+      // second-stream update
+      if (it > 0) EAMD_HIP(hipStreamWaitEvent(s, evB[par ^ 1], 0));
+      EAMD_CHECK(dgemm_device(s, false, true, rs, n1, bs, -1.0, A21, lda, A21, lda, 1.0, A22,
+                              lda, 1, fail));
+      if (n2 > 0) {
+        EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));
+        side_used = true;
+        EAMD_CHECK(dgemm_device(sb, false, true, n2, n2, bs, -1.0, A21 + n1, lda, A21 + n1, lda,
+                                1.0, A22 + n1 + (int64_t)n1 * lda, lda, 1, fail));
+        EAMD_HIP(hipEventRecord(evB[par], sb));
+      } else {
+        // nothing for the second stream: the next panel has no later wait
+        EAMD_HIP(hipEventRecord(evB[par], s));
+      }
     }
+  }
+  if (side_used) {
+    EAMD_HIP(hipEventRecord(h->la_ev[3], sb));
+    EAMD_HIP(hipStreamWaitEvent(s, h->la_ev[3], 0));
   }
   EAMD_HIP(hipGetLastError());
   int hf = -1;
