@@ -98,10 +98,12 @@ __global__ __launch_bounds__(256) void residual_finish_kernel(int n, int nchunks
 
 // row abs sums of A (for ||A||_inf), chunked like the GEMV
 __global__ __launch_bounds__(RB) void rowabs_partial_kernel(int n, const double* __restrict__ A,
-                                                            int64_t lda, double* __restrict__ part) {
+                                                            int64_t lda, double* __restrict__ part,
+                                                            double* __restrict__ out) {
   const int i = blockIdx.x * RB + threadIdx.x;
   const int chunk = blockIdx.y;
   const int j0 = chunk * CCH, j1 = min(n, j0 + CCH);
+  if (i == 0 && chunk == 0) *out = 0.0;  // rowabs_finish_kernel's maximum starts here
   if (i >= n) return;
   double s = 0.0;
   const double* a = A + (int64_t)j0 * lda + i;
@@ -109,24 +111,30 @@ __global__ __launch_bounds__(RB) void rowabs_partial_kernel(int n, const double*
   part[(size_t)chunk * n + i] = s;
 }
 
-// max over i of sum over chunks -> out[0]
+// max over i of sum over chunks -> out[0]: one thread per row, the
+// workgroup's maximum merged by an atomic maximum on the bit pattern (for
+// non-negative doubles, NaN included, the unsigned order of the bits is the
+// numeric order); out[0] was cleared by rowabs_partial_kernel
 __global__ __launch_bounds__(256) void rowabs_finish_kernel(int n, int nchunks,
                                                             const double* __restrict__ part,
                                                             double* __restrict__ out) {
   __shared__ double red[256];
-  double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    double s = 0.0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double s = 0.0;
+  if (i < n)
     for (int k = 0; k < nchunks; ++k) s += part[(size_t)k * n + i];
-    m = fmax(m, s);
-  }
-  red[threadIdx.x] = m;
+  red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    if (threadIdx.x < w) {
+      const double a = red[threadIdx.x], b = red[threadIdx.x + w];
+      red[threadIdx.x] = (b > a || b != b) ? b : a;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  if (threadIdx.x == 0)
+    atomicMax(reinterpret_cast<unsigned long long*>(out),
+              (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // per rhs: max|r|, max|x|, max|b| -> out[3*c + {0,1,2}]; NaN propagates
@@ -187,9 +195,10 @@ int matrix_inf_norm(eamd_handle* h, int n, const double* A, int64_t lda, double*
   EAMD_CHECK(h->ws.get(kWsAux2, sizeof(double) * (size_t)nchunks * n + 256, &p));
   double* part = static_cast<double*>(p);
   dim3 g1(ceil_div(n, RB), nchunks);
-  hipLaunchKernelGGL(rowabs_partial_kernel, g1, dim3(RB), 0, h->stream, n, A, lda, part);
-  hipLaunchKernelGGL(rowabs_finish_kernel, dim3(1), dim3(256), 0, h->stream, n, nchunks, part,
+  hipLaunchKernelGGL(rowabs_partial_kernel, g1, dim3(RB), 0, h->stream, n, A, lda, part,
                      out_dev);
+  hipLaunchKernelGGL(rowabs_finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, n,
+                     nchunks, part, out_dev);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }
