@@ -72,8 +72,34 @@ __global__ __launch_bounds__(256) void llt_batch_kernel(int count, const int* __
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int c = s * T + t;
+      if (N >= 4) {
+        // the whole column as 16-byte words (only 4-byte alignment is
+        // needed), the last word overlapping so every read stays inside the
+        // column: a quarter of the address work of word loads
+        typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+        const float* pc = a + (c < N ? c : 0) * N;
+        float cv[N];
 #pragma unroll
-      for (int i = s * T; i < N; ++i) col[s][i] = (c < N && i >= c) ? a[i + c * N] : 0.0f;
+        for (int q = 0; q + 4 <= N; q += 4) {
+          const f4u w = *reinterpret_cast<const f4u*>(pc + q);
+          cv[q] = w.x;
+          cv[q + 1] = w.y;
+          cv[q + 2] = w.z;
+          cv[q + 3] = w.w;
+        }
+        if (N % 4) {
+          const f4u w = *reinterpret_cast<const f4u*>(pc + N - 4);
+          cv[N - 4] = w.x;
+          cv[N - 3] = w.y;
+          cv[N - 2] = w.z;
+          cv[N - 1] = w.w;
+        }
+#pragma unroll
+        for (int i = s * T; i < N; ++i) col[s][i] = (c < N && i >= c) ? cv[i] : 0.0f;
+      } else {
+#pragma unroll
+        for (int i = s * T; i < N; ++i) col[s][i] = (c < N && i >= c) ? a[i + c * N] : 0.0f;
+      }
     }
     // ---- factorization (right-looking; same quantities as the reference's
     // left-looking unblocked loop)
