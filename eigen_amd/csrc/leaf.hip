@@ -581,17 +581,19 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
 // by the backward error after refinement rather than by Eigen's pivot
 // sequence.  A leaf of m rows costs a few short launches instead of one
 // exchange round trip per column:
-//   1. one wave per 256 rows runs partial pivoting on them, in registers,
-//      and records the rows it picked (up to 32 candidates);
-//   2. while more than 8 candidate groups remain, one wave per 8 groups runs
+//   1. one wave per 256 rows (two waves per 512 rows when more than 32
+//      groups would follow) runs partial pivoting on them, in registers, and
+//      records the rows it picked (up to 32 candidates);
+//   2. while more than 32 candidate groups remain, one wave per 8 groups runs
 //      partial pivoting on the original values of their candidates and keeps
 //      32 of them;
-//   3. one wave does the same over the last (at most 8) groups: the rows it
-//      picks, in order, are the leaf's pivots and its first 32 rows after
-//      elimination are the factored top block L11\U11, written in place; it
-//      also saves the original rows 0..w-1, derives the transpositions,
-//      replays them into the permutation lists and lists where the displaced
-//      rows go;
+//   3. one workgroup does the same over the last groups (one wave for at
+//      most 8, four waves for at most 32, each column's pivot the largest of
+//      the waves' maxima): the rows it picks, in order, are the leaf's pivots
+//      and its first 32 rows after elimination are the factored top block
+//      L11\U11, written in place; its first wave also saves the original rows
+//      0..w-1, derives the transpositions, replays them into the permutation
+//      lists and lists where the displaced rows go;
 //   4. every other row is solved against U11 (l = a U11^-1) in parallel.
 namespace {
 
@@ -1166,7 +1168,8 @@ __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__
 
 // Same outputs as leaf_lu; FP32 only.  Panels of up to 256 rows take the
 // partial-pivoting leaf.  Candidate groups are merged 8 at a time until at
-// most 8 remain for the final round.
+// most 32 remain for the final round (at most 8 for a one-wave final round
+// when the wide rounds are switched off).
 int leaf_lu_tournament(eamd_handle* h, float* P, int64_t ldp, int m, int w, int* piv,
                        int* lperm_n, int* lpos, int* lsrc, int* pperm_n, int* ppos, int* psrc,
                        int row0, int pnb, int* zero_col, int piv_offset, int* fz_global,
