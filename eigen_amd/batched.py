@@ -83,6 +83,20 @@ def spotrf_solve_batched(offA, n, A, offb, b):
     return info
 
 
+def llt_rank_update_batched(offL, n, L, offv, v, sigma=None, upper=False):
+    """LLT::rankUpdate (LLT.h:459-472) for packed factors: L (FP64, factor i
+    column-major n[i] x n[i] at offL[i]) updated in place so that
+    L_i L_i^T = A_i + sigma_i v_i v_i^T; returns info (-1, the failing column
+    of a downdate, or -2 for n outside 0..256)."""
+    info = torch.empty(n.numel(), dtype=torch.int32, device=n.device)
+    check("eamd_dllt_rank_update_batched",
+          lib().eamd_dllt_rank_update_batched(handle(), n.numel(), _p(offL), _p(n), _p(L),
+                                              _p(offv), _p(v),
+                                              _p(sigma) if sigma is not None else None,
+                                              1 if upper else 0, _p(info)))
+    return info
+
+
 def dgesv_ir_batched(offA, n, A, offB, nrhs, B):
     """Batched FP32-LU + FP64-refinement solve; returns (X, info, iters)."""
     X = torch.empty_like(B)
@@ -250,6 +264,34 @@ def bench_lu_fields(count=65536, nrhs=4, steps=3):
     }
 
 
+def bench_rank_update_fields(n, offA, A, offv, steps=5):
+    """Rank-one updates (LLT::rankUpdate, sigma = 1) of the batched workload's
+    factors in FP64: the packed SPD matrices' lower triangles stand in for
+    factors (a positive diagonal is all an update with sigma > 0 needs)."""
+    import time
+    count = n.numel()
+    L = A.to(torch.float64)
+    V = torch.randn(int(n.sum().item()), dtype=torch.float64, device=A.device,
+                    generator=torch.Generator(device=A.device).manual_seed(5))
+    offv = offv.to(torch.int64)
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        info = llt_rank_update_batched(offA, n, L, offv, V)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+    dt = min(times)
+    nn = n.to(torch.int64)
+    bytes_moved = float((8 * (nn * (nn + 1)) + 8 * nn).sum().item())  # L read + write, v
+    return {"workload": f"{count} FP64 factors of the batched LLT sizes, one rank-one "
+                        "update each (sigma = 1), packed + offsets",
+            "updates_per_s": round(count / dt, 1), "ms": round(dt * 1e3, 3),
+            "hbm_gbs": round(bytes_moved / dt / 1e9, 1),
+            "hbm_frac": round(bytes_moved / dt / 8e12, 4),
+            "all_info_ok": bool((info == -1).all().item())}
+
+
 def bench_fields(count=1_000_000, steps=5):
     """Extra fields of the benchmark line for the batched workloads."""
     import time
@@ -312,7 +354,12 @@ def bench_fields(count=1_000_000, steps=5):
         out["batched_llt"]["speedup_vs_per_matrix_loop"] = round(count / dt / loop, 1)
     except Exception as e:  # reported, never hides the batched result
         out["batched_llt"]["per_matrix_loop_error"] = str(e)[:200]
-    del A, b, b0, x
+    del b, b0, x
+    try:
+        out["batched_llt_rank_update"] = bench_rank_update_fields(n, offA, A, offb, steps)
+    except Exception as e:
+        out["batched_llt_rank_update"] = {"error": str(e)[:300]}
+    del A
     try:
         out["batched_lu"] = bench_lu_fields()
     except Exception as e:

@@ -591,6 +591,22 @@ int eamd_dbackward_error(eamd_handle_t h, int64_t n, int64_t nrhs, const double*
 }
 
 // -------------------------------------------------------------- batched
+int eamd_dllt_rank_update_batched(eamd_handle_t h, int64_t count, const int64_t* offL,
+                                  const int32_t* n, double* L, const int64_t* offv,
+                                  const double* v, const double* sigma, int32_t upper,
+                                  int32_t* info) {
+  if (count < 0) return -2;
+  CHECK_HANDLE(h);
+  if (count == 0) return kOk;
+  if (!offL) return -3;
+  if (!n) return -4;
+  if (!L) return -5;
+  if (!offv) return -6;
+  if (!v) return -7;
+  if (!info) return -10;
+  return llt_rank_update_batched(h, count, offL, n, L, offv, v, sigma, upper ? 1 : 0, info);
+}
+
 int eamd_spotrf_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
                               const int32_t* n, const float* A, const int64_t* offb, float* b,
                               int32_t* info) {

@@ -28,6 +28,11 @@ namespace eamd {
 // integer environment setting, dflt when unset or empty
 int env_int(const char* name, int dflt);
 
+// LLT rank-one update of packed factors (llt.hip)
+int llt_rank_update_batched(eamd_handle* h, int64_t count, const int64_t* offL, const int32_t* n,
+                            double* L, const int64_t* offv, const double* v, const double* sigma,
+                            int upper, int32_t* info);
+
 // ---- level 3 (gemm_f64.hip) ----
 // C = alpha*op(A)*op(B) + beta*C, column-major.  tri: 0 full C, 1 only
 // i >= j (lower SYRK), 2 only i <= j (upper).  abort_flag (nullable, device):

@@ -219,6 +219,133 @@ __global__ void set_int_kernel(int* p, int v) {
 
 }  // namespace
 
+namespace {
+// LLT rank-one update, L L^T + sigma v v^T = L' L'^T, in place: one wave per
+// system, row i of the factor on lane i % 64 (up to 4 rows per lane, n <=
+// 256).  Restates internal::llt_rank_update_lower (LLT.h:231-299): for
+// sigma > 0 a Givens rotation per column (JacobiRotation::makeGivens for
+// reals, Jacobi.h:226-262, applied as x' = c x + s y, y' = -s x + c y,
+// Jacobi.h:315-330); otherwise the scalar downdate, stopping at the first
+// column whose new diagonal would not be positive.  Column j's rows are read
+// and written by their own lanes in step j only, and the diagonal element
+// (j, j) is read by all lanes before lane j % 64 writes it, so the steps
+// need no other synchronisation.  info: -1, the failing column, or -2 for a
+// size outside 0..256.
+constexpr int kRankUpdMaxN = 256;
+__global__ __launch_bounds__(256) void llt_rank_update_kernel(
+    int64_t count, const int64_t* __restrict__ offL, const int32_t* __restrict__ nv,
+    double* __restrict__ Lall, const int64_t* __restrict__ offv, const double* __restrict__ V,
+    const double* __restrict__ sigmas, int upper, int32_t* __restrict__ info) {
+  constexpr int K = kRankUpdMaxN / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (s >= count) return;  // whole waves
+  const int n = nv[s];
+  if (n < 0 || n > kRankUpdMaxN) {
+    if (lane == 0) info[s] = -2;
+    return;
+  }
+  double* L = Lall + offL[s];
+  const double* v = V + offv[s];
+  const double sigma = sigmas ? sigmas[s] : 1.0;
+  // element (i, j) of the lower factor (the upper one stores its transpose)
+  auto at = [&](int i, int j) -> double& {
+    return upper ? L[j + (int64_t)i * n] : L[i + (int64_t)j * n];
+  };
+  const double ss = sigma > 0.0 ? sqrt(sigma) : 1.0;
+  double t[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = lane + 64 * k;
+    t[k] = i < n ? (sigma > 0.0 ? ss * v[i] : v[i]) : 0.0;
+  }
+  int ret = -1;
+  double beta = 1.0;
+  for (int j = 0; j < n; ++j) {
+    const int jk = j >> 6, jl = j & 63;
+    const double tsel = jk == 0 ? t[0] : (jk == 1 ? t[1] : (jk == 2 ? t[2] : t[3]));
+    const long long tb = __double_as_longlong(tsel);
+    const double tj = __longlong_as_double(
+        ((long long)__builtin_amdgcn_readlane((int)(tb >> 32), jl) << 32) |
+        (unsigned)__builtin_amdgcn_readlane((int)tb, jl));
+    const double p = at(j, j);
+    if (sigma > 0.0) {
+      const double q = -tj;
+      double c, sn, r;
+      if (q == 0.0) {
+        c = p < 0.0 ? -1.0 : 1.0;
+        sn = 0.0;
+        r = fabs(p);
+      } else if (p == 0.0) {
+        c = 0.0;
+        sn = q < 0.0 ? 1.0 : -1.0;
+        r = fabs(q);
+      } else if (fabs(p) > fabs(q)) {
+        const double tt = q / p;
+        double u = sqrt(1.0 + tt * tt);
+        if (p < 0.0) u = -u;
+        c = 1.0 / u;
+        sn = -tt * c;
+        r = p * u;
+      } else {
+        const double tt = p / q;
+        double u = sqrt(1.0 + tt * tt);
+        if (q < 0.0) u = -u;
+        sn = -1.0 / u;
+        c = -tt * sn;
+        r = q * u;
+      }
+      if (lane == jl) at(j, j) = r;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int i = lane + 64 * k;
+        if (i > j && i < n) {
+          const double x = at(i, j), y = t[k];
+          at(i, j) = c * x + sn * y;
+          t[k] = -sn * x + c * y;
+        }
+      }
+    } else {
+      const double Ljj = p, dj = Ljj * Ljj, wj = tj;
+      const double swj2 = sigma * wj * wj, gamma = dj * beta + swj2;
+      const double x = dj + swj2 / beta;
+      if (x <= 0.0) {
+        ret = j;
+        break;
+      }
+      const double nLjj = sqrt(x);
+      if (lane == jl) at(j, j) = nLjj;
+      beta += swj2 / dj;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int i = lane + 64 * k;
+        if (i > j && i < n) {
+          const double lij = at(i, j);
+          t[k] -= (wj / Ljj) * lij;
+          if (gamma != 0.0) at(i, j) = (nLjj / Ljj) * lij + (nLjj * sigma * wj / gamma) * t[k];
+        }
+      }
+    }
+  }
+  if (lane == 0) info[s] = ret;
+}
+}  // namespace
+
+int llt_rank_update_batched(eamd_handle* h, int64_t count, const int64_t* offL, const int32_t* n,
+                            double* L, const int64_t* offv, const double* v, const double* sigma,
+                            int upper, int32_t* info) {
+  if (count <= 0) return kOk;
+  const int64_t blocks = (count + 3) / 4;  // four waves (systems) per workgroup
+  if (blocks > INT32_MAX) {
+    set_error("rank update: %lld systems exceed one launch", (long long)count);
+    return kErrUnsupported;
+  }
+  hipLaunchKernelGGL(llt_rank_update_kernel, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                     count, offL, n, L, offv, v, sigma, upper, info);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
 int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_col, int nb) {
   hipStream_t s = h->stream;
   if (nb > kMaxNb) nb = kMaxNb;

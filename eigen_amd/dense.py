@@ -282,6 +282,27 @@ class LLT:
         self.solveInPlace(x)
         return x.reshape(-1) if vec else x
 
+    def rankUpdate(self, v, sigma=1.0):
+        """LLT::rankUpdate (LLT.h:459-472): the factor becomes that of
+        A + sigma v v^T, in place; info() is NumericalIssue when a downdate
+        (sigma <= 0) leaves A not positive definite.  The stored 1-norm is
+        left as it was, as in the reference."""
+        from . import batched
+        n = self._m.shape[0]
+        if n > 256:
+            raise ValueError("rankUpdate: sizes up to 256")
+        dev = self._m.device
+        vv = torch.as_tensor(v, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+        if vv.numel() != n:
+            raise ValueError("rankUpdate: vector of size %d for a %d x %d factor"
+                             % (vv.numel(), n, n))
+        z = torch.zeros(1, dtype=torch.int64, device=dev)
+        info = batched.llt_rank_update_batched(
+            z, torch.tensor([n], dtype=torch.int32, device=dev), self._m, z, vv,
+            torch.tensor([float(sigma)], dtype=torch.float64, device=dev), upper=self.uplo == "U")
+        self._info = Success if int(info[0]) == -1 else NumericalIssue
+        return self
+
     def rcond(self):
         """Estimate of 1 / (||A||_1 ||A^{-1}||_1) (LLT.h:161-166); the
         factorization must have succeeded."""

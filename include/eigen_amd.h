@@ -182,6 +182,20 @@ int eamd_dbackward_error(eamd_handle_t h, int64_t n, int64_t nrhs, const double*
                          const double* X, int64_t ldx, const double* B, int64_t ldb,
                          double* berr);
 
+/* ------------------------------------------------------ LLT rank update -- */
+/* Replaces LLT<MatrixXd,UpLo>::rankUpdate(v, sigma) (LLT.h:459-472, through
+ * internal::llt_rank_update_lower, LLT.h:231-299) for many factors in one
+ * launch: factor i is n[i] x n[i], column-major, at L + offL[i] (the lower
+ * triangle, or with upper != 0 the upper one holding L^T), v_i at v + offv[i],
+ * sigma_i = sigma[i] (sigma may be NULL: 1).  On return L_i L_i^T = A_i +
+ * sigma_i v_i v_i^T and info[i] = -1, or the column where a downdate (sigma
+ * <= 0) stopped (the reference's NumericalIssue), or -2 for n[i] outside
+ * 0..256.  All arrays are device arrays; asynchronous on the handle's stream. */
+int eamd_dllt_rank_update_batched(eamd_handle_t h, int64_t count, const int64_t* offL,
+                                  const int32_t* n, double* L, const int64_t* offv,
+                                  const double* v, const double* sigma, int32_t upper,
+                                  int32_t* info);
+
 /* -------------------------------------------------------------- batched -- */
 /* No reference counterpart as one launch; per system it is
  * LLT<MatrixXf>(A_i).solve(b_i) (LLT.h:422-505).  Systems are packed

@@ -441,6 +441,78 @@ double orc_backward_error_d(long n, long nrhs, const double* A, long lda,
 }
 
 /* ------------------------------------------------------------------------- */
+/* LLT rank-one update                                                       */
+/* ------------------------------------------------------------------------- */
+/* L L^T + sigma v v^T = L' L'^T in place on the lower factor (column-major,
+ * ld); returns -1, or the column where a downdate (sigma <= 0) stops.
+ * internal::llt_rank_update_lower (LLT.h:231-299): sigma > 0 by Givens
+ * rotations (JacobiRotation::makeGivens for reals, Jacobi.h:226-262, applied
+ * as x' = c x + s y, y' = -s x + c y, Jacobi.h:315-330); otherwise the
+ * scalar downdate recurrence. */
+long orc_llt_rank_update_d(long n, double* L, long ld, const double* v, double sigma) {
+  double* t = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  long ret = -1;
+  if (sigma > 0) {
+    const double ss = sqrt(sigma);
+    for (long i = 0; i < n; ++i) t[i] = ss * v[i];
+    for (long i = 0; i < n; ++i) {
+      const double p = L[IDX(i, i, ld)], q = -t[i];
+      double c, s, r;
+      if (q == 0.0) {
+        c = p < 0.0 ? -1.0 : 1.0;
+        s = 0.0;
+        r = fabs(p);
+      } else if (p == 0.0) {
+        c = 0.0;
+        s = q < 0.0 ? 1.0 : -1.0;
+        r = fabs(q);
+      } else if (fabs(p) > fabs(q)) {
+        const double tt = q / p;
+        double u = sqrt(1.0 + tt * tt);
+        if (p < 0.0) u = -u;
+        c = 1.0 / u;
+        s = -tt * c;
+        r = p * u;
+      } else {
+        const double tt = p / q;
+        double u = sqrt(1.0 + tt * tt);
+        if (q < 0.0) u = -u;
+        s = -1.0 / u;
+        c = -tt * s;
+        r = q * u;
+      }
+      L[IDX(i, i, ld)] = r;
+      for (long k = i + 1; k < n; ++k) {
+        const double x = L[IDX(k, i, ld)], y = t[k];
+        L[IDX(k, i, ld)] = c * x + s * y;
+        t[k] = -s * x + c * y;
+      }
+    }
+  } else {
+    for (long i = 0; i < n; ++i) t[i] = v[i];
+    double beta = 1.0;
+    for (long j = 0; j < n; ++j) {
+      const double Ljj = L[IDX(j, j, ld)], dj = Ljj * Ljj, wj = t[j];
+      const double swj2 = sigma * wj * wj, gamma = dj * beta + swj2;
+      const double x = dj + swj2 / beta;
+      if (x <= 0.0) {
+        ret = j;
+        break;
+      }
+      const double nLjj = sqrt(x);
+      L[IDX(j, j, ld)] = nLjj;
+      beta += swj2 / dj;
+      for (long k = j + 1; k < n; ++k) t[k] -= (wj / Ljj) * L[IDX(k, j, ld)];
+      if (gamma != 0.0)
+        for (long k = j + 1; k < n; ++k)
+          L[IDX(k, j, ld)] = (nLjj / Ljj) * L[IDX(k, j, ld)] + (nLjj * sigma * wj / gamma) * t[k];
+    }
+  }
+  free(t);
+  return ret;
+}
+
+/* ------------------------------------------------------------------------- */
 /* batched                                                                   */
 /* ------------------------------------------------------------------------- */
 void orc_batched_llt_solve_f(long count, const int64_t* offA, const int32_t* n,

@@ -61,6 +61,8 @@ long orc_llt_blocked_f(long n, float* a, long lda);
 /* LLT::_solve_impl / solveInPlace, LLT.h:475-505 */
 void orc_llt_solve_d(long n, long nrhs, const double* l, long ldl, double* b, long ldb);
 void orc_llt_solve_f(long n, long nrhs, const float* l, long ldl, float* b, long ldb);
+/* LLT rank-one update of a lower factor (LLT.h:231-299); -1 or failing column */
+long orc_llt_rank_update_d(long n, double* L, long ld, const double* v, double sigma);
 
 /* ---- level-3 kernels ---- */
 /* C = alpha*op(A)*op(B) + beta*C  (general_matrix_matrix_product,

@@ -61,6 +61,8 @@ def orc():
         lib.orc_backward_error_d.argtypes = [L, L, _dp, L, _dp, L, _dp, L]
         lib.orc_backward_error_d.restype = C.c_double
         lib.orc_batched_llt_solve_f.argtypes = [L, _i64p, _ip, _f1, _i64p, _f1, _ip]
+        lib.orc_llt_rank_update_d.argtypes = [L, _dp, L, _dp, C.c_double]
+        lib.orc_llt_rank_update_d.restype = L
         lib.orc_batched_lu_solve_d.argtypes = [L, _i64p, _ip, _d1, _i64p, L, _d1, _ip]
         _ORC = lib
     return _ORC
@@ -96,6 +98,8 @@ def ref():
         lib.eref_llt_rcond_d.restype = C.c_double
         lib.eref_llt_blocked_d.argtypes = [L, _dp, L]
         lib.eref_llt_blocked_d.restype = L
+        lib.eref_llt_rank_update_d.argtypes = [L, _dp, L, _dp, C.c_double]
+        lib.eref_llt_rank_update_d.restype = L
         lib.eref_gemm_d.argtypes = [L, L, L, _dp, L, _dp, L, _dp, L]
         lib.eref_trsm_d.argtypes = [C.c_char, C.c_char, C.c_char, C.c_char, L, L, _dp, L, _dp, L]
         lib.eref_syrk_lower_d.argtypes = [L, L, C.c_double, _dp, L, _dp, L]
@@ -142,6 +146,19 @@ def lu_solve(lu_, t, b):
     orc().orc_lu_solve_d(n, b.shape[1], np.asfortranarray(lu_), n,
                          np.ascontiguousarray(t, dtype=np.int32), b, n)
     return b
+
+
+def llt_rank_update(lfac, v, sigma, reference=False):
+    """LLT rank-one update of a lower factor (LLT.h:231-299) on a copy;
+    returns (factor, info) with info -1 or the failing column.  reference=True
+    runs Eigen itself (oracle/_ref)."""
+    f = np.asfortranarray(np.array(lfac, dtype=np.float64))
+    vv = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+    n = f.shape[0]
+    lib = ref() if reference else orc()
+    r = (lib.eref_llt_rank_update_d if reference else lib.orc_llt_rank_update_d)(
+        n, f, n, vv, float(sigma))
+    return f, int(r)
 
 
 def llt(a, dtype=np.float64):

@@ -112,6 +112,14 @@ double eref_llt_rcond_d(long n, const double* A, long lda) {
   return llt.info() == Success ? llt.rcond() : -1.0;
 }
 
+// internal::llt_inplace<double,Lower>::rankUpdate (LLT.h:231-299, 380-384):
+// the lower factor (column-major, ld) updated in place; -1 or failing column
+long eref_llt_rank_update_d(long n, double* L, long ld, const double* v, double sigma) {
+  MapD Lm(L, n, n, OuterStride<>(ld));
+  Map<const VectorXd> vv(v, n);
+  return (long)internal::llt_inplace<double, Lower>::rankUpdate(Lm, vv, sigma);
+}
+
 // internal::llt_inplace<double,Lower>::blocked (LLT.h:330): failing col or -1
 long eref_llt_blocked_d(long n, double* a, long lda) {
   MapD A(a, n, n, OuterStride<>(lda));

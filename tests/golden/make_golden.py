@@ -20,6 +20,10 @@ these fixtures are Eigen's outputs on seeded inputs (tests/inputs.py):
   rcond.npz PartialPivLU::rcond / inverse (PartialPivLU.h:184-202) and
             LLT::rcond (LLT.h:161-166) on the lu.npz / llt.npz inputs
             (`make_golden.py rcond` writes this file alone).
+  rankupdate.npz  LLT rank-one updates (internal::llt_rank_update_lower,
+            LLT.h:231-299) of Eigen's factor of A = spd(seed, n): the updated
+            lower factor and info for sigma = +2 and -0.5, and a downdate
+            that fails at column 3 (`make_golden.py rankupdate`).
 Inputs are not stored (they are regenerated from the seeds); their SHA-256
 is, so a diverging generator is detected.
 """
@@ -182,8 +186,46 @@ def rcond_main():
     print("rcond fixtures written to", HERE)
 
 
+RANKUPD_SIZES = [1, 6, 24, 33, 100, 256]
+RANKUPD_SIGMAS = [("p", 2.0), ("m", -0.5)]
+
+
+def rankupdate_main():
+    ref = po.ref()
+    out = {}
+    for n in RANKUPD_SIZES:
+        a = inputs.spd(7000 + n, n)
+        v = inputs.normal(7100 + n, n)
+        out[f"n{n}_sha"] = np.array(inputs.sha256(a) + inputs.sha256(v))
+        l0 = np.zeros((n, n), order="F")
+        ref.eref_llt_d(n, a, n, 0, a, n, l0, a, n)
+        for tag, sigma in RANKUPD_SIGMAS:
+            f, info = po.llt_rank_update(l0, v, sigma, reference=True)
+            lf = np.tril(f)
+            if n <= 100:
+                out[f"n{n}_{tag}_L"] = lf
+            else:  # diagonal, row and column sums
+                out[f"n{n}_{tag}_diag"] = np.diag(lf).copy()
+                out[f"n{n}_{tag}_rows"] = lf.sum(axis=1)
+                out[f"n{n}_{tag}_cols"] = lf.sum(axis=0)
+            out[f"n{n}_{tag}_info"] = np.array(info)
+    # a downdate that stops at column 3
+    n = 24
+    a = inputs.spd(7000 + n, n)
+    l0 = np.zeros((n, n), order="F")
+    ref.eref_llt_d(n, a, n, 0, a, n, l0, a, n)
+    v = np.zeros(n)
+    v[3] = 10.0 * np.sqrt(a[3, 3])
+    _, info = po.llt_rank_update(l0, v, -1.0, reference=True)
+    out["fail_info"] = np.array(info)
+    np.savez_compressed(os.path.join(HERE, "rankupdate.npz"), **out)
+    print("rank update fixtures written to", HERE)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "rcond":
         rcond_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "rankupdate":
+        rankupdate_main()
     else:
         main()

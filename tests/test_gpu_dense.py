@@ -294,3 +294,78 @@ def test_rcond_closed_forms_gpu(gpu):
     assert ea.dense.LLT(torch.diag(d.abs()).cuda()).rcond() == pytest.approx(1 / 16, rel=1e-13)
     z = torch.zeros(4, 4, dtype=torch.float64)
     assert ea.dense.PartialPivLU(z.cuda()).rcond() == 0.0
+
+
+# ------------------------------------------------------- LLT rank update --
+@pytest.mark.parametrize("n", [1, 6, 24, 33, 100, 256])
+@pytest.mark.parametrize("tag,sigma", [("p", 2.0), ("m", -0.5)])
+def test_llt_rank_update_against_reference_fixture(gpu, n, tag, sigma):
+    g = np.load(os.path.join(G, "rankupdate.npz"), allow_pickle=False)
+    a = inputs.spd(7000 + n, n)
+    v = inputs.normal(7100 + n, n)
+    llt = ea.LLT(dev(a))
+    llt.rankUpdate(dev(v), sigma)
+    assert llt.info() == ea.Success and int(g[f"n{n}_{tag}_info"]) == -1
+    lf = np.tril(host(llt.matrixLLT()))
+    if f"n{n}_{tag}_L" in g.files:
+        np.testing.assert_allclose(lf, g[f"n{n}_{tag}_L"], rtol=1e-11, atol=1e-11)
+    else:
+        np.testing.assert_allclose(np.diag(lf), g[f"n{n}_{tag}_diag"], rtol=1e-11)
+        np.testing.assert_allclose(lf.sum(axis=1), g[f"n{n}_{tag}_rows"], rtol=1e-9, atol=1e-9)
+    target = a + sigma * np.outer(v, v)
+    assert np.abs(lf @ lf.T - target).max() <= 1e-13 * np.abs(target).max()
+    # the updated factor solves the updated system
+    b = inputs.normal(7200 + n, n, 2)
+    x = host(llt.solve(dev(b)))
+    assert po.backward_error(target, x, b) <= 1e-14
+
+
+def test_llt_rank_update_upper_and_failure(gpu):
+    n = 24
+    a = inputs.spd(7000 + n, n)
+    v = inputs.normal(7100 + n, n)
+    lo = ea.LLT(dev(a)).rankUpdate(dev(v), 2.0)
+    up = ea.LLT(dev(a), uplo="U").rankUpdate(dev(v), 2.0)
+    np.testing.assert_allclose(np.triu(host(up.matrixLLT())), np.tril(host(lo.matrixLLT())).T,
+                               rtol=1e-13, atol=1e-14)
+    g = np.load(os.path.join(G, "rankupdate.npz"), allow_pickle=False)
+    w = np.zeros(n)
+    w[3] = 10.0 * np.sqrt(a[3, 3])
+    llt = ea.LLT(dev(a)).rankUpdate(dev(w), -1.0)
+    assert llt.info() == ea.NumericalIssue and int(g["fail_info"]) == 3
+
+
+def test_llt_rank_update_batched(gpu):
+    """Many small factors in one launch (the EKF-style workload): each
+    against the oracle's update of the same GPU factor; a size above 256 is
+    reported, not run."""
+    from eigen_amd import batched
+    rng = np.random.default_rng(11)
+    sizes = rng.choice([6, 9, 12, 15, 18, 24], size=300)
+    sizes[7] = 300
+    offL = np.concatenate([[0], np.cumsum(sizes.astype(np.int64) ** 2)[:-1]])
+    offv = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))[:-1]])
+    L = np.zeros(int((sizes.astype(np.int64) ** 2).sum()))
+    V = rng.standard_normal(int(sizes.sum()))
+    sig = np.where(rng.random(len(sizes)) < 0.5, 1.5, -0.25)
+    for i, n in enumerate(sizes):
+        if n > 256:
+            continue
+        a = inputs.spd(8000 + i, int(n))
+        L[offL[i]:offL[i] + n * n] = np.linalg.cholesky(a).flatten(order="F")
+    Ld = torch.tensor(L, device="cuda")
+    info = batched.llt_rank_update_batched(
+        torch.tensor(offL, device="cuda"), torch.tensor(sizes, dtype=torch.int32, device="cuda"),
+        Ld, torch.tensor(offv, device="cuda"), torch.tensor(V, device="cuda"),
+        torch.tensor(sig, device="cuda"))
+    info = info.cpu().numpy()
+    out = Ld.cpu().numpy()
+    assert info[7] == -2
+    for i, n in enumerate(sizes):
+        if n > 256:
+            continue
+        l0 = L[offL[i]:offL[i] + n * n].reshape(n, n, order="F")
+        ref, rinfo = po.llt_rank_update(l0, V[offv[i]:offv[i] + n], sig[i])
+        assert info[i] == rinfo == -1
+        got = out[offL[i]:offL[i] + n * n].reshape(n, n, order="F")
+        np.testing.assert_allclose(np.tril(got), np.tril(ref), rtol=1e-12, atol=1e-12)

@@ -329,3 +329,66 @@ def test_reference_inverse_fixture_is_an_inverse(grc):
         a = inputs.normal(1000 + n, n, n)
         inv = grc[f"inv{n}"]
         assert np.abs(a @ inv - np.eye(n)).max() < 1e-10 * max(1.0, np.abs(inv).max() * n)
+
+
+# ------------------------------------------------------- LLT rank update --
+RANKUPD_SIZES = [1, 6, 24, 33, 100, 256]
+
+
+@pytest.fixture(scope="module")
+def grk():
+    return np.load(os.path.join(G, "rankupdate.npz"), allow_pickle=False)
+
+
+def _rk_check(g, key, lf):
+    if f"{key}_L" in g.files:
+        np.testing.assert_allclose(np.tril(lf), g[f"{key}_L"], rtol=1e-12, atol=1e-12)
+    else:
+        lf = np.tril(lf)
+        np.testing.assert_allclose(np.diag(lf), g[f"{key}_diag"], rtol=1e-12)
+        np.testing.assert_allclose(lf.sum(axis=1), g[f"{key}_rows"], rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(lf.sum(axis=0), g[f"{key}_cols"], rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("n", RANKUPD_SIZES)
+@pytest.mark.parametrize("tag,sigma", [("p", 2.0), ("m", -0.5)])
+def test_llt_rank_update_matches_reference_fixture(grk, n, tag, sigma):
+    """The oracle's restated rank update (Givens for sigma > 0, the downdate
+    recurrence otherwise) against Eigen's on the same factor, and the
+    defining identity L' L'^T = A + sigma v v^T."""
+    a = inputs.spd(7000 + n, n)
+    v = inputs.normal(7100 + n, n)
+    assert str(grk[f"n{n}_sha"]) == inputs.sha256(a) + inputs.sha256(v)
+    l0, _ = po.llt(a)
+    lf, info = po.llt_rank_update(np.tril(l0), v, sigma)
+    assert info == int(grk[f"n{n}_{tag}_info"]) == -1
+    _rk_check(grk, f"n{n}_{tag}", lf)
+    lf = np.tril(lf)
+    target = a + sigma * np.outer(v, v)
+    assert np.abs(lf @ lf.T - target).max() <= 1e-13 * np.abs(target).max()
+
+
+def test_llt_rank_downdate_failing_column(grk):
+    n = 24
+    a = inputs.spd(7000 + n, n)
+    l0, _ = po.llt(a)
+    v = np.zeros(n)
+    v[3] = 10.0 * np.sqrt(a[3, 3])
+    _, info = po.llt_rank_update(np.tril(l0), v, -1.0)
+    assert info == int(grk["fail_info"]) == 3
+
+
+def test_llt_rank_update_closed_form():
+    """Diagonal A and v = e_k: only L_kk changes, to sqrt(A_kk + sigma)."""
+    n = 7
+    d = np.arange(1.0, n + 1.0)
+    l0 = np.diag(np.sqrt(d))
+    for sigma in (3.0, -0.5):
+        v = np.zeros(n)
+        v[4] = 1.0
+        lf, info = po.llt_rank_update(l0, v, sigma)
+        expect = np.sqrt(d.copy())
+        expect[4] = np.sqrt(d[4] + sigma)
+        assert info == -1
+        np.testing.assert_allclose(np.diag(lf), expect, rtol=1e-15)
+        np.testing.assert_array_equal(np.tril(lf, -1), 0.0)
