@@ -259,19 +259,26 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
 
   const int kslabs = KP / KS;
   gload(0);
-  // the C tile is read while the products run: result row (N index)
-  // 4*lk + r of each 16x16 tile, column (M index) l16
-  float cv[4][4][4];
+  // the C tile is read while the products run.  The products are formed
+  // transposed (A-side operand first), so a lane's four results are four
+  // consecutive M indices 4*lk + r of one N index l16: one 16-byte word of
+  // the row-major C when ldc and C allow it
+  const bool cvec = (ldc & 3) == 0 && ((uintptr_t)C & 15) == 0;
+  v4f cv[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int gi = m0 + wm + i * 16 + l16;
+    const int gi0 = m0 + wm + i * 16 + lk * 4;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      const int gj = n0 + wn + j * 16 + l16;
+      const float* pc = C + (int64_t)gj * ldc + gi0;
+      if (gj < N && cvec && gi0 + 3 < M) {
+        cv[i][j] = *reinterpret_cast<const v4f*>(pc);
+      } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gj = n0 + wn + j * 16 + lk * 4 + r;
-        cv[i][j][r] = (gi < M && gj < N) ? C[(int64_t)gj * ldc + gi] : 0.f;
+        for (int r = 0; r < 4; ++r) cv[i][j][r] = (gj < N && gi0 + r < M) ? pc[r] : 0.f;
       }
+    }
   }
   lstore(0);
   __syncthreads();
@@ -296,16 +303,16 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(bfl[j], af[i], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(af[i], bfl[j], acc[i][j]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(bf[j], afl[i], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(afl[i], bf[j], acc[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(bf[j], af[i], acc[i][j]);
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(af[i], bf[j], acc[i][j]);
     if (ks + 1 < kslabs) lstore(buf ^ 1);
     __syncthreads();
   }
@@ -313,16 +320,21 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
   const float inv = 1.0f / (sa[tm] * sb[tn]);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int gi = m0 + wm + i * 16 + l16;
-    if (gi >= M) continue;
+    const int gi0 = m0 + wm + i * 16 + lk * 4;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      const int gj = n0 + wn + j * 16 + l16;
+      if (gj >= N) continue;
+      float* pc = C + (int64_t)gj * ldc + gi0;
+      const v4f res = cv[i][j] - acc[i][j] * inv;
+      if (cvec && gi0 + 3 < M) {
+        *reinterpret_cast<v4f*>(pc) = res;
+      } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gj = n0 + wn + j * 16 + lk * 4 + r;
-        if (gj >= N) continue;
-        C[(int64_t)gj * ldc + gi] = cv[i][j][r] - acc[i][j][r] * inv;
+        for (int r = 0; r < 4; ++r)
+          if (gi0 + r < M) pc[r] = res[r];
       }
+    }
   }
   }
 }
