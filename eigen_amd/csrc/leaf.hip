@@ -1141,25 +1141,34 @@ __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__
     for (int c = 0; c < LW / 4; ++c)
       reinterpret_cast<v4f*>(row)[c] = v4f{a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]};
   }
-  row += LW;
-  for (int c0 = q; c0 < rest; c0 += kApplyTpr * 8) {
-    float x[8];
+  // the row's rest columns as 16-byte words q, q + 4, ... (rest is a
+  // multiple of 4; LDS reads of U12 stay 4-byte, broadcast across the 16
+  // rows of a wave)
+  v4f* xr = reinterpret_cast<v4f*>(row + LW);
+  const int nq = rest / 4;
+  for (int w0 = q; w0 < nq; w0 += kApplyTpr * 2) {
+    v4f x[2];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int c = c0 + kApplyTpr * t;
-      x[t] = c < rest ? row[c] : 0.f;
+    for (int t = 0; t < 2; ++t) {
+      const int w = w0 + kApplyTpr * t;
+      x[t] = w < nq ? xr[w] : v4f{0.f, 0.f, 0.f, 0.f};
     }
 #pragma unroll 4
-    for (int k = 0; k < LW; ++k)
+    for (int k = 0; k < LW; ++k) {
+      const float* uk = s_u12 + k * rest;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int c = min(c0 + kApplyTpr * t, rest - 1);
-        x[t] -= a[k] * s_u12[k * rest + c];
+      for (int t = 0; t < 2; ++t) {
+        const int c = 4 * min(w0 + kApplyTpr * t, nq - 1);
+        x[t][0] -= a[k] * uk[c];
+        x[t][1] -= a[k] * uk[c + 1];
+        x[t][2] -= a[k] * uk[c + 2];
+        x[t][3] -= a[k] * uk[c + 3];
       }
+    }
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int c = c0 + kApplyTpr * t;
-      if (c < rest) row[c] = x[t];
+    for (int t = 0; t < 2; ++t) {
+      const int w = w0 + kApplyTpr * t;
+      if (w < nq) xr[w] = x[t];
     }
   }
 }
