@@ -7,7 +7,7 @@ The HIP library must be built (``make -C eigen_amd/csrc``); there is no CPU
 fallback.
 """
 from ._lib import EamdError, declared_symbols, lib  # noqa: F401
-from .dense import (LLT, InvalidInput, NoConvergence, NumericalIssue,  # noqa: F401
+from .dense import (LDLT, LLT, InvalidInput, NoConvergence, NumericalIssue,  # noqa: F401
                     PartialPivLU, Success, backward_error, colmajor, gemm, handle,
                     rank_update, triangular_solve)
 

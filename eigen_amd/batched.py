@@ -97,6 +97,40 @@ def llt_rank_update_batched(offL, n, L, offv, v, sigma=None, upper=False):
     return info
 
 
+def ldlt_batched(offA, n, A, offT=None, max_n=None):
+    """LDLT<MatrixXd, Lower>::compute (LDLT.h:491-522) for packed symmetric
+    matrices: A (FP64, matrix i column-major n[i] x n[i] at offA[i]) has its
+    lower triangle replaced by matrixLDLT()'s.  Returns (T, sign, info):
+    transpositions (int32, system i's n[i] entries at offT[i]; offT defaults
+    to the prefix sums of n), sign per system (0 PositiveSemiDef,
+    1 NegativeSemiDef, 2 ZeroSign, 3 Indefinite) and info (0 Success,
+    1 NumericalIssue, -2 for n outside 0..max_n).  max_n (default: the
+    largest n, read once on the host) is at most 64."""
+    dev = n.device
+    count = n.numel()
+    if max_n is None:
+        max_n = int(n.max()) if count else 0
+    if offT is None:
+        offT = (torch.cumsum(n.to(torch.int64), 0) - n.to(torch.int64)).contiguous()
+    total = int(offT[-1] + n[-1].clamp(min=0)) if count else 0
+    T = torch.zeros(max(total, 1), dtype=torch.int32, device=dev)
+    sign = torch.empty(count, dtype=torch.int32, device=dev)
+    info = torch.empty(count, dtype=torch.int32, device=dev)
+    check("eamd_dldlt_batched",
+          lib().eamd_dldlt_batched(handle(), count, _p(offA), _p(n), int(max_n), _p(A), _p(offT),
+                                   _p(T), _p(sign), _p(info)))
+    return T, offT, sign, info
+
+
+def ldlt_solve_batched(offA, n, A, offT, T, offb, b):
+    """LDLT::solve (LDLT.h:559-593) with ldlt_batched's factors: b (FP64,
+    system i's right-hand side at offb[i]) overwritten with the solution."""
+    check("eamd_dldlt_solve_batched",
+          lib().eamd_dldlt_solve_batched(handle(), n.numel(), _p(offA), _p(n), _p(A), _p(offT),
+                                         _p(T), _p(offb), _p(b)))
+    return b
+
+
 def dgesv_ir_batched(offA, n, A, offB, nrhs, B):
     """Batched FP32-LU + FP64-refinement solve; returns (X, info, iters)."""
     X = torch.empty_like(B)
@@ -292,6 +326,35 @@ def bench_rank_update_fields(n, offA, A, offv, steps=5):
             "all_info_ok": bool((info == -1).all().item())}
 
 
+def bench_ldlt_fields(n, offA, A, offb, b, steps=5):
+    """LDLT compute + solve (LDLT.h:491-522, 559-593) of the batched
+    workload's SPD matrices in FP64, one right-hand side each; the packed
+    inputs are restored (untimed) before every timed step."""
+    import time
+    count = n.numel()
+    A0 = A.to(torch.float64)
+    b0 = b.to(torch.float64)
+    offT = offb.to(torch.int64)
+    max_n = int(n.max().item())
+    Aw, bw = A0.clone(), b0.clone()
+    times = []
+    for _ in range(steps):
+        Aw.copy_(A0)
+        bw.copy_(b0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        T, _, sign, info = ldlt_batched(offA, n, Aw, offT, max_n=max_n)
+        ldlt_solve_batched(offA, n, Aw, offT, T, offT, bw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+    dt = min(times)
+    return {"workload": f"{count} FP64 symmetric systems of the batched LLT sizes, LDLT "
+                        "compute + solve (one right-hand side), packed + offsets",
+            "systems_per_s": round(count / dt, 1), "ms": round(dt * 1e3, 3),
+            "all_info_ok": bool((info == 0).all().item()),
+            "all_positive": bool((sign == 0).all().item())}
+
+
 def bench_fields(count=1_000_000, steps=5):
     """Extra fields of the benchmark line for the batched workloads."""
     import time
@@ -354,11 +417,16 @@ def bench_fields(count=1_000_000, steps=5):
         out["batched_llt"]["speedup_vs_per_matrix_loop"] = round(count / dt / loop, 1)
     except Exception as e:  # reported, never hides the batched result
         out["batched_llt"]["per_matrix_loop_error"] = str(e)[:200]
-    del b, b0, x
+    del b, x
     try:
         out["batched_llt_rank_update"] = bench_rank_update_fields(n, offA, A, offb, steps)
     except Exception as e:
         out["batched_llt_rank_update"] = {"error": str(e)[:300]}
+    try:
+        out["batched_ldlt"] = bench_ldlt_fields(n, offA, A, offb, b0, steps)
+    except Exception as e:  # reported, not fatal to the headline line
+        out["batched_ldlt"] = {"error": str(e)[:300]}
+    del b0
     del A
     try:
         out["batched_lu"] = bench_lu_fields()

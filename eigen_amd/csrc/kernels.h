@@ -32,6 +32,11 @@ int env_int(const char* name, int dflt);
 int llt_rank_update_batched(eamd_handle* h, int64_t count, const int64_t* offL, const int32_t* n,
                             double* L, const int64_t* offv, const double* v, const double* sigma,
                             int upper, int32_t* info);
+int ldlt_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n, int max_n,
+                 double* A, const int64_t* offT, int32_t* T, int32_t* sign, int32_t* info);
+int ldlt_solve_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n,
+                       const double* A, const int64_t* offT, const int32_t* T,
+                       const int64_t* offb, double* b);
 
 // ---- level 3 (gemm_f64.hip) ----
 // C = alpha*op(A)*op(B) + beta*C, column-major.  tri: 0 full C, 1 only

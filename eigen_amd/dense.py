@@ -312,6 +312,100 @@ class LLT:
         return _rcond_estimate(self._l1_norm, n, self.solve, self.solve, self._m.device)
 
 
+class LDLT:
+    """Robust Cholesky A = P^T L D L^T P with symmetric diagonal pivoting
+    (Eigen/src/Cholesky/LDLT.h, lower storage), sizes up to 64: one system
+    of the batched GPU factorization."""
+
+    def __init__(self, matrix=None):
+        self._m = None
+        self._info = InvalidInput
+        if matrix is not None:
+            self.compute(matrix)
+
+    def compute(self, matrix):
+        """LDLT.h:491-522; the stored 1-norm is that of the selfadjoint
+        matrix read from the lower triangle (LDLT.h:500-511)."""
+        from . import batched
+        m = colmajor(matrix, torch.float64).clone()
+        m = colmajor(m)
+        n = m.shape[0]
+        if n > 64:
+            raise ValueError("LDLT: sizes up to 64")
+        tri = torch.tril(m)
+        sym = tri + tri.t() - torch.diag(torch.diagonal(tri))
+        self._l1_norm = float(sym.abs().sum(dim=0).max()) if n else 0.0
+        dev = m.device
+        z = torch.zeros(1, dtype=torch.int64, device=dev)
+        nn = torch.tensor([n], dtype=torch.int32, device=dev)
+        T, _, sign, info = batched.ldlt_batched(z, nn, m, z, max_n=n)
+        self._m, self._t = m, T[:n]
+        self._sign = int(sign[0])
+        self._info = Success if int(info[0]) == 0 else NumericalIssue
+        return self
+
+    def info(self):
+        return self._info
+
+    def matrixLDLT(self):
+        return self._m
+
+    def matrixL(self):
+        return torch.tril(self._m, -1) + torch.eye(self._m.shape[0], dtype=torch.float64,
+                                                   device=self._m.device)
+
+    def matrixU(self):
+        return self.matrixL().t()
+
+    def vectorD(self):
+        return torch.diagonal(self._m).clone()
+
+    def transpositionsP(self):
+        """The 0-based transposition indices (LDLT.h:143-147)."""
+        return self._t
+
+    def isPositive(self):
+        return self._sign in (0, 2)
+
+    def isNegative(self):
+        return self._sign in (1, 2)
+
+    def solve(self, b):
+        """x = A^{-1} b, one right-hand side or a column per system
+        (LDLT.h:559-593; D^+ zeroes rows whose |d| is not above 1/DBL_MAX)."""
+        from . import batched
+        n = self._m.shape[0]
+        dev = self._m.device
+        vec = b.dim() == 1
+        x = colmajor(b, torch.float64).clone()
+        x = colmajor(x.reshape(n, -1))
+        r = x.shape[1]
+        z = torch.zeros(r, dtype=torch.int64, device=dev)
+        offb = torch.arange(r, dtype=torch.int64, device=dev) * n
+        nn = torch.full((r,), n, dtype=torch.int32, device=dev)
+        batched.ldlt_solve_batched(z, nn, self._m, z, self._t, offb, x)
+        return x.reshape(-1) if vec else x
+
+    def reconstructedMatrix(self):
+        """P^T L D L^T P (LDLT.h:620-643)."""
+        L = self.matrixL()
+        res = (L * self.vectorD()) @ L.t()
+        # (P^T M P)_ij = M_{s(i), s(j)} with P e_i = e_{s(i)}: s(i) follows
+        # index i through the transpositions in order
+        s = []
+        tr = self._t.tolist()
+        for i in range(len(tr)):
+            pos = i
+            for k, j in enumerate(tr):
+                if pos == k:
+                    pos = j
+                elif pos == j:
+                    pos = k
+            s.append(pos)
+        idx = torch.tensor(s, dtype=torch.int64, device=self._m.device)
+        return res[idx][:, idx]
+
+
 def _rcond_estimate(norm1, n, solve, solve_adj, device):
     """1 / (norm1 * est(||A^{-1}||_1)) from solves with A and A^T: Hager's
     one-norm estimator with Higham's refinements (N. J. Higham, ACM TOMS 14,

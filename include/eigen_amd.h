@@ -196,6 +196,31 @@ int eamd_dllt_rank_update_batched(eamd_handle_t h, int64_t count, const int64_t*
                                   const double* v, const double* sigma, int32_t upper,
                                   int32_t* info);
 
+/* ----------------------------------------------------------------- LDLT -- */
+/* Replaces LDLT<MatrixXd, Lower>::compute (LDLT.h:491-522, through
+ * internal::ldlt_inplace<Lower>::unblocked, LDLT.h:290-395) for many small
+ * symmetric matrices in one launch: matrix i is n[i] x n[i], column-major, at
+ * A + offA[i]; its lower triangle is read and replaced by matrixLDLT()'s (L
+ * strictly below the diagonal, D on it; the strict upper part is left as it
+ * was).  transpositions + offT[i] receives transpositionsP().indices()
+ * (n[i] entries, 0-based), sign[i] the reference's internal::SignMatrix
+ * (0 PositiveSemiDef, 1 NegativeSemiDef, 2 ZeroSign, 3 Indefinite: isPositive()
+ * is 0 or 2, isNegative() 1 or 2), info[i] 0 (Success), 1 (NumericalIssue)
+ * or -2 for n[i] outside 0..max_n.  max_n (at most 64) bounds every n[i] and
+ * sizes the launch's LDS.  Device arrays; asynchronous on the handle's
+ * stream.  Returns -5 for max_n outside 0..64. */
+int eamd_dldlt_batched(eamd_handle_t h, int64_t count, const int64_t* offA, const int32_t* n,
+                       int32_t max_n, double* A, const int64_t* offT, int32_t* transpositions,
+                       int32_t* sign, int32_t* info);
+
+/* Replaces LDLT<MatrixXd, Lower>::solve(b) (LDLT.h:559-593) for each system
+ * factored by eamd_dldlt_batched: b + offb[i] (n[i] values) is overwritten
+ * with P^T L^-T D^+ L^-1 P b, D^+ zeroing the rows whose |d| is not above
+ * 1 / DBL_MAX.  Systems with n[i] outside 0..64 are left as they are. */
+int eamd_dldlt_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
+                             const int32_t* n, const double* A, const int64_t* offT,
+                             const int32_t* transpositions, const int64_t* offb, double* b);
+
 /* -------------------------------------------------------------- batched -- */
 /* No reference counterpart as one launch; per system it is
  * LLT<MatrixXf>(A_i).solve(b_i) (LLT.h:422-505).  Systems are packed

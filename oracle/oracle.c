@@ -562,3 +562,136 @@ void orc_batched_lu_solve_d(long count, const int64_t* offA, const int32_t* n,
   free(work);
   free(tp);
 }
+
+/* ------------------------------------------------------------------------- */
+/* LDLT (symmetric diagonal pivoting), lower storage                          */
+/* ------------------------------------------------------------------------- */
+/* In place on the lower triangle of A (column-major, ld): strictly lower L
+ * (unit diagonal implied), D on the diagonal; trans[k] = the row exchanged
+ * with k at step k; *sign = 0 PositiveSemiDef, 1 NegativeSemiDef,
+ * 2 ZeroSign, 3 Indefinite (internal::SignMatrix, LDLT.h:22).  Returns 1 for
+ * Success, 0 for NumericalIssue.  Restates internal::ldlt_inplace<Lower>::
+ * unblocked (LDLT.h:290-395) with compute()'s sign = ZeroSign start
+ * (LDLT.h:516): the pivot is the first largest |diagonal| of the trailing
+ * block (maxCoeff's strict '>' scan, Visitor.h:173-185), the lower-only
+ * symmetric swap, then the left-looking column
+ * temp = D A10^T, a_kk -= A10 temp, A21 -= A20 temp, A21 /= a_kk. */
+long orc_ldlt_d(long n, double* A, long ld, int* trans, int* sign) {
+  int sg = 2;
+  long ret = 1, found_zero = 0;
+  if (n <= 1) {
+    if (n == 1) {
+      trans[0] = 0;
+      sg = A[0] > 0.0 ? 0 : (A[0] < 0.0 ? 1 : 2);
+    }
+    *sign = sg;
+    return 1;
+  }
+  double* temp = (double*)malloc(sizeof(double) * (size_t)n);
+  for (long k = 0; k < n; ++k) {
+    long p = k;
+    double best = fabs(A[IDX(k, k, ld)]);
+    for (long i = k + 1; i < n; ++i)
+      if (fabs(A[IDX(i, i, ld)]) > best) {
+        best = fabs(A[IDX(i, i, ld)]);
+        p = i;
+      }
+    trans[k] = (int)p;
+    if (p != k) {
+      for (long j = 0; j < k; ++j) {
+        double t = A[IDX(k, j, ld)];
+        A[IDX(k, j, ld)] = A[IDX(p, j, ld)];
+        A[IDX(p, j, ld)] = t;
+      }
+      for (long i = p + 1; i < n; ++i) {
+        double t = A[IDX(i, k, ld)];
+        A[IDX(i, k, ld)] = A[IDX(i, p, ld)];
+        A[IDX(i, p, ld)] = t;
+      }
+      double t = A[IDX(k, k, ld)];
+      A[IDX(k, k, ld)] = A[IDX(p, p, ld)];
+      A[IDX(p, p, ld)] = t;
+      for (long i = k + 1; i < p; ++i) {
+        double u = A[IDX(i, k, ld)];
+        A[IDX(i, k, ld)] = A[IDX(p, i, ld)];
+        A[IDX(p, i, ld)] = u;
+      }
+    }
+    if (k > 0) {
+      double s = 0.0;
+      for (long j = 0; j < k; ++j) {
+        temp[j] = A[IDX(j, j, ld)] * A[IDX(k, j, ld)];
+        s += A[IDX(k, j, ld)] * temp[j];
+      }
+      A[IDX(k, k, ld)] -= s;
+      for (long i = k + 1; i < n; ++i) {
+        double r = 0.0;
+        for (long j = 0; j < k; ++j) r += A[IDX(i, j, ld)] * temp[j];
+        A[IDX(i, k, ld)] -= r;
+      }
+    }
+    const double akk = A[IDX(k, k, ld)];
+    const int valid = fabs(akk) > 0.0;
+    if (k == 0 && !valid) {
+      for (long j = 0; j < n; ++j) {
+        trans[j] = (int)j;
+        for (long i = j + 1; i < n; ++i)
+          if (!(A[IDX(i, j, ld)] == 0.0)) ret = 0;
+      }
+      free(temp);
+      *sign = 2;
+      return ret;
+    }
+    if (k + 1 < n && valid) {
+      for (long i = k + 1; i < n; ++i) A[IDX(i, k, ld)] /= akk;
+    } else if (k + 1 < n) {
+      for (long i = k + 1; i < n; ++i)
+        if (!(A[IDX(i, k, ld)] == 0.0)) ret = 0;
+    }
+    if (found_zero && valid) ret = 0;
+    else if (!valid) found_zero = 1;
+    if (sg == 0) {
+      if (akk < 0.0) sg = 3;
+    } else if (sg == 1) {
+      if (akk > 0.0) sg = 3;
+    } else if (sg == 2) {
+      if (akk > 0.0) sg = 0;
+      else if (akk < 0.0) sg = 1;
+    }
+  }
+  free(temp);
+  *sign = sg;
+  return ret;
+}
+
+/* x <- A^{-1} b from the LDLT factors, nrhs columns (LDLT.h:559-593):
+ * x = P b (the transpositions in order), L^{-1}, D^+ (rows whose |d| is not
+ * above 1 / DBL_MAX are set to zero), L^{-T}, then P^T (in reverse order). */
+void orc_ldlt_solve_d(long n, long nrhs, const double* A, long ld, const int* trans, double* B,
+                      long ldb) {
+  const double tol = 1.0 / 1.7976931348623157e308;
+  for (long c = 0; c < nrhs; ++c) {
+    double* x = B + (size_t)c * (size_t)ldb;
+    for (long k = 0; k < n; ++k) {
+      double t = x[k];
+      x[k] = x[trans[k]];
+      x[trans[k]] = t;
+    }
+    for (long k = 0; k < n; ++k)
+      for (long i = k + 1; i < n; ++i) x[i] -= A[IDX(i, k, ld)] * x[k];
+    for (long i = 0; i < n; ++i) {
+      const double d = A[IDX(i, i, ld)];
+      x[i] = fabs(d) > tol ? x[i] / d : 0.0;
+    }
+    for (long k = n - 1; k >= 0; --k) {
+      double s = x[k];
+      for (long i = k + 1; i < n; ++i) s -= A[IDX(i, k, ld)] * x[i];
+      x[k] = s;
+    }
+    for (long k = n - 1; k >= 0; --k) {
+      double t = x[k];
+      x[k] = x[trans[k]];
+      x[trans[k]] = t;
+    }
+  }
+}

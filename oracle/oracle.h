@@ -63,6 +63,9 @@ void orc_llt_solve_d(long n, long nrhs, const double* l, long ldl, double* b, lo
 void orc_llt_solve_f(long n, long nrhs, const float* l, long ldl, float* b, long ldb);
 /* LLT rank-one update of a lower factor (LLT.h:231-299); -1 or failing column */
 long orc_llt_rank_update_d(long n, double* L, long ld, const double* v, double sigma);
+long orc_ldlt_d(long n, double* A, long ld, int* trans, int* sign);
+void orc_ldlt_solve_d(long n, long nrhs, const double* A, long ld, const int* trans, double* B,
+                      long ldb);
 
 /* ---- level-3 kernels ---- */
 /* C = alpha*op(A)*op(B) + beta*C  (general_matrix_matrix_product,

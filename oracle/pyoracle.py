@@ -63,6 +63,9 @@ def orc():
         lib.orc_batched_llt_solve_f.argtypes = [L, _i64p, _ip, _f1, _i64p, _f1, _ip]
         lib.orc_llt_rank_update_d.argtypes = [L, _dp, L, _dp, C.c_double]
         lib.orc_llt_rank_update_d.restype = L
+        lib.orc_ldlt_d.argtypes = [L, _dp, L, _ip, C.POINTER(C.c_int)]
+        lib.orc_ldlt_d.restype = L
+        lib.orc_ldlt_solve_d.argtypes = [L, L, _dp, L, _ip, _dp, L]
         lib.orc_batched_lu_solve_d.argtypes = [L, _i64p, _ip, _d1, _i64p, L, _d1, _ip]
         _ORC = lib
     return _ORC
@@ -100,6 +103,9 @@ def ref():
         lib.eref_llt_blocked_d.restype = L
         lib.eref_llt_rank_update_d.argtypes = [L, _dp, L, _dp, C.c_double]
         lib.eref_llt_rank_update_d.restype = L
+        lib.eref_ldlt_d.argtypes = [L, _dp, L, _ip, C.POINTER(C.c_int)]
+        lib.eref_ldlt_d.restype = L
+        lib.eref_ldlt_solve_d.argtypes = [L, L, _dp, L, _dp, L]
         lib.eref_gemm_d.argtypes = [L, L, L, _dp, L, _dp, L, _dp, L]
         lib.eref_trsm_d.argtypes = [C.c_char, C.c_char, C.c_char, C.c_char, L, L, _dp, L, _dp, L]
         lib.eref_syrk_lower_d.argtypes = [L, L, C.c_double, _dp, L, _dp, L]
@@ -159,6 +165,44 @@ def llt_rank_update(lfac, v, sigma, reference=False):
     r = (lib.eref_llt_rank_update_d if reference else lib.orc_llt_rank_update_d)(
         n, f, n, vv, float(sigma))
     return f, int(r)
+
+
+def ldlt(a, reference=False):
+    """LDLT<MatrixXd, Lower> (LDLT.h:290-395, 491-522) of a copy of a:
+    returns (matrixLDLT with the strict upper part of a kept, transpositions,
+    sign 0..3 = PositiveSemiDef, NegativeSemiDef, ZeroSign, Indefinite,
+    ok).  reference=True runs Eigen itself (oracle/_ref)."""
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    n = f.shape[0]
+    t = np.zeros(max(n, 1), dtype=np.int32)
+    sg = C.c_int(-1)
+    lib = ref() if reference else orc()
+    r = (lib.eref_ldlt_d if reference else lib.orc_ldlt_d)(n, f, max(n, 1), t, C.byref(sg))
+    return f, t[:n], int(sg.value), bool(r)
+
+
+def ldlt_solve(f, t, b):
+    """x = A^{-1} b from ldlt()'s factors (LDLT.h:559-593)."""
+    b = np.array(b, dtype=np.float64, order="F", copy=True)
+    vec = b.ndim == 1
+    if vec:
+        b = b.reshape(-1, 1, order="F")
+    n = f.shape[0]
+    orc().orc_ldlt_solve_d(n, b.shape[1], np.asfortranarray(f), max(n, 1),
+                           np.ascontiguousarray(t, dtype=np.int32), b, max(n, 1))
+    return b.reshape(-1) if vec else b
+
+
+def ldlt_solve_reference(a, b):
+    """Eigen's LDLT<MatrixXd, Lower>(a).solve(b), run by oracle/_ref."""
+    b = np.array(b, dtype=np.float64, order="F", copy=True)
+    vec = b.ndim == 1
+    if vec:
+        b = b.reshape(-1, 1, order="F")
+    a = np.asfortranarray(np.array(a, dtype=np.float64))
+    n = a.shape[0]
+    ref().eref_ldlt_solve_d(n, b.shape[1], a, max(n, 1), b, max(n, 1))
+    return b.reshape(-1) if vec else b
 
 
 def llt(a, dtype=np.float64):

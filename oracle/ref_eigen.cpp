@@ -120,6 +120,35 @@ long eref_llt_rank_update_d(long n, double* L, long ld, const double* v, double 
   return (long)internal::llt_inplace<double, Lower>::rankUpdate(Lm, vv, sigma);
 }
 
+// LDLT<MatrixXd, Lower>::compute (LDLT.h:491-522): the lower triangle of A
+// (column-major, lda) replaced by matrixLDLT()'s, the transpositions' indices,
+// *sign: 0 PositiveSemiDef, 1 NegativeSemiDef, 2 ZeroSign, 3 Indefinite
+// (read back through isPositive()/isNegative(), LDLT.h:170-181); returns 1
+// for Success, 0 for NumericalIssue
+long eref_ldlt_d(long n, double* a, long lda, int* trans, int* sign) {
+  MapD A(a, n, n, OuterStride<>(lda));
+  LDLT<MatrixXd, Lower> f(n);
+  f.compute(MatrixXd(A));
+  const MatrixXd& m = f.matrixLDLT();
+  for (long j = 0; j < n; ++j)
+    for (long i = j; i < n; ++i) A(i, j) = m(i, j);
+  for (long k = 0; k < n; ++k) trans[k] = (int)f.transpositionsP().indices()(k);
+  const bool pos = f.isPositive(), neg = f.isNegative();
+  *sign = pos && neg ? 2 : (pos ? 0 : (neg ? 1 : 3));
+  return f.info() == Success ? 1 : 0;
+}
+
+// LDLT<MatrixXd, Lower>::compute(A).solve(B) (LDLT.h:559-593), B in place
+void eref_ldlt_solve_d(long n, long nrhs, const double* a, long lda, double* b, long ldb) {
+  MapD A(const_cast<double*>(a), n, n, OuterStride<>(lda));
+  MapD Bm(b, n, nrhs, OuterStride<>(ldb));
+  const MatrixXd Am = A;
+  LDLT<MatrixXd, Lower> f(Am);
+  const MatrixXd Bc = Bm;
+  MatrixXd x = f.solve(Bc);
+  Bm = x;
+}
+
 // internal::llt_inplace<double,Lower>::blocked (LLT.h:330): failing col or -1
 long eref_llt_blocked_d(long n, double* a, long lda) {
   MapD A(a, n, n, OuterStride<>(lda));

@@ -24,6 +24,12 @@ these fixtures are Eigen's outputs on seeded inputs (tests/inputs.py):
             LLT.h:231-299) of Eigen's factor of A = spd(seed, n): the updated
             lower factor and info for sigma = +2 and -0.5, and a downdate
             that fails at column 3 (`make_golden.py rankupdate`).
+  ldlt.npz  LDLT<MatrixXd, Lower> (internal::ldlt_inplace<Lower>::unblocked,
+            LDLT.h:290-395, and solve, :559-593) of symmetric indefinite,
+            positive and negative definite matrices: transpositions, sign,
+            info, the factor (n <= 24; diagonal, row and column sums else) and
+            the solution of A x = b; plus small exact semidefinite and failing
+            cases (`make_golden.py ldlt`).
 Inputs are not stored (they are regenerated from the seeds); their SHA-256
 is, so a diverging generator is detected.
 """
@@ -222,10 +228,56 @@ def rankupdate_main():
     print("rank update fixtures written to", HERE)
 
 
+LDLT_SIZES = [1, 2, 5, 16, 24, 33, 64]
+LDLT_KINDS = ["sym", "pos", "neg"]
+
+
+LDLT_EXACT = {
+    "zero3": np.zeros((3, 3)),
+    "offdiag2": np.array([[0.0, 1.0], [1.0, 0.0]]),
+    "midzero3": np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 1.0], [0.0, 1.0, 0.0]]),
+    "psd4": np.diag([0.0, 3.0, 0.0, 1.0]),
+    "nsd3": np.diag([-1.0, 0.0, -4.0]),
+    "indef4": np.diag([2.0, -5.0, 0.0, 1.0]),
+}
+
+
+def ldlt_main():
+    out = {}
+    for kind in LDLT_KINDS:
+        for n in LDLT_SIZES:
+            a = inputs.ldlt_matrix(kind, n)
+            b = inputs.normal(7600 + n, n)
+            key = f"{kind}{n}"
+            out[key + "_sha"] = np.array(inputs.sha256(a) + inputs.sha256(b))
+            f, t, sg, ok = po.ldlt(a, reference=True)
+            lf = np.tril(f)
+            if n <= 24:
+                out[key + "_f"] = lf
+            else:
+                out[key + "_diag"] = np.diag(lf).copy()
+                out[key + "_rows"] = lf.sum(axis=1)
+                out[key + "_cols"] = lf.sum(axis=0)
+            out[key + "_t"] = t.copy()
+            out[key + "_sign"] = np.array(sg)
+            out[key + "_ok"] = np.array(ok)
+            out[key + "_x"] = po.ldlt_solve_reference(a, b)
+    for name, a in LDLT_EXACT.items():
+        f, t, sg, ok = po.ldlt(np.asfortranarray(a), reference=True)
+        out[name + "_f"] = np.tril(f)
+        out[name + "_t"] = t.copy()
+        out[name + "_sign"] = np.array(sg)
+        out[name + "_ok"] = np.array(ok)
+    np.savez_compressed(os.path.join(HERE, "ldlt.npz"), **out)
+    print("ldlt fixtures written to", HERE)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "rcond":
         rcond_main()
     elif len(sys.argv) > 1 and sys.argv[1] == "rankupdate":
         rankupdate_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "ldlt":
+        ldlt_main()
     else:
         main()

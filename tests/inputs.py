@@ -50,3 +50,14 @@ def packed_systems(seed, count, sizes, dtype, spd_systems, nrhs=1):
         A[offA[i]:offA[i] + k * k] = a.astype(dtype).ravel(order="F")
     B = g.standard_normal(int((n64 * nrhs).sum())).astype(dtype)
     return n, offA, A, offB, B
+
+
+def ldlt_matrix(kind, n):
+    """Symmetric test matrices of the LDLT fixtures: "sym" (indefinite),
+    "pos" (positive definite) or "neg" (negative definite)."""
+    if kind == "sym":
+        g = normal(7300 + n, n, n)
+        return np.asfortranarray(g + g.T)
+    if kind == "pos":
+        return spd(7400 + n, n)
+    return np.asfortranarray(-spd(7500 + n, n))

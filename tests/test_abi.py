@@ -131,3 +131,11 @@ def test_lapack_layer_argument_codes():
     # empty problems return at once with info 0
     L.dgetrf_(_ip(0), _ip(0), None, _ip(1), piv, C.byref(info))
     assert info.value == 0
+
+
+def test_ldlt_argument_codes():
+    L = eigen_amd.lib()
+    assert L.eamd_dldlt_batched(NULL, -1, NULL, NULL, 8, NULL, NULL, NULL, NULL, NULL) == -2
+    assert L.eamd_dldlt_batched(NULL, 4, NULL, NULL, 65, NULL, NULL, NULL, NULL, NULL) == -5
+    assert L.eamd_dldlt_batched(NULL, 4, NULL, NULL, -1, NULL, NULL, NULL, NULL, NULL) == -5
+    assert L.eamd_dldlt_solve_batched(NULL, -3, NULL, NULL, NULL, NULL, NULL, NULL, NULL) == -2

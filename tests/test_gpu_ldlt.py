@@ -1,0 +1,179 @@
+"""GPU parity of the batched LDLT (LDLT.h:290-395 compute, :559-593 solve)
+against Eigen's outputs (tests/golden/ldlt.npz) and the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import eigen_amd as ea
+from eigen_amd import batched
+from oracle import pyoracle as po
+from tests import inputs
+from tests.golden.make_golden import LDLT_EXACT
+from tests.test_oracle import (LDLT_KINDS, LDLT_SIZES, ldlt_check_factor, ldlt_reconstruct)
+
+pytestmark = pytest.mark.gpu
+
+D64 = dict(dtype=torch.float64, device="cuda")
+
+
+@pytest.fixture(scope="module")
+def gld():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "ldlt.npz"),
+                   allow_pickle=False)
+
+
+def pack(mats):
+    """Column-major packing of square matrices: (offsets, sizes, flat)."""
+    sizes = [m.shape[0] for m in mats]
+    off = np.concatenate([[0], np.cumsum([s * s for s in sizes])[:-1]]).astype(np.int64)
+    flat = np.concatenate([np.asarray(m, dtype=np.float64).reshape(-1, order="F")
+                           for m in mats] + [np.zeros(1)])
+    return off, np.array(sizes, dtype=np.int32), flat
+
+
+def run_ldlt(mats, max_n=None):
+    off, sizes, flat = pack(mats)
+    offA = torch.from_numpy(off).cuda()
+    n = torch.from_numpy(sizes).cuda()
+    A = torch.from_numpy(flat).cuda()
+    T, offT, sign, info = batched.ldlt_batched(offA, n, A, max_n=max_n)
+    torch.cuda.synchronize()
+    return offA, n, A, T, offT, sign, info, off, sizes
+
+
+def unpack(A, off, sizes, i):
+    k = int(sizes[i])
+    return A[int(off[i]):int(off[i]) + k * k].cpu().numpy().reshape(k, k, order="F")
+
+
+def test_ldlt_batched_matches_reference_fixture(gld):
+    """Every fixture matrix in one uneven batch: transpositions, sign and
+    info equal Eigen's; the factor within 1e-10 (indefinite) / 1e-13
+    (definite) of Eigen's, relative to its largest entry; the strict upper
+    part untouched; the solve against Eigen's solution."""
+    keys, mats, rhs = [], [], []
+    for kind in LDLT_KINDS:
+        for n in LDLT_SIZES:
+            keys.append(f"{kind}{n}")
+            mats.append(inputs.ldlt_matrix(kind, n))
+            rhs.append(inputs.normal(7600 + n, n))
+    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt(mats)
+    Th, sh, ih = T.cpu().numpy(), sign.cpu().numpy(), info.cpu().numpy()
+    ot = offT.cpu().numpy()
+    for i, key in enumerate(keys):
+        k = int(sizes[i])
+        print("ldlt fixture", key, flush=True)
+        np.testing.assert_array_equal(Th[ot[i]:ot[i] + k], gld[key + "_t"])
+        assert int(sh[i]) == int(gld[key + "_sign"])
+        assert (int(ih[i]) == 0) == bool(gld[key + "_ok"]) and int(ih[i]) in (0, 1)
+        f = unpack(A, off, sizes, i)
+        ldlt_check_factor(gld, key, f, 1e-10 if key.startswith("sym") else 1e-13)
+        np.testing.assert_array_equal(np.triu(f, 1), np.triu(mats[i], 1))
+    offb = torch.from_numpy(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)).cuda()
+    b = torch.from_numpy(np.concatenate(rhs)).cuda()
+    batched.ldlt_solve_batched(offA, n, A, offT, T, offb, b)
+    bh = b.cpu().numpy()
+    ob = offb.cpu().numpy()
+    for i, key in enumerate(keys):
+        k = int(sizes[i])
+        x, xr = bh[ob[i]:ob[i] + k], gld[key + "_x"]
+        assert np.abs(x - xr).max() <= 1e-8 * max(1.0, np.abs(xr).max()), key
+
+
+def test_ldlt_exact_cases_bit_for_bit(gld):
+    """Zero, semidefinite and failing matrices: factor, transpositions,
+    sign and info exactly Eigen's."""
+    names = list(LDLT_EXACT)
+    mats = [LDLT_EXACT[k] for k in names]
+    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt(mats)
+    Th, ot = T.cpu().numpy(), offT.cpu().numpy()
+    for i, name in enumerate(names):
+        k = int(sizes[i])
+        np.testing.assert_array_equal(Th[ot[i]:ot[i] + k], gld[name + "_t"])
+        assert int(sign[i]) == int(gld[name + "_sign"]), name
+        assert (int(info[i]) == 0) == bool(gld[name + "_ok"]), name
+        np.testing.assert_array_equal(np.tril(unpack(A, off, sizes, i)), gld[name + "_f"])
+
+
+def test_ldlt_many_uneven_systems_against_oracle():
+    """3,000 systems of sizes 0..64 (indefinite, definite, semidefinite
+    rank-deficient by an exact zero block): every output against the oracle
+    on the same input; P^T L D L^T P = A."""
+    rng = np.random.default_rng(11)
+    mats = []
+    for i in range(3000):
+        k = int(rng.integers(0, 65))
+        g = rng.standard_normal((k, k))
+        kind = i % 4
+        if kind == 0:
+            a = g + g.T
+        elif kind == 1:
+            a = g @ g.T + k * np.eye(k)
+        elif kind == 2:
+            a = -(g @ g.T) - np.eye(k)
+        else:
+            a = np.zeros((k, k))
+            h = k // 2
+            a[:h, :h] = (g @ g.T)[:h, :h] + np.eye(h)
+        mats.append(np.asfortranarray(a))
+    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt(mats, max_n=64)
+    Th, ot = T.cpu().numpy(), offT.cpu().numpy()
+    sh, ih = sign.cpu().numpy(), info.cpu().numpy()
+    Ah = A.cpu().numpy()
+    for i, a in enumerate(mats):
+        k = a.shape[0]
+        f, t, sg, ok = po.ldlt(a)
+        np.testing.assert_array_equal(Th[ot[i]:ot[i] + k], t, err_msg=str(i))
+        assert int(sh[i]) == sg and (int(ih[i]) == 0) == ok, i
+        if k == 0:
+            continue
+        fg = Ah[int(off[i]):int(off[i]) + k * k].reshape(k, k, order="F")
+        scale = max(1.0, np.abs(np.tril(f)).max())
+        # indefinite inputs: pivoting on the original diagonal lets entries
+        # grow, and rounding differences grow with them (Eigen's own factor
+        # differs from the oracle's by up to 8e-8 of its largest entry on
+        # these inputs); definite ones stay near machine precision
+        tol = 1e-6 if i % 4 == 0 else 1e-12
+        assert np.abs(np.tril(fg) - np.tril(f)).max() <= tol * scale, i
+        if i % 4 != 3:
+            # backward error: Eigen's and the oracle's stay below 4e-13 here
+            assert np.abs(ldlt_reconstruct(fg, t) - a).max() <= 1e-11 * max(1.0, np.abs(a).max()) * k
+
+
+def test_ldlt_size_limits():
+    """A size above max_n reports -2 and leaves its matrix alone; max_n
+    above 64 is refused at the boundary."""
+    a = np.eye(5)
+    b = np.diag([1.0, 2.0, 3.0])
+    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt([a, b], max_n=4)
+    assert int(info[0]) == -2 and int(info[1]) == 0
+    np.testing.assert_array_equal(unpack(A, off, sizes, 0), a)
+    big = np.eye(65)
+    with pytest.raises(Exception):
+        run_ldlt([big], max_n=65)
+
+
+def test_ldlt_class_mirror(gld):
+    """LDLT(A): matrixLDLT / transpositionsP / vectorD / isPositive /
+    isNegative / solve / reconstructedMatrix on the fixture's n = 24."""
+    for kind in LDLT_KINDS:
+        a = inputs.ldlt_matrix(kind, 24)
+        b = inputs.normal(7624, 24)
+        f = ea.LDLT(torch.from_numpy(a).cuda())
+        key = f"{kind}24"
+        assert f.info() == ea.Success
+        np.testing.assert_array_equal(f.transpositionsP().cpu().numpy(), gld[key + "_t"])
+        sg = int(gld[key + "_sign"])
+        assert f.isPositive() == (sg in (0, 2)) and f.isNegative() == (sg in (1, 2))
+        ldlt_check_factor(gld, key, f.matrixLDLT().cpu().numpy(),
+                          1e-10 if kind == "sym" else 1e-13)
+        x = f.solve(torch.from_numpy(b).cuda()).cpu().numpy()
+        xr = gld[key + "_x"]
+        assert np.abs(x - xr).max() <= 1e-8 * max(1.0, np.abs(xr).max())
+        B = np.stack([b, 2 * b, -b], axis=1)
+        X = f.solve(torch.from_numpy(np.asfortranarray(B)).cuda()).cpu().numpy()
+        np.testing.assert_allclose(X, np.stack([x, 2 * x, -x], axis=1), rtol=1e-12,
+                                   atol=1e-12 * np.abs(x).max())
+        r = f.reconstructedMatrix().cpu().numpy()
+        assert np.abs(r - a).max() <= 1e-12 * np.abs(a).max() * 24

@@ -392,3 +392,126 @@ def test_llt_rank_update_closed_form():
         assert info == -1
         np.testing.assert_allclose(np.diag(lf), expect, rtol=1e-15)
         np.testing.assert_array_equal(np.tril(lf, -1), 0.0)
+
+
+# ------------------------------------------------------------------ LDLT --
+LDLT_SIZES = [1, 2, 5, 16, 24, 33, 64]
+LDLT_KINDS = ["sym", "pos", "neg"]
+LDLT_EXACT = ["zero3", "offdiag2", "midzero3", "psd4", "nsd3", "indef4"]
+
+
+@pytest.fixture(scope="module")
+def gld():
+    return np.load(os.path.join(G, "ldlt.npz"), allow_pickle=False)
+
+
+def ldlt_check_factor(g, key, f, rtol):
+    """The lower factor against the fixture: whole (n <= 24) or its
+    diagonal, row and column sums."""
+    lf = np.tril(f)
+    scale = max(1.0, np.abs(lf).max())
+    if f"{key}_f" in g.files:
+        assert np.abs(lf - g[f"{key}_f"]).max() <= rtol * scale
+    else:
+        assert np.abs(np.diag(lf) - g[f"{key}_diag"]).max() <= rtol * scale
+        n = lf.shape[0]
+        assert np.abs(lf.sum(axis=1) - g[f"{key}_rows"]).max() <= rtol * scale * n
+        assert np.abs(lf.sum(axis=0) - g[f"{key}_cols"]).max() <= rtol * scale * n
+
+
+def ldlt_reconstruct(f, t):
+    """P^T L D L^T P from a factor and its transpositions; (P e_i = e_s(i),
+    s(i) following i through the transpositions in order)."""
+    n = f.shape[0]
+    L = np.tril(f, -1) + np.eye(n)
+    res = (L * np.diag(f)) @ L.T
+    s = []
+    for i in range(n):
+        pos = i
+        for k, j in enumerate(t):
+            if pos == k:
+                pos = int(j)
+            elif pos == j:
+                pos = k
+        s.append(pos)
+    return res[np.ix_(s, s)]
+
+
+@pytest.mark.parametrize("kind", LDLT_KINDS)
+@pytest.mark.parametrize("n", LDLT_SIZES)
+def test_ldlt_matches_reference_fixture(gld, kind, n):
+    """The restated LDLT (pivot rule, lower-only exchange, left-looking
+    column, sign and info) against Eigen's outputs on the same matrix, and
+    the defining identity P^T L D L^T P = A."""
+    a = inputs.ldlt_matrix(kind, n)
+    b = inputs.normal(7600 + n, n)
+    key = f"{kind}{n}"
+    assert str(gld[key + "_sha"]) == inputs.sha256(a) + inputs.sha256(b)
+    f, t, sg, ok = po.ldlt(a)
+    np.testing.assert_array_equal(t, gld[key + "_t"])
+    assert sg == int(gld[key + "_sign"]) and ok == bool(gld[key + "_ok"])
+    ldlt_check_factor(gld, key, f, 1e-10 if kind == "sym" else 1e-13)
+    assert np.abs(ldlt_reconstruct(f, t) - a).max() <= 1e-12 * max(1.0, np.abs(a).max()) * n
+    x = po.ldlt_solve(f, t, b)
+    xr = gld[key + "_x"]
+    assert np.abs(x - xr).max() <= 1e-8 * max(1.0, np.abs(xr).max())
+    assert np.abs(a @ x - b).max() <= 1e-9 * (np.abs(a).max() * np.abs(x).max() + 1.0) * n
+
+
+@pytest.mark.parametrize("name", LDLT_EXACT)
+def test_ldlt_exact_cases(gld, name):
+    """Semidefinite, all-zero and failing inputs whose factors are exact:
+    transpositions, sign, info and factor equal Eigen's bit for bit."""
+    from tests.golden.make_golden import LDLT_EXACT as MATS
+    f, t, sg, ok = po.ldlt(np.asfortranarray(MATS[name]))
+    np.testing.assert_array_equal(t, gld[name + "_t"])
+    assert sg == int(gld[name + "_sign"]) and ok == bool(gld[name + "_ok"])
+    np.testing.assert_array_equal(np.tril(f), gld[name + "_f"])
+
+
+def test_ldlt_closed_forms():
+    """2 x 2 by hand, the pivot order of a diagonal matrix (largest |d|
+    first, the lower index on ties) and the sign states."""
+    a, b, c = 4.0, 2.0, -3.0
+    f, t, sg, ok = po.ldlt(np.array([[a, b], [b, c]]))
+    assert list(t) == [0, 1] and ok and sg == 3
+    np.testing.assert_allclose([f[0, 0], f[1, 0], f[1, 1]], [a, b / a, c - b * b / a],
+                               rtol=1e-15)
+    f, t, sg, ok = po.ldlt(np.array([[c, b], [b, a]]))  # pivot 1 first
+    assert list(t) == [1, 1] and ok
+    np.testing.assert_allclose([f[0, 0], f[1, 0], f[1, 1]], [a, b / a, c - b * b / a],
+                               rtol=1e-15)
+    d = np.array([1.0, -7.0, 3.0, 7.0, -2.0])
+    f, t, sg, ok = po.ldlt(np.diag(d))
+    assert list(t) == [1, 3, 2, 4, 4] and ok and sg == 3
+    np.testing.assert_array_equal(np.diag(f), [-7.0, 7.0, 3.0, -2.0, 1.0])
+    assert po.ldlt(np.diag([2.0, 0.0, 1.0]))[2] == 0  # positive semidefinite
+    assert po.ldlt(np.diag([-2.0, 0.0]))[2] == 1
+    assert po.ldlt(np.zeros((4, 4)))[2:] == (2, True)
+    assert po.ldlt(np.zeros((0, 0)))[2:] == (2, True)
+
+
+def test_ldlt_brute_force_small():
+    """Tiny random matrices: the factor is the unique unit-lower L and
+    diagonal D with P A P^T = L D L^T for the recorded P, compared against
+    a direct elimination of the permuted matrix."""
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        n = int(rng.integers(1, 6))
+        g = rng.standard_normal((n, n))
+        a = g + g.T
+        f, t, sg, ok = po.ldlt(a)
+        # apply the transpositions to A symmetrically, then eliminate without pivoting
+        pa = a.copy()
+        for k, j in enumerate(t):
+            pa[[k, j], :] = pa[[j, k], :]
+            pa[:, [k, j]] = pa[:, [j, k]]
+        L = np.eye(n)
+        D = np.zeros(n)
+        w = pa.copy()
+        for k in range(n):
+            D[k] = w[k, k]
+            L[k + 1:, k] = w[k + 1:, k] / D[k]
+            w[k + 1:, k + 1:] -= np.outer(L[k + 1:, k], w[k, k + 1:])
+        np.testing.assert_allclose(np.diag(f), D, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(np.tril(f, -1), np.tril(L, -1), rtol=1e-10, atol=1e-12)
