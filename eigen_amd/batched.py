@@ -122,12 +122,15 @@ def ldlt_batched(offA, n, A, offT=None, max_n=None):
     return T, offT, sign, info
 
 
-def ldlt_solve_batched(offA, n, A, offT, T, offb, b):
+def ldlt_solve_batched(offA, n, A, offT, T, offb, b, max_n=None):
     """LDLT::solve (LDLT.h:559-593) with ldlt_batched's factors: b (FP64,
-    system i's right-hand side at offb[i]) overwritten with the solution."""
+    system i's right-hand side at offb[i]) overwritten with the solution;
+    max_n as for ldlt_batched."""
+    if max_n is None:
+        max_n = int(n.max()) if n.numel() else 0
     check("eamd_dldlt_solve_batched",
-          lib().eamd_dldlt_solve_batched(handle(), n.numel(), _p(offA), _p(n), _p(A), _p(offT),
-                                         _p(T), _p(offb), _p(b)))
+          lib().eamd_dldlt_solve_batched(handle(), n.numel(), _p(offA), _p(n), int(max_n), _p(A),
+                                         _p(offT), _p(T), _p(offb), _p(b)))
     return b
 
 
@@ -344,7 +347,7 @@ def bench_ldlt_fields(n, offA, A, offb, b, steps=5):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         T, _, sign, info = ldlt_batched(offA, n, Aw, offT, max_n=max_n)
-        ldlt_solve_batched(offA, n, Aw, offT, T, offT, bw)
+        ldlt_solve_batched(offA, n, Aw, offT, T, offT, bw, max_n=max_n)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t1)
     dt = min(times)

@@ -625,19 +625,21 @@ int eamd_dldlt_batched(eamd_handle_t h, int64_t count, const int64_t* offA, cons
 }
 
 int eamd_dldlt_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
-                             const int32_t* n, const double* A, const int64_t* offT,
-                             const int32_t* transpositions, const int64_t* offb, double* b) {
+                             const int32_t* n, int32_t max_n, const double* A,
+                             const int64_t* offT, const int32_t* transpositions,
+                             const int64_t* offb, double* b) {
   if (count < 0) return -2;
+  if (max_n < 0 || max_n > 64) return -5;
   CHECK_HANDLE(h);
   if (count == 0) return kOk;
   if (!offA) return -3;
   if (!n) return -4;
-  if (!A) return -5;
-  if (!offT) return -6;
-  if (!transpositions) return -7;
-  if (!offb) return -8;
-  if (!b) return -9;
-  return ldlt_solve_batched(h, count, offA, n, A, offT, transpositions, offb, b);
+  if (!A) return -6;
+  if (!offT) return -7;
+  if (!transpositions) return -8;
+  if (!offb) return -9;
+  if (!b) return -10;
+  return ldlt_solve_batched(h, count, offA, n, max_n, A, offT, transpositions, offb, b);
 }
 
 int eamd_spotrf_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,

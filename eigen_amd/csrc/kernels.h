@@ -35,7 +35,7 @@ int llt_rank_update_batched(eamd_handle* h, int64_t count, const int64_t* offL, 
 int ldlt_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n, int max_n,
                  double* A, const int64_t* offT, int32_t* T, int32_t* sign, int32_t* info);
 int ldlt_solve_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n,
-                       const double* A, const int64_t* offT, const int32_t* T,
+                       int max_n, const double* A, const int64_t* offT, const int32_t* T,
                        const int64_t* offb, double* b);
 
 // ---- level 3 (gemm_f64.hip) ----

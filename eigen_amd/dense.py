@@ -383,7 +383,7 @@ class LDLT:
         z = torch.zeros(r, dtype=torch.int64, device=dev)
         offb = torch.arange(r, dtype=torch.int64, device=dev) * n
         nn = torch.full((r,), n, dtype=torch.int32, device=dev)
-        batched.ldlt_solve_batched(z, nn, self._m, z, self._t, offb, x)
+        batched.ldlt_solve_batched(z, nn, self._m, z, self._t, offb, x, max_n=n)
         return x.reshape(-1) if vec else x
 
     def reconstructedMatrix(self):

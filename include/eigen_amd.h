@@ -216,10 +216,12 @@ int eamd_dldlt_batched(eamd_handle_t h, int64_t count, const int64_t* offA, cons
 /* Replaces LDLT<MatrixXd, Lower>::solve(b) (LDLT.h:559-593) for each system
  * factored by eamd_dldlt_batched: b + offb[i] (n[i] values) is overwritten
  * with P^T L^-T D^+ L^-1 P b, D^+ zeroing the rows whose |d| is not above
- * 1 / DBL_MAX.  Systems with n[i] outside 0..64 are left as they are. */
+ * 1 / DBL_MAX.  max_n as for the factorization (returns -5 outside 0..64);
+ * systems with n[i] outside 0..max_n are left as they are. */
 int eamd_dldlt_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
-                             const int32_t* n, const double* A, const int64_t* offT,
-                             const int32_t* transpositions, const int64_t* offb, double* b);
+                             const int32_t* n, int32_t max_n, const double* A,
+                             const int64_t* offT, const int32_t* transpositions,
+                             const int64_t* offb, double* b);
 
 /* -------------------------------------------------------------- batched -- */
 /* No reference counterpart as one launch; per system it is

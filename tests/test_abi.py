@@ -138,4 +138,5 @@ def test_ldlt_argument_codes():
     assert L.eamd_dldlt_batched(NULL, -1, NULL, NULL, 8, NULL, NULL, NULL, NULL, NULL) == -2
     assert L.eamd_dldlt_batched(NULL, 4, NULL, NULL, 65, NULL, NULL, NULL, NULL, NULL) == -5
     assert L.eamd_dldlt_batched(NULL, 4, NULL, NULL, -1, NULL, NULL, NULL, NULL, NULL) == -5
-    assert L.eamd_dldlt_solve_batched(NULL, -3, NULL, NULL, NULL, NULL, NULL, NULL, NULL) == -2
+    assert L.eamd_dldlt_solve_batched(NULL, -3, NULL, NULL, 8, NULL, NULL, NULL, NULL, NULL) == -2
+    assert L.eamd_dldlt_solve_batched(NULL, 3, NULL, NULL, 70, NULL, NULL, NULL, NULL, NULL) == -5

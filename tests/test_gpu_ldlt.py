@@ -96,14 +96,16 @@ def test_ldlt_exact_cases_bit_for_bit(gld):
         np.testing.assert_array_equal(np.tril(unpack(A, off, sizes, i)), gld[name + "_f"])
 
 
-def test_ldlt_many_uneven_systems_against_oracle():
-    """3,000 systems of sizes 0..64 (indefinite, definite, semidefinite
-    rank-deficient by an exact zero block): every output against the oracle
-    on the same input; P^T L D L^T P = A."""
-    rng = np.random.default_rng(11)
+@pytest.mark.parametrize("max_n,count", [(64, 3000), (32, 2001), (16, 1001)])
+def test_ldlt_many_uneven_systems_against_oracle(max_n, count):
+    """Systems of sizes 0..max_n (indefinite, definite, semidefinite
+    rank-deficient by an exact zero block; 1, 2 or 4 systems per wavefront,
+    the last wavefront part-filled): every output against the oracle on the
+    same input; P^T L D L^T P = A; then the solve against the oracle's."""
+    rng = np.random.default_rng(11 + max_n)
     mats = []
-    for i in range(3000):
-        k = int(rng.integers(0, 65))
+    for i in range(count):
+        k = int(rng.integers(0, max_n + 1))
         g = rng.standard_normal((k, k))
         kind = i % 4
         if kind == 0:
@@ -117,7 +119,8 @@ def test_ldlt_many_uneven_systems_against_oracle():
             h = k // 2
             a[:h, :h] = (g @ g.T)[:h, :h] + np.eye(h)
         mats.append(np.asfortranarray(a))
-    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt(mats, max_n=64)
+    print("ldlt uneven batch max_n", max_n, "count", count, flush=True)
+    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt(mats, max_n=max_n)
     Th, ot = T.cpu().numpy(), offT.cpu().numpy()
     sh, ih = sign.cpu().numpy(), info.cpu().numpy()
     Ah = A.cpu().numpy()
@@ -139,6 +142,21 @@ def test_ldlt_many_uneven_systems_against_oracle():
         if i % 4 != 3:
             # backward error: Eigen's and the oracle's stay below 4e-13 here
             assert np.abs(ldlt_reconstruct(fg, t) - a).max() <= 1e-11 * max(1.0, np.abs(a).max()) * k
+    # the solve with the GPU factors against the oracle's solve with them
+    offb_h = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    bh = rng.standard_normal(int(sizes.sum()) + 1)
+    b = torch.from_numpy(bh.copy()).cuda()
+    batched.ldlt_solve_batched(offA, n, A, offT, T, torch.from_numpy(offb_h).cuda(), b,
+                               max_n=max_n)
+    xg = b.cpu().numpy()
+    for i, a in enumerate(mats):
+        k = a.shape[0]
+        if k == 0 or i % 4 == 3:
+            continue
+        fg = Ah[int(off[i]):int(off[i]) + k * k].reshape(k, k, order="F")
+        o = int(offb_h[i])
+        x = po.ldlt_solve(fg, Th[ot[i]:ot[i] + k], bh[o:o + k])
+        assert np.abs(xg[o:o + k] - x).max() <= 1e-10 * max(1.0, np.abs(x).max()), i
 
 
 def test_ldlt_size_limits():
