@@ -84,9 +84,25 @@ __global__ __launch_bounds__(64) void ldlt_batch_kernel(int64_t count,
   double* temp = M + max_n * (max_n + 1);
   const unsigned long long segmask =
       W == 64 ? ~0ull : (((1ull << (W & 63)) - 1ull) << (sub * W));
-  // the lower triangle, one coalesced column at a time (lane = row)
-  for (int j = 0; j < nl; ++j)
-    if (sl >= j && sl < nl) M[sl + j * ld] = a[sl + (int64_t)j * nl];
+  // the matrix as one contiguous run of nl * nl values, eight loads per
+  // lane in flight before any is stored (the lower triangle is kept)
+  const int ne = nl * nl;
+  for (int e0 = 0; e0 < ne; e0 += 8 * W) {
+    double vals[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * W + sl;
+      vals[u] = e < ne ? a[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * W + sl;
+      if (e < ne) {
+        const int j = e / nl, i = e - j * nl;
+        if (i >= j) M[i + j * ld] = vals[u];
+      }
+    }
+  }
   __syncthreads();
 
   int sg = 2;
@@ -221,12 +237,25 @@ __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
   const int n = nv[s];
   if (n <= 0 || n > W) return;
   const double* a = Aall + offA[s];
-  const int32_t* tr = Tall + offT[s];
   double* xp = B + offb[s];
+  // everything a lane needs, loaded up front so the loads overlap: its
+  // transposition, right-hand side, d_i, and (W <= 32) row i and column i
+  // of L in registers
+  const int trv = sl < n ? Tall[offT[s] + sl] : 0;
   double x = sl < n ? xp[sl] : 0.0;
+  const double d = sl < n ? a[sl + (int64_t)sl * n] : 1.0;
+  constexpr bool kRegs = W <= 32;
+  double lrow[kRegs ? W : 1], lcol[kRegs ? W : 1];
+  if constexpr (kRegs) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      lrow[k] = (k < sl && sl < n) ? a[sl + (int64_t)k * n] : 0.0;  // L(i, k)
+      lcol[k] = (k > sl && k < n) ? a[k + (int64_t)sl * n] : 0.0;   // L(k, i)
+    }
+  }
   // x = P b: the transpositions in order
   for (int k = 0; k < n; ++k) {
-    const int p = tr[k];
+    const int p = __shfl(trv, base + k);
     if (p != k) {
       const double xk = __shfl(x, base + k), xq = __shfl(x, base + p);
       if (sl == k) x = xq;
@@ -234,23 +263,38 @@ __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
     }
   }
   // L y = x (unit lower), column k of L per step
-  for (int k = 0; k < n - 1; ++k) {
-    const double xk = __shfl(x, base + k);
-    if (sl > k && sl < n) x -= a[sl + (int64_t)k * n] * xk;
+  if constexpr (kRegs) {
+#pragma unroll
+    for (int k = 0; k < W - 1; ++k)
+      if (k < n - 1) {
+        const double xk = __shfl(x, base + k);
+        if (sl > k) x -= lrow[k] * xk;
+      }
+  } else {
+    for (int k = 0; k < n - 1; ++k) {
+      const double xk = __shfl(x, base + k);
+      if (sl > k && sl < n) x -= a[sl + (int64_t)k * n] * xk;
+    }
   }
   // D^+ y: rows whose |d| is not above 1 / DBL_MAX become zero
-  if (sl < n) {
-    const double d = a[sl + (int64_t)sl * n];
-    x = fabs(d) > 1.0 / 1.7976931348623157e308 ? x / d : 0.0;
-  }
+  if (sl < n) x = fabs(d) > 1.0 / 1.7976931348623157e308 ? x / d : 0.0;
   // L^T x = y, row i of L per step
-  for (int i = n - 1; i > 0; --i) {
-    const double xi = __shfl(x, base + i);
-    if (sl < i) x -= a[i + (int64_t)sl * n] * xi;
+  if constexpr (kRegs) {
+#pragma unroll
+    for (int i = W - 1; i > 0; --i)
+      if (i < n) {
+        const double xi = __shfl(x, base + i);
+        if (sl < i) x -= lcol[i] * xi;
+      }
+  } else {
+    for (int i = n - 1; i > 0; --i) {
+      const double xi = __shfl(x, base + i);
+      if (sl < i) x -= a[i + (int64_t)sl * n] * xi;
+    }
   }
   // x = P^T x: the transpositions in reverse order
   for (int k = n - 1; k >= 0; --k) {
-    const int p = tr[k];
+    const int p = __shfl(trv, base + k);
     if (p != k) {
       const double xk = __shfl(x, base + k), xq = __shfl(x, base + p);
       if (sl == k) x = xq;
