@@ -195,3 +195,56 @@ def test_ldlt_class_mirror(gld):
                                    atol=1e-12 * np.abs(x).max())
         r = f.reconstructedMatrix().cpu().numpy()
         assert np.abs(r - a).max() <= 1e-12 * np.abs(a).max() * 24
+
+
+def test_ldlt_graph_replays():
+    """Factor + solve captured in one graph (no host synchronisation in
+    either entry point): three replays, with the inputs restored and eager
+    work in between, each equal to the eager result bit for bit."""
+    from eigen_amd._lib import check, lib
+    from eigen_amd.dense import _p, handle
+    n, offA, A0, offb, b0 = batched.packed_spd_gpu(5000, [6, 9, 12, 15, 18, 24], seed=13)
+    A0, b0 = A0.to(torch.float64), b0.to(torch.float64)
+    offT = offb.to(torch.int64)
+    count = n.numel()
+    # eager reference
+    Ae, be = A0.clone(), b0.clone()
+    Te, _, se, ie = batched.ldlt_batched(offA, n, Ae, offT, max_n=24)
+    batched.ldlt_solve_batched(offA, n, Ae, offT, Te, offT, be, max_n=24)
+    torch.cuda.synchronize()
+    assert (ie == 0).all() and (se == 0).all()
+    # captured: every buffer allocated before the capture
+    A, b = A0.clone(), b0.clone()
+    T = torch.zeros_like(Te)
+    sign = torch.empty_like(se)
+    info = torch.empty_like(ie)
+
+    def step():
+        h = handle()
+        check("eamd_dldlt_batched",
+              lib().eamd_dldlt_batched(h, count, _p(offA), _p(n), 24, _p(A), _p(offT), _p(T),
+                                       _p(sign), _p(info)))
+        check("eamd_dldlt_solve_batched",
+              lib().eamd_dldlt_solve_batched(h, count, _p(offA), _p(n), 24, _p(A), _p(offT),
+                                             _p(T), _p(offT), _p(b)))
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()  # bind the handle to this stream outside the capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        step()
+    for rep in range(3):
+        A.copy_(A0)
+        b.copy_(b0)
+        info.fill_(7)
+        junk = torch.randn(1000, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(A, Ae) and torch.equal(T, Te), f"replay {rep}"
+        assert torch.equal(b, be) and torch.equal(info, ie) and torch.equal(sign, se)
+        del junk
