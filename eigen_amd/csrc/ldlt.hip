@@ -116,17 +116,13 @@ __global__ __launch_bounds__(64) void ldlt_batch_kernel(int64_t count,
       v = fabs(M[ci + ci * ld]);
       if (v != v) v = sl == 0 ? __builtin_inf() : -1.0;
     }
-    int idx = ci;
+    // the group's largest value by a butterfly max, then the lowest lane
+    // holding it from one ballot (invalid lanes hold -1, below any |a_ii|)
+    double vmax = v;
 #pragma unroll
-    for (int off = W / 2; off >= 1; off >>= 1) {
-      const double ov = __shfl_xor(v, off);
-      const int oi = __shfl_xor(idx, off);
-      if (ov > v || (ov == v && oi < idx)) {
-        v = ov;
-        idx = oi;
-      }
-    }
-    const int p = idx;
+    for (int off = W / 2; off >= 1; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off));
+    const unsigned long long hit = __ballot(v == vmax) & segmask;
+    const int p = k + (hit ? __builtin_ctzll(hit) - sub * W : 0);
     if (on && sl == 0) tr[k] = p;
     if (on && p != k) {
       // the lower-only exchange (LDLT.h:321-337); the four sets of
@@ -161,7 +157,8 @@ __global__ __launch_bounds__(64) void ldlt_batch_kernel(int64_t count,
       __syncthreads();
       if (on && ci < nl) {
         double r = 0.0;
-        for (int j = 0; j < k; ++j) r += M[ci + j * ld] * temp[j];
+#pragma unroll 4
+        for (int j = 0; j < k; ++j) r += M[ci + j * ld] * temp[j];  // loads run ahead, order kept
         M[ci + k * ld] -= r;
       }
       __syncthreads();
