@@ -16,7 +16,8 @@
 // smallest power of two >= the batch's largest n; 64 / W systems per
 // wavefront): the lower triangle is staged in LDS (column stride n + 1),
 // lane i of the group owns row i, so the pivot search is one butterfly
-// arg-max inside the group, the exchange is a set of disjoint element swaps
+// max inside the group plus a ballot for the first lane holding it, the
+// exchange is a set of disjoint element swaps
 // done by all lanes at once, and the column update is one length-k dot
 // product per lane against temp held in LDS.  The solve uses the same
 // groups with x_i in lane i: the transpositions and the substitutions
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(64) void ldlt_batch_kernel(int64_t count,
   bool ret = true, found_zero = false, stop = false, zero_exit = false;
   for (int k = 0; k < nmax; ++k) {
     const bool on = !stop && k < nl;  // the same for the W lanes of a system
-    // ---- pivot: (|a_ii|, i) arg-max over the system's lanes, lower index on ties
+    // ---- pivot: the largest |a_ii| over the system's lanes, lower index on ties
     const int ci = k + sl;
     double v = -1.0;
     if (on && ci < nl) {
