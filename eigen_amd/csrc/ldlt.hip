@@ -17,12 +17,12 @@
 // wavefront): the lower triangle is staged in LDS (column stride n + 1),
 // lane i of the group owns row i, so the pivot search is one butterfly
 // max inside the group plus a ballot for the first lane holding it, the
-// exchange is a set of disjoint element swaps
-// done by all lanes at once, and the column update is one length-k dot
-// product per lane against temp held in LDS.  The solve uses the same
-// groups with x_i in lane i: the transpositions and the substitutions
-// exchange values by lane shuffles inside the group, and L is read straight
-// from HBM (column k for the forward sweep, row i for the backward one).
+// exchange is a set of disjoint element swaps done by all lanes at once,
+// and the column update is one length-k dot product per lane against temp
+// held in LDS.  The solve uses the same groups with x_i in lane i: the
+// transpositions and the substitutions exchange values by lane shuffles
+// inside the group, and each lane's operands are loaded before the sweeps
+// (W <= 32: row i and column i of L in registers).
 // No host synchronisation: both launches can be captured in a HIP graph.
 #include "../../include/eigen_amd.h"
 #include "common.h"
