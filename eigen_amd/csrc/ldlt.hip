@@ -19,8 +19,9 @@
 // max inside the group plus a ballot for the first lane holding it, the
 // exchange is a set of disjoint element swaps done by all lanes at once,
 // and the column update is one length-k dot product per lane against temp
-// held in LDS.  The solve uses the same groups with x_i in lane i: the
-// transpositions and the substitutions exchange values by lane shuffles
+// held in LDS.  The solve uses the same groups with x_i in lane i: each
+// permutation is one gather (every lane follows its position through the
+// transpositions), the substitutions exchange values by lane shuffles
 // inside the group, and each lane's operands are loaded before the sweeps
 // (W <= 32: row i and column i of L in registers).
 // No host synchronisation: both launches can be captured in a HIP graph.
@@ -251,15 +252,30 @@ __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
       lcol[k] = (k > sl && k < n) ? a[k + (int64_t)sl * n] : 0.0;   // L(k, i)
     }
   }
-  // x = P b: the transpositions in order
-  for (int k = 0; k < n; ++k) {
-    const int p = __shfl(trv, base + k);
-    if (p != k) {
-      const double xk = __shfl(x, base + k), xq = __shfl(x, base + p);
-      if (sl == k) x = xq;
-      if (sl == p) x = xk;
+  // P and P^T as gathers: every lane follows its own position through the
+  // transpositions (all W of them fetched by independent shuffles), so each
+  // permutation costs one shuffle of x instead of a chain of swaps
+  int pk[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) pk[k] = __shfl(trv, base + k);
+  int src_p = sl, src_pt = sl;  // x (P b)_i = b_src_p; (P^T y)_i = y_src_pt
+#pragma unroll
+  for (int k = W - 1; k >= 0; --k)
+    if (k < n) {
+      if (src_p == k)
+        src_p = pk[k];
+      else if (src_p == pk[k])
+        src_p = k;
     }
-  }
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    if (k < n) {
+      if (src_pt == k)
+        src_pt = pk[k];
+      else if (src_pt == pk[k])
+        src_pt = k;
+    }
+  x = __shfl(x, base + src_p);
   // L y = x (unit lower), column k of L per step
   if constexpr (kRegs) {
 #pragma unroll
@@ -290,15 +306,8 @@ __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
       if (sl < i) x -= a[i + (int64_t)sl * n] * xi;
     }
   }
-  // x = P^T x: the transpositions in reverse order
-  for (int k = n - 1; k >= 0; --k) {
-    const int p = __shfl(trv, base + k);
-    if (p != k) {
-      const double xk = __shfl(x, base + k), xq = __shfl(x, base + p);
-      if (sl == k) x = xq;
-      if (sl == p) x = xk;
-    }
-  }
+  // x = P^T x
+  x = __shfl(x, base + src_pt);
   if (sl < n) xp[sl] = x;
 }
 
