@@ -334,13 +334,13 @@ int laswp_trsm_rowmajor(hipStream_t s, const float* L, int64_t ldl, int nb, floa
   if (!attr_set) {
     EAMD_HIP(hipFuncSetAttribute((const void*)laswp_trsm_kernel<32, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    EAMD_HIP(hipFuncSetAttribute((const void*)laswp_trsm_kernel<8, true>,
+    EAMD_HIP(hipFuncSetAttribute((const void*)laswp_trsm_kernel<4, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attr_set = true;
   }
   if (compact)
-    hipLaunchKernelGGL((laswp_trsm_kernel<8, true>), dim3(ceil_div(ncols, 8)), dim3(256),
-                       lds_of(8, true), s, L, ldl, nb, X, ldx, ncols, perm_n, pos, src);
+    hipLaunchKernelGGL((laswp_trsm_kernel<4, true>), dim3(ceil_div(ncols, 4)), dim3(256),
+                       lds_of(4, true), s, L, ldl, nb, X, ldx, ncols, perm_n, pos, src);
   else
     hipLaunchKernelGGL((laswp_trsm_kernel<32, false>), dim3(ceil_div(ncols, 32)), dim3(256),
                        lds_of(32, false), s, L, ldl, nb, X, ldx, ncols, perm_n, pos, src);
