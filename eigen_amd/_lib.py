@@ -81,6 +81,8 @@ _SIGS = {
     "eamd_host_dpotrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, c_int64], c_int),
     "eamd_host_dgemm": ([vp, c_char, c_char, c_int64, c_int64, c_int64, c_dbl, vp, c_int64, vp,
                          c_int64, c_dbl, vp, c_int64], c_int),
+    "eamd_host_dsyrk": ([vp, c_char, c_char, c_int64, c_int64, c_dbl, vp, c_int64, c_dbl, vp,
+                         c_int64], c_int),
     "eamd_host_dtrsm": ([vp, c_char, c_char, c_char, c_char, c_int64, c_int64, c_dbl, vp,
                          c_int64, vp, c_int64], c_int),
     "eamd_host_dgesv_ir": ([vp, c_int64, c_int64, vp, c_int64, vp, c_int64, vp, c_int64,

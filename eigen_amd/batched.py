@@ -112,7 +112,8 @@ def ldlt_batched(offA, n, A, offT=None, max_n=None):
         max_n = int(n.max()) if count else 0
     if offT is None:
         offT = (torch.cumsum(n.to(torch.int64), 0) - n.to(torch.int64)).contiguous()
-    total = int(offT[-1] + n[-1].clamp(min=0)) if count else 0
+    # the furthest end of any system's slots (offT need not be ascending)
+    total = int((offT + n.to(torch.int64).clamp(min=0)).max()) if count else 0
     T = torch.zeros(max(total, 1), dtype=torch.int32, device=dev)
     sign = torch.empty(count, dtype=torch.int32, device=dev)
     info = torch.empty(count, dtype=torch.int32, device=dev)

@@ -429,25 +429,30 @@ int eamd_dgetrf(eamd_handle_t h, int64_t m, int64_t n, double* A, int64_t lda, i
   if (nb_transp) *nb_transp = 0;
   if (first_zero_pivot) *first_zero_pivot = -1;
   if (m == 0 || n == 0) return kOk;
-  if (m != n) {
-    set_error("dgetrf: only square matrices are supported (PartialPivLU is square-only)");
-    return kErrUnsupported;
-  }
-  if (n > 1000000) return kErrUnsupported;
+  if (m > 1000000 || n > 1000000) return kErrUnsupported;
   hipStream_t s = h->stream;
-  const int N = (int)n;
-  const int64_t ldw = ((n + 15) / 16) * 16;
+  const int64_t size = std::min(m, n);
+  // A wide matrix whose min(m, n) exceeds the reference's unblocked limit of
+  // 16 is factored by its blocked loop over the first m columns only: the
+  // trailing size there is min(m, n) - k - bs (PartialPivLU.h:450-452,
+  // 464-465), so columns m..n-1 keep their input values, neither permuted
+  // nor solved.  The same columns are left alone here; up to 16 the
+  // unblocked loop (PartialPivLU.h:365-409) updates every column, and so
+  // does this driver.
+  const int64_t ncols = (m < n && size > 16) ? m : n;
+  const int M = (int)m, NC = (int)ncols;
+  const int64_t ldw = ((ncols + 15) / 16) * 16;
   void* wp = nullptr;
-  EAMD_CHECK(h->ws.get(kWsWork, sizeof(double) * (size_t)ldw * n, &wp));
+  EAMD_CHECK(h->ws.get(kWsWork, sizeof(double) * (size_t)ldw * m, &wp));
   double* W = static_cast<double*>(wp);
-  // W (row-major) = A
-  EAMD_CHECK(transpose_copy<double, double>(s, A, lda, N, N, W, ldw));
+  // W (row-major, m x ncols) = A
+  EAMD_CHECK(transpose_copy<double, double>(s, A, lda, M, NC, W, ldw));
   int64_t fz = -1;
-  EAMD_CHECK(dgetrf_rowmajor(h, W, ldw, N, transp, &fz, lu_block_width(n)));
-  EAMD_CHECK(transpose_copy<double, double>(s, W, ldw, N, N, A, lda));
+  EAMD_CHECK(dgetrf_rowmajor(h, W, ldw, M, NC, transp, &fz, lu_block_width(size)));
+  EAMD_CHECK(transpose_copy<double, double>(s, W, ldw, NC, M, A, lda));
   void* cnt = nullptr;
   EAMD_CHECK(h->ws.get(kWsAux2, 256, &cnt));
-  hipLaunchKernelGGL(count_nontrivial_kernel, dim3(1), dim3(1024), 0, s, transp, N,
+  hipLaunchKernelGGL(count_nontrivial_kernel, dim3(1), dim3(1024), 0, s, transp, (int)size,
                      static_cast<int*>(cnt));
   int hc = 0;
   EAMD_HIP(hipMemcpyAsync(&hc, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -899,6 +904,30 @@ int eamd_host_dgemm(eamd_handle_t h, char transa, char transb, int64_t m, int64_
   EAMD_CHECK(eamd_dgemm(h, transa, transb, m, n, k, alpha, (double*)dA.p, std::max<int64_t>(1, ar),
                         (double*)dB.p, std::max<int64_t>(1, br), beta, (double*)dC.p, m));
   EAMD_CHECK(down2d(h, C, ldc, (double*)dC.p, m, m, n));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_host_dsyrk(eamd_handle_t h, char uplo, char trans, int64_t n, int64_t k, double alpha,
+                    const double* A, int64_t lda, double beta, double* C, int64_t ldc) {
+  if (!is_uplo(uplo)) return -1;
+  if (!is_op(trans)) return -2;
+  if (n < 0) return -3;
+  if (k < 0) return -4;
+  const bool t = is_trans(trans);
+  const int64_t ar = t ? k : n, ac = t ? n : k;
+  if (lda < std::max<int64_t>(1, ar)) return -7;
+  if (ldc < std::max<int64_t>(1, n)) return -10;
+  CHECK_HANDLE(h);
+  if (n == 0) return kOk;
+  DevBuf dA, dC;
+  EAMD_CHECK(dA.alloc(sizeof(double) * (size_t)std::max<int64_t>(1, ar * ac)));
+  EAMD_CHECK(dC.alloc(sizeof(double) * (size_t)n * n));
+  if (ar * ac) EAMD_CHECK(up2d(h, (double*)dA.p, ar, A, lda, ar, ac));
+  EAMD_CHECK(up2d(h, (double*)dC.p, n, C, ldc, n, n));
+  EAMD_CHECK(eamd_dsyrk(h, uplo, trans, n, k, alpha, (double*)dA.p, std::max<int64_t>(1, ar),
+                        beta, (double*)dC.p, n));
+  EAMD_CHECK(down2d(h, C, ldc, (double*)dC.p, n, n, n));
   EAMD_HIP(hipStreamSynchronize(h->stream));
   return kOk;
 }

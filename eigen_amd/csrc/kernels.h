@@ -70,9 +70,10 @@ template <typename Ti, typename To>
 int transpose_copy(hipStream_t s, const Ti* A, int64_t lda, int rows, int cols, To* B,
                    int64_t ldb);
 
-// FP64 LU driver on a row-major working matrix (n x n, ld ldw), pivots
-// (0-based transpositions, absolute) into piv_dev.  first_zero on the host.
-int dgetrf_rowmajor(eamd_handle* h, double* W, int64_t ldw, int n, int* piv_dev,
+// FP64 LU driver on a row-major working matrix (m x n, ld ldw), min(m, n)
+// pivots (0-based transpositions, absolute) into piv_dev.  first_zero on the
+// host.
+int dgetrf_rowmajor(eamd_handle* h, double* W, int64_t ldw, int m, int n, int* piv_dev,
                     int64_t* first_zero, int nb);
 int sgetrf_rowmajor(eamd_handle* h, float* W, int64_t ldw, int n, int* piv_dev,
                     int64_t* first_zero, int nb, int update_precision);

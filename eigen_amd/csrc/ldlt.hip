@@ -221,11 +221,12 @@ __global__ __launch_bounds__(64) void ldlt_batch_kernel(int64_t count,
 
 // x <- A^{-1} b for each system from ldlt_batch_kernel's factors; W lanes
 // per system (x_i in lane i of its group), 64 / W systems per wavefront,
-// four wavefronts per workgroup.  Systems of a size outside 0..W are left as
-// they are (their factorization reported -2).
+// four wavefronts per workgroup.  Systems of a size outside 0..max_n are
+// left as they are (their factorization reported -2 and wrote no
+// transpositions).
 template <int W>
 __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
-    int64_t count, const int64_t* __restrict__ offA, const int32_t* __restrict__ nv,
+    int64_t count, const int64_t* __restrict__ offA, const int32_t* __restrict__ nv, int max_n,
     const double* __restrict__ Aall, const int64_t* __restrict__ offT,
     const int32_t* __restrict__ Tall, const int64_t* __restrict__ offb, double* __restrict__ B) {
   constexpr int G = 64 / W;
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
   const int64_t s = (((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6) * G + sub;
   if (s >= count) return;  // whole groups: every shuffle below stays in the group
   const int n = nv[s];
-  if (n <= 0 || n > W) return;
+  if (n <= 0 || n > max_n || n > W) return;
   const double* a = Aall + offA[s];
   double* xp = B + offb[s];
   // everything a lane needs, loaded up front so the loads overlap: its
@@ -360,14 +361,14 @@ int ldlt_solve_batched(eamd_handle* h, int64_t count, const int64_t* offA, const
   }
   const dim3 grid((unsigned)blocks), block(256);
   if (W == 16)
-    hipLaunchKernelGGL(ldlt_solve_batch_kernel<16>, grid, block, 0, h->stream, count, offA, n, A,
-                       offT, T, offb, b);
+    hipLaunchKernelGGL(ldlt_solve_batch_kernel<16>, grid, block, 0, h->stream, count, offA, n,
+                       max_n, A, offT, T, offb, b);
   else if (W == 32)
-    hipLaunchKernelGGL(ldlt_solve_batch_kernel<32>, grid, block, 0, h->stream, count, offA, n, A,
-                       offT, T, offb, b);
+    hipLaunchKernelGGL(ldlt_solve_batch_kernel<32>, grid, block, 0, h->stream, count, offA, n,
+                       max_n, A, offT, T, offb, b);
   else
-    hipLaunchKernelGGL(ldlt_solve_batch_kernel<64>, grid, block, 0, h->stream, count, offA, n, A,
-                       offT, T, offb, b);
+    hipLaunchKernelGGL(ldlt_solve_batch_kernel<64>, grid, block, 0, h->stream, count, offA, n,
+                       max_n, A, offT, T, offb, b);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }

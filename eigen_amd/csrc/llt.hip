@@ -296,6 +296,9 @@ __global__ __launch_bounds__(256) void llt_rank_update_kernel(
         r = q * u;
       }
       if (lane == jl) at(j, j) = r;
+      // the identity rotation leaves the column and the work vector as they
+      // are (Jacobi.h:460-461), also where they hold non-finite values
+      if (c == 1.0 && sn == 0.0) continue;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const int i = lane + 64 * k;

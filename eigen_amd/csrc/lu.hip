@@ -467,12 +467,13 @@ int finish_getrf(eamd_handle* h, const int* fz_dev, int64_t* first_zero) {
   return kOk;
 }
 
-// Row-major blocked LU driver.  `update(A12, A21, A22, mrows, ncols, k,
-// within_panel)` performs A22 -= A21 * A12 (row-major, ld ldw).  ord
-// (nullable, device, n ints) receives the final row order: row i of the
-// factors is row ord[i] of the input, i.e. (P b)[i] = b[ord[i]].
+// Row-major blocked LU driver of an mr x n matrix (min(mr, n) pivots).
+// `update(A12, A21, A22, mrows, ncols, k, within_panel)` performs
+// A22 -= A21 * A12 (row-major, ld ldw).  ord (nullable, device, mr ints)
+// receives the final row order: row i of the factors is row ord[i] of the
+// input, i.e. (P b)[i] = b[ord[i]].
 template <typename T, typename Update>
-int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int* ord,
+int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int mr, int n, int* piv_dev, int* ord,
                      int64_t* first_zero, int nb, Update&& update) {
   hipStream_t s = h->stream;
   if (nb > 256) nb = 256;
@@ -492,10 +493,11 @@ int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int
   int* ppos = lsrc + 2 * kLeaf;
   int* psrc = ppos + 2 * nb;
   EAMD_HIP(hipMemsetAsync(fz_dev, 0xff, sizeof(int), s));  // -1
-  if (ord) EAMD_CHECK(init_row_order(s, ord, n));
-  for (int k = 0; k < n; k += nb) {
-    const int bs = min(nb, n - k);
-    const int m = n - k;
+  if (ord) EAMD_CHECK(init_row_order(s, ord, mr));
+  const int size = min(mr, n);
+  for (int k = 0; k < size; k += nb) {
+    const int bs = min(nb, size - k);
+    const int m = mr - k;
     T* Pk = W + (int64_t)k * ldw + k;
     EAMD_CHECK(factor_panel<T>(h, W, ldw, k, bs, m, piv_dev, lperm_n, lpos, lsrc, pperm_n, ppos,
                                psrc, zero_col, fz_dev, update));
@@ -516,7 +518,7 @@ int getrf_rowmajor_t(eamd_handle* h, T* W, int64_t ldw, int n, int* piv_dev, int
       }
       T* A21 = W + (int64_t)(k + bs) * ldw + k;
       T* A22 = W + (int64_t)(k + bs) * ldw + (k + bs);
-      EAMD_CHECK(update(A12, A21, A22, m - bs, tn, bs, false));
+      if (m - bs > 0) EAMD_CHECK(update(A12, A21, A22, m - bs, tn, bs, false));
     }
   }
   return finish_getrf(h, fz_dev, first_zero);
@@ -673,13 +675,13 @@ int dgemm_rowmajor_update(eamd_handle* h, int64_t ldw, double* A12, double* A21,
 }
 }  // namespace
 
-int dgetrf_rowmajor(eamd_handle* h, double* W, int64_t ldw, int n, int* piv_dev,
+int dgetrf_rowmajor(eamd_handle* h, double* W, int64_t ldw, int m, int n, int* piv_dev,
                     int64_t* first_zero, int nb) {
   auto upd = [&](double* A12, double* A21, double* A22, int mrows, int ncols, int k,
                  bool within) -> int {
     return dgemm_rowmajor_update(h, ldw, A12, A21, A22, mrows, ncols, k, within);
   };
-  return getrf_rowmajor_t<double>(h, W, ldw, n, piv_dev, nullptr, first_zero, nb, upd);
+  return getrf_rowmajor_t<double>(h, W, ldw, m, n, piv_dev, nullptr, first_zero, nb, upd);
 }
 
 int sgetrf_rowmajor(eamd_handle* h, float* W, int64_t ldw, int n, int* piv_dev,
@@ -690,7 +692,7 @@ int sgetrf_rowmajor(eamd_handle* h, float* W, int64_t ldw, int n, int* piv_dev,
     return sgemm_update_rowmajor(h, A12, A21, A22, ldw, mrows, ncols, k,
                                  within ? kPrecPanelFp32 : update_precision);
   };
-  return getrf_rowmajor_t<float>(h, W, ldw, n, piv_dev, nullptr, first_zero, nb, upd);
+  return getrf_rowmajor_t<float>(h, W, ldw, n, n, piv_dev, nullptr, first_zero, nb, upd);
 }
 
 // The refinement solver's factorization: FP64 (plain) or FP32 with the
@@ -703,7 +705,7 @@ int getrf_rowmajor_order(eamd_handle* h, void* W, bool fp64, int64_t ldw, int n,
                    bool within) -> int {
       return dgemm_rowmajor_update(h, ldw, A12, A21, A22, mrows, ncols, k, within);
     };
-    return getrf_rowmajor_t<double>(h, Wd, ldw, n, piv, ord, first_zero, nb, upd);
+    return getrf_rowmajor_t<double>(h, Wd, ldw, n, n, piv, ord, first_zero, nb, upd);
   }
   float* Wf = static_cast<float*>(W);
   static const bool lookahead = env_int("EAMD_LOOKAHEAD", 1) != 0;
@@ -715,7 +717,7 @@ int getrf_rowmajor_order(eamd_handle* h, void* W, bool fp64, int64_t ldw, int n,
     return sgemm_update_rowmajor(h, A12, A21, A22, ldw, mrows, ncols, k,
                                  within ? kPrecPanelFp32 : precision);
   };
-  return getrf_rowmajor_t<float>(h, Wf, ldw, n, piv, ord, first_zero, nb, upd);
+  return getrf_rowmajor_t<float>(h, Wf, ldw, n, n, piv, ord, first_zero, nb, upd);
 }
 
 }  // namespace eamd

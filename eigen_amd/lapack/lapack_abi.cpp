@@ -1,6 +1,7 @@
-// lapack_abi.cpp -- dgetrf_/dgetrs_/dpotrf_/dpotrs_/dgemm_/dtrsm_ over the
-// eigen_amd host-buffer entry points.  Argument checks and info codes follow
-// lapack/lu.cpp:14-89, lapack/cholesky.cpp:14-72 and blas/level3_impl.h:12-179;
+// lapack_abi.cpp -- dgetrf_/dgetrs_/dpotrf_/dpotrs_/dgemm_/dtrsm_/dsyrk_ over
+// the eigen_amd host-buffer entry points.  Argument checks and info codes
+// follow lapack/lu.cpp:14-89, lapack/cholesky.cpp:14-72 and
+// blas/level3_impl.h:12-179, 357-391;
 // each host thread gets its own handle (handles are thread-confined).
 #include "../../include/eigen_amd_lapack.h"
 
@@ -164,6 +165,27 @@ int dtrsm_(const char* side, const char* uplo, const char* opa, const char* diag
   const int st = eamd_host_dtrsm(h, sd, ul, ta == 'N' ? 'N' : 'T', dg, *m, *n, *alpha, a, *lda,
                                  b, *ldb);
   if (st != 0) return device_failure("DTRSM", st, nullptr);
+  return 0;
+}
+
+int dsyrk_(const char* uplo, const char* op, const int* n, const int* k, const double* alpha,
+           const double* a, const int* lda, const double* beta, double* c, const int* ldc) {
+  int info = 0;
+  const char ul = up(*uplo), ta = up(*op);
+  if (!is_uplo(ul)) info = 1;
+  else if (!is_trans(ta)) info = 2;
+  else if (*n < 0) info = 3;
+  else if (*k < 0) info = 4;
+  else if (*lda < std::max(1, ta == 'N' ? *n : *k)) info = 7;
+  else if (*ldc < std::max(1, *n)) info = 10;
+  if (info) return illegal("DSYRK ", info, nullptr);
+  if (*n == 0) return 0;
+  eamd_handle_t h = handle();
+  if (!h) return device_failure("DSYRK", -1000, nullptr);
+  // k == 0 still scales the triangle by beta (blas/level3_impl.h:381-389)
+  const int st = eamd_host_dsyrk(h, ul, ta == 'N' ? 'N' : 'T', *n, *k, *alpha, a, *lda, *beta, c,
+                                 *ldc);
+  if (st != 0) return device_failure("DSYRK", st, nullptr);
   return 0;
 }
 

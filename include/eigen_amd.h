@@ -97,8 +97,10 @@ int eamd_dsyrk(eamd_handle_t h, char uplo, char trans, int64_t n, int64_t k, dou
                const double* A, int64_t lda, double beta, double* C, int64_t ldc);
 
 /* ------------------------------------------------------------------- LU -- */
-/* In-place LU with partial pivoting of an m x n matrix (m >= n, or the
- * reference's square use).  transp[min(m,n)] receives 0-based row
+/* In-place LU with partial pivoting of an m x n matrix.  As in the
+ * reference's blocked loop, a wide matrix (m < n) with min(m, n) > 16 has
+ * only its first m columns factored; columns m..n-1 keep their values
+ * (PartialPivLU.h:450-465).  transp[min(m,n)] receives 0-based row
  * transpositions (device int32), *nb_transp (host) their non-trivial count
  * and *first_zero_pivot (host) the first exactly-zero pivot or -1; the
  * factorization continues past it, as the reference does.
@@ -276,7 +278,9 @@ int eamd_dgesv_ir_batched_planned(eamd_handle_t h, eamd_batched_plan_t plan,
 /* Same operations on host (pageable) buffers: upload, compute, download.
  * These back the Eigen plug-point specialisations, whose arguments are host
  * pointers (PartialPivLU_LAPACKE.h:42-77, LLT_LAPACKE.h:42-93,
- * GeneralMatrixMatrix_BLAS.h:49-116, TriangularSolverMatrix_BLAS.h:41-157). */
+ * GeneralMatrixMatrix_BLAS.h:49-116, TriangularSolverMatrix_BLAS.h:41-157,
+ * GeneralMatrixMatrixTriangular_BLAS.h:79-98 for eamd_host_dsyrk), and the
+ * Fortran ABI of eigen_amd_lapack.h. */
 int eamd_host_dgetrf(eamd_handle_t h, int64_t m, int64_t n, double* A, int64_t lda,
                      int32_t* transp, int32_t* nb_transp, int64_t* first_zero_pivot);
 int eamd_host_dgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const double* LU,
@@ -288,6 +292,8 @@ int eamd_host_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const 
 int eamd_host_dgemm(eamd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
                     double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                     double beta, double* C, int64_t ldc);
+int eamd_host_dsyrk(eamd_handle_t h, char uplo, char trans, int64_t n, int64_t k, double alpha,
+                    const double* A, int64_t lda, double beta, double* C, int64_t ldc);
 int eamd_host_dtrsm(eamd_handle_t h, char side, char uplo, char transa, char diag, int64_t m,
                     int64_t n, double alpha, const double* A, int64_t lda, double* B,
                     int64_t ldb);

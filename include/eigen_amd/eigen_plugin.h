@@ -15,17 +15,35 @@
  *                                 (GeneralMatrixMatrix.h:49-199; as
  *                                 GeneralMatrixMatrix_BLAS.h:49-116), when
  *                                 EIGEN_AMD_USE_GEMM is defined
+ *   S.selfadjointView<Lower|Upper>().rankUpdate(A), and C.triangularView<..>() += A * A^T
+ *                              -> general_matrix_matrix_triangular_product<..., double, ...,
+ *                                 ColMajor, UpLo, Specialized>::run
+ *                                 (GeneralMatrixMatrixTriangular.h:56-125; as
+ *                                 GeneralMatrixMatrixTriangular_BLAS.h:47-98:
+ *                                 SYRK when both operands are the same array,
+ *                                 Eigen's built-in kernel for other triangular
+ *                                 products, as there), when EIGEN_AMD_USE_GEMM
+ *                                 is defined
  *   T.solveInPlace(X), matrix X -> triangular_solve_matrix<double, ..., ColMajor>::run
  *                                 (TriangularSolverMatrix.h:38-329; as
  *                                 TriangularSolverMatrix_BLAS.h:41-157), when
  *                                 EIGEN_AMD_USE_TRSM is defined
+ *   T.solveInPlace(x), vector x -> triangular_solve_vector<double, double, ..., OnTheLeft, ...>::run
+ *                                 (TriangularSolverVector.h:88-139 / 30-85; the
+ *                                 OnTheRight form maps onto these, :17-26), so
+ *                                 PartialPivLU / LLT solve(VectorXd) runs on the
+ *                                 GPU too, when EIGEN_AMD_USE_TRSM is defined
  *
  * Buffers are Eigen's host buffers: each call uploads, computes and
  * downloads through the eamd_host_* entry points on a per-thread handle
- * (handles are thread-confined, like Eigen objects).  A failure of the GPU
- * path throws std::runtime_error with the library's message: there is no
- * silent CPU fallback.  Rank updates of an existing LLT keep Eigen's own
- * algorithm (llt_rank_update_lower, LLT.h:231-299). */
+ * (handles are thread-confined, like Eigen objects).  The decompositions
+ * hold their factors in host memory (m_lu, m_matrix), so each triangular
+ * solve uploads the factor it solves with; callers that solve many times
+ * with one factor keep it on the device through the C ABI (eamd_dgetrf /
+ * eamd_dgetrs) instead.  A failure of the GPU path throws std::runtime_error
+ * with the library's message: there is no silent CPU fallback.  Rank updates
+ * of an existing LLT keep Eigen's own algorithm (llt_rank_update_lower,
+ * LLT.h:231-299). */
 #ifndef EIGEN_AMD_EIGEN_PLUGIN_H
 #define EIGEN_AMD_EIGEN_PLUGIN_H
 
@@ -52,6 +70,17 @@ inline eamd_handle_t handle() {
   thread_local Owner owner;
   if (!owner.h) throw std::runtime_error(std::string("eigen_amd: ") + eamd_last_error());
   return owner.h;
+}
+
+/* How many times each plug point reached the C ABI in this process (for
+ * callers and tests that want to see which of Eigen's calls ran on the
+ * GPU). */
+struct CallCounts {
+  long getrf = 0, potrf = 0, gemm = 0, syrk = 0, trsm = 0, trsv = 0;
+};
+inline CallCounts& call_counts() {
+  static CallCounts c;
+  return c;
 }
 
 inline void check(int status, const char* what) {
@@ -91,6 +120,13 @@ struct partial_lu_impl<double, StorageOrder, int> {
     EIGEN_UNUSED_VARIABLE(maxBlockSize);
     int32_t nbt = 0;
     int64_t first_zero = -1;
+    /* an empty matrix has no pivots; its stride may be 0, which the C ABI's
+     * leading-dimension check would refuse */
+    if (rows == 0 || cols == 0) {
+      nb_transpositions = 0;
+      return -1;
+    }
+    ++eigen_amd_plugin::call_counts().getrf;
     if (StorageOrder == ColMajor) {
       eigen_amd_plugin::check(
           eamd_host_dgetrf(eigen_amd_plugin::handle(), rows, cols, lu_data, luStride,
@@ -118,7 +154,9 @@ struct eigen_amd_llt {
     eigen_assert(m.rows() == m.cols());
     const Index n = m.rows();
     int64_t fail = -1;
+    if (n == 0) return -1; /* as the unblocked loop over no columns (LLT.h:304-328) */
     const char uplo = UpLo == Lower ? 'L' : 'U';
+    ++eigen_amd_plugin::call_counts().potrf;
     if (!(MatrixType::Flags & RowMajorBit)) {
       eigen_amd_plugin::check(eamd_host_dpotrf(eigen_amd_plugin::handle(), uplo, n,
                                                &m.coeffRef(0, 0), m.outerStride(), &fail),
@@ -175,6 +213,7 @@ struct general_matrix_matrix_product<Index_, double, LhsStorageOrder, ConjugateL
                   GemmParallelInfo<Index_>* /*info*/ = 0) {
     const char ta = LhsStorageOrder == RowMajor ? 'T' : 'N';
     const char tb = RhsStorageOrder == RowMajor ? 'T' : 'N';
+    ++eigen_amd_plugin::call_counts().gemm;
     eigen_amd_plugin::check(eamd_host_dgemm(eigen_amd_plugin::handle(), ta, tb, rows, cols,
                                             depth, alpha, lhs, lhsStride, rhs, rhsStride, 1.0,
                                             res, resStride),
@@ -183,7 +222,73 @@ struct general_matrix_matrix_product<Index_, double, LhsStorageOrder, ConjugateL
 };
 #endif
 
+#ifdef EIGEN_AMD_USE_GEMM
+/* res(UpLo) += alpha * lhs * rhs.  With lhs and rhs the same array in
+ * opposite storage orders the product is op(A) op(A)^T, a SYRK; any other
+ * triangular product keeps Eigen's built-in kernel, as the reference's BLAS
+ * binding does (GeneralMatrixMatrixTriangular_BLAS.h:53-64). */
+template <typename Index_, int LhsStorageOrder, bool ConjugateLhs, int RhsStorageOrder,
+          bool ConjugateRhs, int UpLo>
+struct general_matrix_matrix_triangular_product<Index_, double, LhsStorageOrder, ConjugateLhs,
+                                                double, RhsStorageOrder, ConjugateRhs, ColMajor,
+                                                UpLo, Specialized> {
+  static void run(Index_ size, Index_ depth, const double* lhs, Index_ lhsStride,
+                  const double* rhs, Index_ rhsStride, double* res, Index_ resStride,
+                  const double& alpha, level3_blocking<double, double>& blocking) {
+    if (lhs == rhs && lhsStride == rhsStride && LhsStorageOrder != RhsStorageOrder) {
+      if (size == 0) return;
+      /* a column-major lhs is A (size x depth): A A^T; a row-major lhs is the
+       * column-major array A (depth x size) read transposed: A^T A */
+      const char uplo = (UpLo & Lower) == Lower ? 'L' : 'U';
+      const char trans = LhsStorageOrder == RowMajor ? 'T' : 'N';
+      ++eigen_amd_plugin::call_counts().syrk;
+      eigen_amd_plugin::check(eamd_host_dsyrk(eigen_amd_plugin::handle(), uplo, trans, size,
+                                              depth, alpha, lhs, lhsStride, 1.0, res,
+                                              resStride),
+                              "eamd_host_dsyrk");
+    } else {
+      general_matrix_matrix_triangular_product<Index_, double, LhsStorageOrder, ConjugateLhs,
+                                               double, RhsStorageOrder, ConjugateRhs, ColMajor,
+                                               UpLo, BuiltIn>::run(size, depth, lhs, lhsStride,
+                                                                   rhs, rhsStride, res,
+                                                                   resStride, alpha, blocking);
+    }
+  }
+};
+#endif
+
 #ifdef EIGEN_AMD_USE_TRSM
+/* rhs <- tri^-1 rhs for one right-hand side vector (contiguous). */
+template <typename Index_, int Mode, int TriStorageOrder>
+struct eigen_amd_trsv {
+  static void run(Index_ size, const double* tri, Index_ triStride, double* rhs) {
+    if (size == 0) return;
+    const bool lower = (Mode & Lower) == Lower;
+    const char diag = (Mode & UnitDiag) ? 'U' : 'N';
+    /* a row-major triangle is the transpose of the column-major array */
+    const char uplo = (TriStorageOrder == RowMajor) ? (lower ? 'U' : 'L') : (lower ? 'L' : 'U');
+    const char trans = TriStorageOrder == RowMajor ? 'T' : 'N';
+    ++eigen_amd_plugin::call_counts().trsv;
+    eigen_amd_plugin::check(eamd_host_dtrsm(eigen_amd_plugin::handle(), 'L', uplo, trans, diag,
+                                            size, 1, 1.0, tri, triStride, rhs, size),
+                            "eamd_host_dtrsm");
+  }
+};
+
+template <typename Index_, int Mode, bool Conjugate>
+struct triangular_solve_vector<double, double, Index_, OnTheLeft, Mode, Conjugate, ColMajor> {
+  static void run(Index_ size, const double* lhs, Index_ lhsStride, double* rhs) {
+    eigen_amd_trsv<Index_, Mode, ColMajor>::run(size, lhs, lhsStride, rhs);
+  }
+};
+
+template <typename Index_, int Mode, bool Conjugate>
+struct triangular_solve_vector<double, double, Index_, OnTheLeft, Mode, Conjugate, RowMajor> {
+  static void run(Index_ size, const double* lhs, Index_ lhsStride, double* rhs) {
+    eigen_amd_trsv<Index_, Mode, RowMajor>::run(size, lhs, lhsStride, rhs);
+  }
+};
+
 /* other <- tri^-1 other (left) or other tri^-1 (right); other column-major. */
 template <typename Index_, int Side, int Mode, bool Conjugate, int TriStorageOrder>
 struct eigen_amd_trsm {
@@ -197,6 +302,7 @@ struct eigen_amd_trsm {
     const char side = Side == OnTheLeft ? 'L' : 'R';
     const Index_ m = Side == OnTheLeft ? size : otherSize;
     const Index_ n = Side == OnTheLeft ? otherSize : size;
+    ++eigen_amd_plugin::call_counts().trsm;
     eigen_amd_plugin::check(eamd_host_dtrsm(eigen_amd_plugin::handle(), side, uplo, trans, diag,
                                             m, n, 1.0, tri, triStride, other, otherStride),
                             "eamd_host_dtrsm");

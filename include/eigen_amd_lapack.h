@@ -31,6 +31,11 @@ int dtrsm_(const char* side, const char* uplo, const char* opa, const char* diag
            const int* n, const double* alpha, const double* a, const int* lda, double* b,
            const int* ldb);
 
+/* C(uplo) = alpha op(A) op(A)^T + beta C(uplo), op 'N' (A n x k) or 'T'/'C'
+ * (A k x n) (blas/level3_impl.h:357-391) */
+int dsyrk_(const char* uplo, const char* op, const int* n, const int* k, const double* alpha,
+           const double* a, const int* lda, const double* beta, double* c, const int* ldc);
+
 #ifdef __cplusplus
 }
 #endif

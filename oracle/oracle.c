@@ -482,6 +482,8 @@ long orc_llt_rank_update_d(long n, double* L, long ld, const double* v, double s
         r = q * u;
       }
       L[IDX(i, i, ld)] = r;
+      /* the identity rotation is not applied (Jacobi.h:460-461) */
+      if (c == 1.0 && s == 0.0) continue;
       for (long k = i + 1; k < n; ++k) {
         const double x = L[IDX(k, i, ld)], y = t[k];
         L[IDX(k, i, ld)] = c * x + s * y;

@@ -1,10 +1,12 @@
 // eigen_plugin_check.cpp -- unmodified Eigen user code, compiled with
 // include/eigen_amd/eigen_plugin.h so that PartialPivLU<MatrixXd>,
-// LLT<MatrixXd>, C.noalias() += A*B and TriangularView::solveInPlace run
-// through the eamd C ABI on the GPU.  Writes the inputs and the results as
-// raw little-endian arrays into the directory given as argv[1]; the Python
-// test compares them with the oracle.  Exit status 0 when every call
-// returned, 1 on an exception from the GPU path.
+// LLT<MatrixXd>, C.noalias() += A*B, selfadjointView().rankUpdate(A) and
+// TriangularView::solveInPlace (matrix and vector right-hand sides) run
+// through the eamd C ABI on the GPU.  Writes the inputs, the results and the
+// plug points' call counts per section as raw little-endian arrays into the
+// directory given as argv[1]; the Python test compares them with the oracle.
+// Exit status 0 when every call returned, 1 on an exception from the GPU
+// path.
 #include <Eigen/Dense>
 
 #define EIGEN_AMD_USE_GEMM
@@ -58,6 +60,13 @@ void dump(const char* name, const T* p, size_t count) {
 }
 void dump(const char* name, const MatrixXd& m) { dump(name, m.data(), (size_t)m.size()); }
 
+// the plug points' call counts: getrf, potrf, gemm, syrk, trsm, trsv
+void dump_counts(const char* name) {
+  const auto& c = eigen_amd_plugin::call_counts();
+  const long v[6] = {c.getrf, c.potrf, c.gemm, c.syrk, c.trsm, c.trsv};
+  dump(name, v, 6);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -80,6 +89,14 @@ int main(int argc, char** argv) {
     dump("lu_x.f64", x);
     double det = lu.determinant();
     dump("lu_det.f64", &det, 1);
+    dump_counts("cnt_lu.i64");
+
+    // one right-hand side as a vector (C1's shape): triangular_solve_vector
+    VectorXd bv = normal(n, 1, 22).col(0);
+    VectorXd xv = lu.solve(bv);
+    dump("luv_b.f64", bv.data(), (size_t)n);
+    dump("luv_x.f64", xv.data(), (size_t)n);
+    dump_counts("cnt_luv.i64");
 
     // LLT<MatrixXd>, lower and upper
     MatrixXd G = normal(n, n, 13);
@@ -97,6 +114,11 @@ int main(int argc, char** argv) {
     dump("llt_U.f64", U);
     dump("llt_x.f64", xs);
     dump("llt_info.i32", info, 2);
+    VectorXd bsv = bs.col(0);
+    VectorXd xsv = llt.solve(bsv), xsu = lltu.solve(bsv);
+    dump("lltv_x.f64", xsv.data(), (size_t)n);
+    dump("lltv_xu.f64", xsu.data(), (size_t)n);
+    dump_counts("cnt_llt.i64");
 
     // a non-SPD matrix: the failing column is reported as NumericalIssue
     MatrixXd Sbad = S;
@@ -132,6 +154,27 @@ int main(int argc, char** argv) {
     dump("trsm_X20.f64", X20);
     dump("trsm_X1.f64", X1);
     dump("trsm_X2.f64", X2);
+    dump_counts("cnt_trsm.i64");
+
+    // SYRK through selfadjointView().rankUpdate: C += alpha A A^T (lower),
+    // C += alpha B^T B (upper, B read transposed)
+    MatrixXd Sa = normal(150, 70, 23), Sb = normal(60, 150, 24), Sc0 = normal(150, 150, 25);
+    MatrixXd Sl = Sc0, Su = Sc0;
+    Sl.selfadjointView<Eigen::Lower>().rankUpdate(Sa, 0.5);
+    Su.selfadjointView<Eigen::Upper>().rankUpdate(Sb.transpose(), -2.0);
+    dump("syrk_A.f64", Sa);
+    dump("syrk_B.f64", Sb);
+    dump("syrk_C0.f64", Sc0);
+    dump("syrk_L.f64", Sl);
+    dump("syrk_U.f64", Su);
+    dump_counts("cnt_syrk.i64");
+
+    // empty matrices: Success, no GPU call needed
+    MatrixXd E(0, 0);
+    Eigen::PartialPivLU<MatrixXd> elu(E);
+    Eigen::LLT<MatrixXd> ellt(E);
+    int empty_ok = ellt.info() == Eigen::Success && elu.matrixLU().size() == 0 ? 1 : 0;
+    dump("empty_ok.i32", &empty_ok, 1);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;

@@ -196,6 +196,17 @@ RANKUPD_SIZES = [1, 6, 24, 33, 100, 256]
 RANKUPD_SIGMAS = [("p", 2.0), ("m", -0.5)]
 
 
+def rankupdate_identity_case():
+    """(L, v) of the identity-rotation fixture: n = 6, v_0 = 0, v_3 = inf."""
+    n = 6
+    a = inputs.spd(7000 + n, n)
+    l0 = np.tril(np.linalg.cholesky(a))
+    v = inputs.normal(7100 + n, n)
+    v[0] = 0.0
+    v[3] = np.inf
+    return np.asfortranarray(l0), v
+
+
 def rankupdate_main():
     ref = po.ref()
     out = {}
@@ -224,6 +235,11 @@ def rankupdate_main():
     v[3] = 10.0 * np.sqrt(a[3, 3])
     _, info = po.llt_rank_update(l0, v, -1.0, reference=True)
     out["fail_info"] = np.array(info)
+    # an identity rotation (v_0 = 0 over a positive diagonal) followed by a
+    # non-finite entry in v: Eigen leaves column 0 untouched
+    f, info = po.llt_rank_update(*rankupdate_identity_case(), 2.0, reference=True)
+    out["ident_L"] = np.tril(f)
+    out["ident_info"] = np.array(info)
     np.savez_compressed(os.path.join(HERE, "rankupdate.npz"), **out)
     print("rank update fixtures written to", HERE)
 

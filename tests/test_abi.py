@@ -97,7 +97,7 @@ def _lapack():
 def test_lapack_layer_exports_every_declared_symbol():
     text = re.sub(r"/\*.*?\*/", "", open(LAPACK_H).read(), flags=re.S)
     names = sorted(set(re.findall(r"\b(d\w+_)\s*\(", text)))
-    assert names == ["dgemm_", "dgetrf_", "dgetrs_", "dpotrf_", "dpotrs_", "dtrsm_"]
+    assert names == ["dgemm_", "dgetrf_", "dgetrs_", "dpotrf_", "dpotrs_", "dsyrk_", "dtrsm_"]
     L = _lapack()
     assert all(hasattr(L, n) for n in names)
 
@@ -128,6 +128,10 @@ def test_lapack_layer_argument_codes():
     assert info.value == -4
     L.dpotrs_(C.c_char_p(b"L"), _ip(4), _ip(-1), None, _ip(4), None, _ip(4), C.byref(info))
     assert info.value == -3
+    # dsyrk_ reports through xerbla only (blas/level3_impl.h:381-389)
+    one = C.byref(C.c_double(1.0))
+    L.dsyrk_(C.c_char_p(b"X"), C.c_char_p(b"N"), _ip(4), _ip(2), one, None, _ip(4), one, None,
+             _ip(4))
     # empty problems return at once with info 0
     L.dgetrf_(_ip(0), _ip(0), None, _ip(1), piv, C.byref(info))
     assert info.value == 0

@@ -335,6 +335,28 @@ def test_llt_rank_update_upper_and_failure(gpu):
     assert llt.info() == ea.NumericalIssue and int(g["fail_info"]) == 3
 
 
+def test_llt_rank_update_identity_rotation(gpu):
+    """The identity rotation (v_0 = 0, L_00 > 0) is skipped as in Eigen, so
+    an infinity in v reaches only the columns Eigen's update reaches."""
+    from eigen_amd import batched
+    from tests.golden.make_golden import rankupdate_identity_case
+    g = np.load(os.path.join(G, "rankupdate.npz"), allow_pickle=False)
+    l0, v = rankupdate_identity_case()
+    n = l0.shape[0]
+    L = dev(np.asfortranarray(l0).reshape(-1, order="F"))
+    info = batched.llt_rank_update_batched(
+        torch.zeros(1, dtype=torch.int64, device="cuda"),
+        torch.full((1,), n, dtype=torch.int32, device="cuda"), L,
+        torch.zeros(1, dtype=torch.int64, device="cuda"), dev(v),
+        torch.full((1,), 2.0, dtype=torch.float64, device="cuda"))
+    lf = host(L).reshape(n, n, order="F")
+    assert int(info[0]) == int(g["ident_info"]) == -1
+    # the non-finite entries where Eigen has them (assert_allclose checks
+    # their positions and signs), the rest to rounding; column 0 untouched
+    np.testing.assert_allclose(np.tril(lf), g["ident_L"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(lf[:, 0], l0[:, 0])
+
+
 def test_llt_rank_update_batched(gpu):
     """Many small factors in one launch (the EKF-style workload): each
     against the oracle's update of the same GPU factor; a size above 256 is

@@ -1,5 +1,5 @@
 """GPU tests of the Fortran-ABI layer (libeigen_amd_lapack.so): dgetrf_,
-dgetrs_, dpotrf_, dpotrs_, dgemm_, dtrsm_ on host column-major arrays,
+dgetrs_, dpotrf_, dpotrs_, dgemm_, dtrsm_, dsyrk_ on host column-major arrays,
 against the oracle (1-based ipiv equal to the oracle's transpositions + 1,
 info rules of lapack/lu.cpp:14-41 and lapack/cholesky.cpp:14-36)."""
 import ctypes as C
@@ -103,3 +103,64 @@ def test_dgemm_dtrsm(gpu, L):
     L.dtrsm_(C.c_char_p(b"L"), C.c_char_p(b"L"), C.c_char_p(b"N"), C.c_char_p(b"N"), _i(m), _i(n),
              C.byref(one), _d(t), _i(m), _d(x), _i(m))
     np.testing.assert_allclose(np.tril(t) @ x, x0, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("m,n", [(300, 200), (200, 300), (40, 13), (13, 40), (12, 16), (1, 5),
+                                 (7, 1)])
+def test_dgetrf_rectangular(gpu, L, m, n):
+    """dgetrf_ on m x n (lapack/lu.cpp:14-41 -> blocked_lu(rows, cols)):
+    min(m, n) pivots equal to Eigen's own blocked_lu (oracle/_ref) and the
+    factors within rounding of its output, including the reference's
+    treatment of a wide matrix's columns beyond m (left as they were above
+    16 pivots, solved below)."""
+    a0 = inputs.normal(51 + m, m, n)
+    a = np.asfortranarray(a0.copy())
+    k = min(m, n)
+    ipiv = np.zeros(k, dtype=np.int32)
+    info = C.c_int(-7)
+    L.dgetrf_(_i(m), _i(n), _d(a), _i(m), ipiv.ctypes.data_as(C.POINTER(C.c_int)),
+              C.byref(info))
+    assert info.value == 0
+    ref = np.asfortranarray(a0.copy())
+    t_ref = np.zeros(k, dtype=np.int32)
+    nbt = C.c_int(0)
+    fz = po.ref().eref_lu_blocked_d(m, n, ref, m, t_ref, C.byref(nbt))
+    assert fz == -1
+    np.testing.assert_array_equal(ipiv, t_ref + 1)
+    np.testing.assert_allclose(a, ref, rtol=0, atol=1e-11 * max(1.0, np.abs(ref).max()))
+    if m < n and k > 16:
+        np.testing.assert_array_equal(a[:, m:], a0[:, m:])
+    else:
+        # P A = L U over every column
+        pa = a0.copy()
+        for i in range(k):
+            pa[[i, t_ref[i]]] = pa[[t_ref[i], i]]
+        lo = np.tril(a[:, :k], -1) + np.eye(m, k)
+        np.testing.assert_allclose(lo @ np.triu(a[:k, :]), pa, rtol=0, atol=1e-12 * n)
+
+
+@pytest.mark.parametrize("uplo", [b"L", b"U"])
+@pytest.mark.parametrize("op", [b"N", b"T"])
+def test_dsyrk(gpu, L, uplo, op):
+    """dsyrk_ (blas/level3_impl.h:357-391): the chosen triangle of
+    alpha op(A) op(A)^T + beta C against the FP64 product, the other
+    triangle untouched; k = 0 only scales by beta."""
+    n, k = 90, 37
+    a = np.asfortranarray(inputs.normal(60, n, k) if op == b"N" else inputs.normal(60, k, n))
+    opa = a if op == b"N" else a.T
+    c0 = np.asfortranarray(inputs.normal(61, n, n))
+    c = np.asfortranarray(c0.copy())
+    alpha, beta = C.c_double(0.75), C.c_double(-0.5)
+    lda = n if op == b"N" else k
+    L.dsyrk_(C.c_char_p(uplo), C.c_char_p(op), _i(n), _i(k), C.byref(alpha), _d(a), _i(lda),
+             C.byref(beta), _d(c), _i(n))
+    ref = 0.75 * opa @ opa.T - 0.5 * c0
+    tri, other = (np.tril, lambda x: np.triu(x, 1)) if uplo == b"L" else \
+        (np.triu, lambda x: np.tril(x, -1))
+    np.testing.assert_allclose(tri(c), tri(ref), rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_array_equal(other(c), other(c0))
+    c = np.asfortranarray(c0.copy())
+    L.dsyrk_(C.c_char_p(uplo), C.c_char_p(op), _i(n), _i(0), C.byref(alpha), _d(a), _i(lda),
+             C.byref(beta), _d(c), _i(n))
+    np.testing.assert_array_equal(tri(c), tri(-0.5 * c0))
+    np.testing.assert_array_equal(other(c), other(c0))

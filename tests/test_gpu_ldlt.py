@@ -167,9 +167,38 @@ def test_ldlt_size_limits():
     offA, n, A, T, offT, sign, info, off, sizes = run_ldlt([a, b], max_n=4)
     assert int(info[0]) == -2 and int(info[1]) == 0
     np.testing.assert_array_equal(unpack(A, off, sizes, 0), a)
+    # the solve with the same max_n leaves the -2 system's right-hand side
+    # alone and solves the other one
+    rhs = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 2.0, 4.0, 9.0])
+    offb = torch.tensor([0, 5], dtype=torch.int64, device="cuda")
+    bt = torch.from_numpy(rhs.copy()).cuda()
+    batched.ldlt_solve_batched(offA, n, A, offT, T, offb, bt, max_n=4)
+    bh = bt.cpu().numpy()
+    np.testing.assert_array_equal(bh[:5], rhs[:5])
+    np.testing.assert_allclose(bh[5:], [2.0, 2.0, 3.0], rtol=1e-15)
     big = np.eye(65)
     with pytest.raises(Exception):
         run_ldlt([big], max_n=65)
+
+
+def test_ldlt_offsets_out_of_order():
+    """Transposition offsets given in descending order: the buffer is
+    sized by the furthest slot, and each system's transpositions and solve
+    equal those of the same matrix factored alone."""
+    mats = [inputs.ldlt_matrix("sym", 16), inputs.ldlt_matrix("pos", 5)]
+    off, sizes, flat = pack(mats)
+    offA = torch.from_numpy(off).cuda()
+    n = torch.from_numpy(sizes).cuda()
+    A = torch.from_numpy(flat).cuda()
+    offT = torch.tensor([5, 0], dtype=torch.int64, device="cuda")
+    T, _, sign, info = batched.ldlt_batched(offA, n, A, offT=offT)
+    assert T.numel() >= 21
+    Th = T.cpu().numpy()
+    for i, m in enumerate(mats):
+        _, t_ref, s_ref, ok = po.ldlt(m)
+        o = int(offT[i])
+        np.testing.assert_array_equal(Th[o:o + m.shape[0]], t_ref)
+        assert int(sign[i]) == s_ref and (int(info[i]) == 0) == ok
 
 
 def test_ldlt_class_mirror(gld):

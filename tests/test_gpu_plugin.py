@@ -93,3 +93,52 @@ def test_plugin_trsm(gpu, out):
     Tu = _load(out, "trsm_Tu.f64", (180, 180))
     Uu = np.triu(Tu, 1) + np.eye(180)
     np.testing.assert_allclose(X2 @ Uu, X20, rtol=0, atol=1e-11 * np.abs(X20).max())
+
+
+def _counts(d, name):
+    return dict(zip(["getrf", "potrf", "gemm", "syrk", "trsm", "trsv"],
+                    np.fromfile(os.path.join(d, name), dtype=np.int64)))
+
+
+def test_plugin_vector_solves_reach_the_gpu(gpu, out):
+    """lu.solve(VectorXd) and llt.solve(VectorXd) go through the vector
+    triangular-solve plug point (TriangularSolverVector.h:88-139): two GPU
+    solves each, backward error <= 1e-14."""
+    n = 300
+    A = _load(out, "lu_A.f64", (n, n))
+    b = _load(out, "luv_b.f64", (n,))
+    x = _load(out, "luv_x.f64", (n,))
+    before, after = _counts(out, "cnt_lu.i64"), _counts(out, "cnt_luv.i64")
+    assert after["trsv"] - before["trsv"] == 2
+    assert po.backward_error(A, x, b) <= 1e-14
+    S = _load(out, "llt_S.f64", (n, n))
+    bs = _load(out, "llt_b.f64", (n, 3))[:, 0]
+    for name in ("lltv_x.f64", "lltv_xu.f64"):
+        assert po.backward_error(S, _load(out, name, (n,)), bs) <= 1e-14
+    c = _counts(out, "cnt_llt.i64")
+    assert c["trsv"] - after["trsv"] == 4 and c["potrf"] >= 3
+
+
+def test_plugin_syrk(gpu, out):
+    """selfadjointView<Lower>().rankUpdate(A, 0.5) and
+    selfadjointView<Upper>().rankUpdate(B^T, -2) run as GPU SYRKs: the
+    updated triangle against the FP64 product, the other one untouched."""
+    Sa = _load(out, "syrk_A.f64", (150, 70))
+    Sb = _load(out, "syrk_B.f64", (60, 150))
+    C0 = _load(out, "syrk_C0.f64", (150, 150))
+    Sl = _load(out, "syrk_L.f64", (150, 150))
+    Su = _load(out, "syrk_U.f64", (150, 150))
+    rl = C0 + 0.5 * Sa @ Sa.T
+    ru = C0 - 2.0 * Sb.T @ Sb
+    tol = 1e-13 * max(np.abs(rl).max(), np.abs(ru).max())
+    np.testing.assert_allclose(np.tril(Sl), np.tril(rl), rtol=0, atol=tol)
+    np.testing.assert_array_equal(np.triu(Sl, 1), np.triu(C0, 1))
+    np.testing.assert_allclose(np.triu(Su), np.triu(ru), rtol=0, atol=tol)
+    np.testing.assert_array_equal(np.tril(Su, -1), np.tril(C0, -1))
+    before, after = _counts(out, "cnt_trsm.i64"), _counts(out, "cnt_syrk.i64")
+    assert after["syrk"] - before["syrk"] == 2
+
+
+def test_plugin_empty_matrices(gpu, out):
+    """PartialPivLU and LLT of a 0 x 0 matrix succeed, as in the reference."""
+    assert _load(out, "empty_ok.i32", (1,), np.int32)[0] == 1

@@ -378,6 +378,20 @@ def test_llt_rank_downdate_failing_column(grk):
     assert info == int(grk["fail_info"]) == 3
 
 
+def test_llt_rank_update_identity_rotation(grk):
+    """v_0 = 0 over a positive L_00 makes the identity rotation, which
+    Eigen does not apply (Jacobi.h:460-461): column 0 keeps its values
+    although v holds an infinity further down.  The oracle matches Eigen's
+    output entry for entry, infinities included."""
+    from tests.golden.make_golden import rankupdate_identity_case
+    l0, v = rankupdate_identity_case()
+    lf, info = po.llt_rank_update(l0, v, 2.0)
+    assert info == int(grk["ident_info"]) == -1
+    # the non-finite entries where Eigen has them, the rest to rounding
+    np.testing.assert_allclose(np.tril(lf), grk["ident_L"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(lf[:, 0], l0[:, 0])
+
+
 def test_llt_rank_update_closed_form():
     """Diagonal A and v = e_k: only L_kk changes, to sqrt(A_kk + sigma)."""
     n = 7
