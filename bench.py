@@ -97,10 +97,63 @@ def cpu_baseline(n=4096):
         po.lu_solve(lu_, t, b)
         dt = time.perf_counter() - t0
         kind, cores = "port", 1
-    return {"value": round(hpl_flops(n) / dt / 1e12, 5), "unit": "TFLOP/s", "cores": cores,
-            "kind": kind,
-            "sample": f"PartialPivLU<MatrixXd> factor+solve, n={n} FP64 N(0,1), 1 RHS, "
-                      f"{dt:.2f} s (HPL flop count)"}
+    out = {"value": round(hpl_flops(n) / dt / 1e12, 5), "unit": "TFLOP/s", "cores": cores,
+           "kind": kind,
+           "sample": f"PartialPivLU<MatrixXd> factor+solve, n={n} FP64 N(0,1), 1 RHS, "
+                     f"{dt:.2f} s (HPL flop count)",
+           "cpu_model": cpu_model()}
+    if ref is not None:
+        out.update(cpu_batched_legs(ref, cores))
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name and logical CPU count as the OS reports them."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs, {len(os.sched_getaffinity(0))} usable)"
+
+
+def cpu_batched_legs(ref, cores):
+    """The reference per system on the host cores (OpenMP over systems, Eigen
+    serial inside: Parallelizer.h:123): LLT<MatrixXf> solves of the C4
+    workload and PartialPivLU<MatrixXd> 4-RHS solves of a bounded sample of
+    the C5 workload, on the same synthetic inputs as the GPU legs."""
+    import numpy as np
+
+    from tests import inputs
+    legs = {}
+    n, offA, A, offb, b = inputs.packed_spd_gpu(1_000_000, [6, 9, 12, 15, 18, 24], seed=4)
+    nh, oah, Ah, obh = (n.cpu().numpy(), offA.cpu().numpy(), A.cpu().numpy(),
+                        offb.cpu().numpy())
+    bh = b.cpu().numpy()
+    del n, offA, A, offb, b
+    info = np.zeros(len(nh), dtype=np.int32)
+    t0 = time.perf_counter()
+    ref.eref_batched_llt_solve_f(len(nh), oah, nh, Ah, obh, bh, info)
+    dt = time.perf_counter() - t0
+    legs["batched_llt"] = {"solves_per_s": round(len(nh) / dt, 1), "cores": cores,
+                           "sample": f"all {len(nh)} systems of the batched LLT workload, "
+                                     f"{dt:.3f} s"}
+    cnt = 8192
+    n, offA, A, offB, B = inputs.packed_general_gpu(cnt, 16, 256, 4, seed=5)
+    nh, oah, Ah, obh, Bh = (n.cpu().numpy(), offA.cpu().numpy(), A.cpu().numpy(),
+                            offB.cpu().numpy(), B.cpu().numpy())
+    info = np.zeros(cnt, dtype=np.int32)
+    t0 = time.perf_counter()
+    ref.eref_batched_lu_solve_d(cnt, oah, nh, Ah, obh, 4, Bh, info)
+    dt = time.perf_counter() - t0
+    legs["batched_lu"] = {"solves_per_s": round(cnt / dt, 1), "cores": cores,
+                          "sample": f"{cnt} systems of the batched LU workload's distribution "
+                                    f"(n uniform in [16, 256], 4 RHS), {dt:.3f} s"}
+    return legs
 
 
 def measured_traffic():
@@ -181,6 +234,261 @@ def dense_fields():
                              "factor_tflops": round(n ** 3 / 3.0 / ms_f / 1e9, 2),
                              "solve_ms": round(ms_s, 2),
                              "backward_error": ea.backward_error(S, st["x"], B)}
+    return out
+
+
+# ------------------------------------------------------- batched workloads --
+def _sample_berr(n, offA, A, offB, nrhs, B, X, idx):
+    """max normwise backward error over the sampled systems (FP64, host)."""
+    import numpy as np
+    nn, oa, ob = n.cpu().numpy(), offA.cpu().numpy(), offB.cpu().numpy()
+    Ah, Bh, Xh = A.cpu().numpy(), B.cpu().numpy(), X.cpu().numpy()
+    worst = 0.0
+    for i in idx:
+        k = int(nn[i])
+        a = Ah[int(oa[i]):int(oa[i]) + k * k].reshape(k, k, order="F").astype(np.float64)
+        b = Bh[int(ob[i]):int(ob[i]) + k * nrhs].reshape(k, nrhs, order="F").astype(np.float64)
+        x = Xh[int(ob[i]):int(ob[i]) + k * nrhs].reshape(k, nrhs, order="F").astype(np.float64)
+        r = b - a @ x
+        an = np.abs(a).sum(axis=1).max()
+        for q in range(nrhs):
+            den = an * np.abs(x[:, q]).max() + np.abs(b[:, q]).max()
+            worst = max(worst, float(np.abs(r[:, q]).max() / den))
+    return worst
+
+
+def loop_rate_cabi(n, offA, A, offB, nrhs, B, idx, kind):
+    """The per-matrix launch loop of the plain FP64 drop-in, through the C ABI
+    (no Python objects in the loop): eamd_dgetrf + eamd_dgetrs ("lu") or
+    eamd_dpotrf + eamd_dpotrs ("llt") on each sampled system, with device
+    copies of the systems prepared before the timed loop; solves per second."""
+    import ctypes as C
+
+    import torch
+
+    import eigen_amd as ea
+    L, h = ea.lib(), ea.handle()
+    nn, oa, ob = n.cpu().numpy(), offA.cpu().numpy(), offB.cpu().numpy()
+    mats = []
+    for i in idx:
+        k = int(nn[i])
+        a = A[int(oa[i]):int(oa[i]) + k * k].to(torch.float64).clone()
+        b = B[int(ob[i]):int(ob[i]) + k * nrhs].to(torch.float64).clone()
+        piv = torch.empty(k, dtype=torch.int32, device=A.device)
+        mats.append((k, a, b, piv))
+    nbt, fz, fail = C.c_int32(), C.c_int64(), C.c_int64()
+
+    def run(batch):
+        for k, a, b, piv in batch:
+            if kind == "lu":
+                L.eamd_dgetrf(h, k, k, a.data_ptr(), k, piv.data_ptr(), C.byref(nbt),
+                              C.byref(fz))
+                L.eamd_dgetrs(h, b"N", k, nrhs, a.data_ptr(), k, piv.data_ptr(), b.data_ptr(),
+                              k)
+            else:
+                L.eamd_dpotrf(h, b"L", k, a.data_ptr(), k, C.byref(fail))
+                L.eamd_dpotrs(h, b"L", k, nrhs, a.data_ptr(), k, b.data_ptr(), k)
+
+    warm = [(k, a.clone(), b.clone(), p) for k, a, b, p in mats[:5]]
+    run(warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(mats)
+    torch.cuda.synchronize()
+    return len(mats) / (time.perf_counter() - t0)
+
+
+def batched_llt_fields(count=1_000_000, steps=5):
+    """C4: 1M packed FP32 SPD systems, one persistent launch captured in a
+    graph; returns (fields, inputs kept for the rank-update and LDLT legs)."""
+    import numpy as np
+    import torch
+
+    from eigen_amd import batched
+    from tests import inputs
+    sizes = [6, 9, 12, 15, 18, 24]
+    n, offA, A, offb, b0 = inputs.packed_spd_gpu(count, sizes, seed=4)
+    b = b0.clone()
+    info = torch.empty(count, dtype=torch.int32, device="cuda")
+    t0 = time.perf_counter()
+    plan = batched.BatchedPlan(n, 32)
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    g = batched.capture_llt(plan, offA, A, offb, b, info)
+    # each replay solves in place: restore b before each timed replay; the
+    # replay is timed with events on the stream it runs on
+    times = []
+    for _ in range(steps):
+        b.copy_(b0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    dt = min(times)
+    idx = np.random.Generator(np.random.Philox(3)).choice(count, 1000, replace=False)
+    worst = _sample_berr(n, offA, A, offb, 1, b0, b, idx)
+    nn = n.cpu().numpy().astype(np.int64)
+    bytes_moved = float(4 * (nn ** 2 + 2 * nn).sum() + 8 * count)
+    out = {
+        "workload": f"{count} SPD systems, n uniform in {sizes}, FP32, packed + offsets, "
+                    "1 RHS, one persistent launch captured in a graph",
+        "solves_per_s": round(count / dt, 1),
+        "ms": round(dt * 1e3, 4),
+        "plan_ms": round(plan_ms, 2),
+        "hbm_gbs": round(bytes_moved / dt / 1e9, 1),
+        "hbm_frac": round(bytes_moved / dt / 8e12, 4),
+        "lanes_per_system": {s: plan.lanes(s) for s in sizes},
+        "sample_backward_error_max": worst,
+        "all_info_ok": bool((info == -1).all().item()),
+    }
+    try:
+        loop = loop_rate_cabi(n, offA, A, offb, 1, b0, idx[:300], "llt")
+        out["per_matrix_loop_solves_per_s"] = round(loop, 1)
+        out["speedup_vs_per_matrix_loop"] = round(count / dt / loop, 1)
+    except Exception as e:  # reported, never hides the batched result
+        out["per_matrix_loop_error"] = str(e)[:200]
+    del b
+    return out, (n, offA, A, offb, b0)
+
+
+def batched_lu_fields(count=65536, nrhs=4, steps=3):
+    """C5: batched uneven LU + FP64 refinement (n_i uniform in [16, 256],
+    4 right-hand sides)."""
+    import numpy as np
+    import torch
+
+    from eigen_amd import batched
+    from tests import inputs
+    n, offA, A, offB, B = inputs.packed_general_gpu(count, 16, 256, nrhs, seed=5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = batched.BatchedLuPlan(n, nrhs)
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    X = torch.empty_like(B)
+    info = torch.empty(count, dtype=torch.int32, device="cuda")
+    iters = torch.empty_like(info)
+    plan.solve(offA, n, A, offB, B, X, info, iters)  # warm-up
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        plan.solve(offA, n, A, offB, B, X, info, iters)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+    dt = min(times)
+    n64 = n.to(torch.int64).double()
+    flops = float((2.0 / 3.0 * n64 ** 3 + 2.0 * nrhs * n64 ** 2).sum())
+    bytes_moved = float((8.0 * n64 ** 2 + 16.0 * nrhs * n64 + 24.0).sum())
+    idx = np.random.Generator(np.random.Philox(6)).choice(count, 200, replace=False)
+    loop = loop_rate_cabi(n, offA, A, offB, nrhs, B, idx[:100], "lu")
+    inf = info.cpu().numpy()
+    return {
+        "workload": f"{count} general systems, n uniform integer in [16, 256], FP64 N(0,1), "
+                    f"{nrhs} RHS, FP32 LU on MFMA + FP64 refinement to 1e-14",
+        "solves_per_s": round(count / dt, 1),
+        "ms": round(dt * 1e3, 3),
+        "plan_ms": round(plan_ms, 2),
+        "tflops_fp64_equiv": round(flops / dt / 1e12, 3),
+        "hbm_gbs": round(bytes_moved / dt / 1e9, 1),
+        "hbm_frac": round(bytes_moved / dt / 8e12, 4),
+        "sample_backward_error_max": _sample_berr(n, offA, A, offB, nrhs, B, X, idx),
+        "refined_from_fp32": int((inf == 0).sum()),
+        "fp64_refactor": int((inf == 1).sum()),
+        "not_converged": int((inf == 2).sum()),
+        "max_refinement_steps": int(iters.max().item()),
+        "per_matrix_loop_solves_per_s": round(loop, 1),
+        "speedup_vs_per_matrix_loop": round(count / dt / loop, 1),
+        "threads_per_system": {k: plan.threads(k) for k in (16, 64, 128, 192, 256)},
+    }
+
+
+def rank_update_fields(n, offA, A, offv, steps=5):
+    """Rank-one updates (LLT::rankUpdate, sigma = 1) of the C4 workload's
+    factors in FP64: the packed SPD matrices' lower triangles stand in for
+    factors (a positive diagonal is all an update with sigma > 0 needs)."""
+    import torch
+
+    from eigen_amd import batched
+    count = n.numel()
+    L = A.to(torch.float64)
+    V = torch.randn(int(n.sum().item()), dtype=torch.float64, device=A.device,
+                    generator=torch.Generator(device=A.device).manual_seed(5))
+    offv = offv.to(torch.int64)
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        info = batched.llt_rank_update_batched(offA, n, L, offv, V)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+    dt = min(times)
+    nn = n.to(torch.int64)
+    bytes_moved = float((8 * (nn * (nn + 1)) + 8 * nn).sum().item())  # L read + write, v
+    return {"workload": f"{count} FP64 factors of the batched LLT sizes, one rank-one "
+                        "update each (sigma = 1), packed + offsets",
+            "updates_per_s": round(count / dt, 1), "ms": round(dt * 1e3, 3),
+            "hbm_gbs": round(bytes_moved / dt / 1e9, 1),
+            "hbm_frac": round(bytes_moved / dt / 8e12, 4),
+            "all_info_ok": bool((info == -1).all().item())}
+
+
+def ldlt_fields(n, offA, A, offb, b, steps=5):
+    """LDLT compute + solve (LDLT.h:491-522, 559-593) of the C4 workload's SPD
+    matrices in FP64, one right-hand side each; the packed inputs are
+    restored (untimed) before every timed step."""
+    import torch
+
+    from eigen_amd import batched
+    count = n.numel()
+    A0 = A.to(torch.float64)
+    b0 = b.to(torch.float64)
+    offT = offb.to(torch.int64)
+    max_n = int(n.max().item())
+    Aw, bw = A0.clone(), b0.clone()
+    times = []
+    for _ in range(steps):
+        Aw.copy_(A0)
+        bw.copy_(b0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        T, _, sign, info = batched.ldlt_batched(offA, n, Aw, offT, max_n=max_n)
+        batched.ldlt_solve_batched(offA, n, Aw, offT, T, offT, bw, max_n=max_n)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+    dt = min(times)
+    return {"workload": f"{count} FP64 symmetric systems of the batched LLT sizes, LDLT "
+                        "compute + solve (one right-hand side), packed + offsets",
+            "systems_per_s": round(count / dt, 1), "ms": round(dt * 1e3, 3),
+            "all_info_ok": bool((info == 0).all().item()),
+            "all_positive": bool((sign == 0).all().item())}
+
+
+def batched_fields():
+    """Extra fields of the benchmark line for the batched workloads; each leg
+    reports its own failure without hiding the others."""
+    out = {}
+    kept = None
+    try:
+        out["batched_llt"], kept = batched_llt_fields()
+    except Exception as e:
+        out["batched_llt"] = {"error": str(e)[:300]}
+    if kept is not None:
+        n, offA, A, offb, b0 = kept
+        for key, fn in (("batched_llt_rank_update", lambda: rank_update_fields(n, offA, A, offb)),
+                        ("batched_ldlt", lambda: ldlt_fields(n, offA, A, offb, b0))):
+            try:
+                out[key] = fn()
+            except Exception as e:
+                out[key] = {"error": str(e)[:300]}
+        del n, offA, A, offb, b0, kept
+    try:
+        out["batched_lu"] = batched_lu_fields()
+    except Exception as e:
+        out["batched_lu"] = {"error": str(e)[:300]}
     return out
 
 
@@ -292,11 +600,7 @@ def main():
             extras.update(dense_fields())
         except Exception as e:
             extras["dense"] = {"error": str(e)[:200]}
-        try:
-            from eigen_amd import batched
-            extras.update(batched.bench_fields())
-        except Exception as e:
-            extras["batched"] = {"error": str(e)[:200]}
+        extras.update(batched_fields())
 
     ach_gbs = upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(ach_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -12,7 +12,8 @@ struct eamd_batched_plan {
   int* order = nullptr;     // device: system indices sorted by size (descending)
   int bucket_start[258] = {0};  // host: first position of each size in `order`
   int bucket_count[258] = {0};  // host: systems of each size
-  int lanes[258] = {0};         // host: lanes (or threads) per system chosen per size
+  int lanes[258] = {0};         // host: threads per system chosen per size (batched LU)
+  unsigned char llt_path[33] = {0};  // host: code path chosen per size (batched LLT)
   // batched LU only (kind 1)
   int kind = 0;             // 0 batched LLT, 1 batched LU
   int nrhs = 0;

@@ -1024,7 +1024,7 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
   float a[LW];
   // 16-byte words when the rows allow (saved rows always do)
   const bool vec = FULL && ((uintptr_t)P & 15) == 0 && (ldp & 3) == 0;
-  if (src >= 0 || vec) {
+  if (FULL && (src >= 0 || vec)) {
     const v4f* ar = reinterpret_cast<const v4f*>(src >= 0 ? saved + src * LW
                                                           : P + (int64_t)i * ldp);
 #pragma unroll
@@ -1036,9 +1036,15 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
       a[4 * q + 3] = t.w;
     }
   } else {
-    const float* row = P + (int64_t)i * ldp;
+    // one load path (a saved row or the row itself), word by word, every
+    // load unconditional (columns past w re-read column w - 1 and are
+    // masked): conditional loads of a narrow leaf's row keep `a` in scratch
+    const float* row = src >= 0 ? saved + src * LW : P + (int64_t)i * ldp;
 #pragma unroll
-    for (int c = 0; c < LW; ++c) a[c] = (FULL || c < w) ? row[c] : 0.f;
+    for (int c = 0; c < LW; ++c) {
+      const float v = row[FULL ? c : min(c, w - 1)];
+      a[c] = (FULL || c < w) ? v : 0.f;
+    }
   }
 #pragma unroll
   for (int k = 0; k < LW; ++k) {
@@ -1054,9 +1060,11 @@ __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, i
     for (int q = 0; q < LW / 4; ++q)
       reinterpret_cast<v4f*>(dst)[q] = v4f{a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};
   } else {
+    // unconditional stores, last column first: columns past w land on
+    // column w - 1, which its own value overwrites last (conditional stores
+    // of a narrow leaf keep `a` in scratch memory)
 #pragma unroll
-    for (int c = 0; c < LW; ++c)
-      if (FULL || c < w) dst[c] = a[c];
+    for (int c = LW - 1; c >= 0; --c) dst[FULL ? c : min(c, w - 1)] = a[c];
   }
 }
 

@@ -61,3 +61,53 @@ def ldlt_matrix(kind, n):
     if kind == "pos":
         return spd(7400 + n, n)
     return np.asfortranarray(-spd(7500 + n, n))
+
+
+# ------------------------------------------------- device-side generators --
+def packed_spd_gpu(count, sizes, seed, device="cuda"):
+    """Packed FP32 SPD systems A_i = B B^T + n_i I (B iid N(0,1)), b_i N(0,1);
+    sizes drawn uniformly from `sizes`, generated on the GPU with torch.
+    Returns n (int32), offA, A, offb, b (device tensors)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    sz = torch.tensor(sizes, dtype=torch.int32, device=device)
+    pick = torch.randint(0, len(sizes), (count,), generator=g, device=device)
+    n = sz[pick]
+    n64 = n.to(torch.int64)
+    offA = torch.zeros(count, dtype=torch.int64, device=device)
+    offA[1:] = torch.cumsum(n64 * n64, 0)[:-1]
+    offb = torch.zeros(count, dtype=torch.int64, device=device)
+    offb[1:] = torch.cumsum(n64, 0)[:-1]
+    A = torch.empty(int((n64 * n64).sum()), dtype=torch.float32, device=device)
+    for s in sizes:
+        idx = torch.nonzero(n == s).flatten()
+        k = idx.numel()
+        if k == 0:
+            continue
+        for c0 in range(0, k, 65536):
+            ii = idx[c0:c0 + 65536]
+            Bm = torch.randn(ii.numel(), s, s, generator=g, device=device, dtype=torch.float32)
+            Am = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
+            Am += s * torch.eye(s, device=device, dtype=torch.float32)
+            flat = offA[ii][:, None] + torch.arange(s * s, device=device)[None, :]
+            A[flat.flatten()] = Am.transpose(1, 2).reshape(-1)
+    b = torch.randn(int(n64.sum()), generator=g, device=device, dtype=torch.float32)
+    return n, offA, A, offb, b
+
+
+def packed_general_gpu(count, lo, hi, nrhs, seed, device="cuda"):
+    """Packed FP64 general systems, A_i iid N(0,1) with n_i uniform integer in
+    [lo, hi], B_i n_i x nrhs N(0,1), generated on the GPU with torch.
+    Returns n (int32), offA, A, offB, B (device tensors)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    n = torch.randint(lo, hi + 1, (count,), generator=g, device=device, dtype=torch.int64)
+    offA = torch.zeros(count, dtype=torch.int64, device=device)
+    offA[1:] = torch.cumsum(n * n, 0)[:-1]
+    offB = torch.zeros(count, dtype=torch.int64, device=device)
+    offB[1:] = torch.cumsum(n * nrhs, 0)[:-1]
+    A = torch.randn(int((n * n).sum()), generator=g, device=device, dtype=torch.float64)
+    B = torch.randn(int((n * nrhs).sum()), generator=g, device=device, dtype=torch.float64)
+    return n.to(torch.int32), offA, A, offB, B

@@ -71,7 +71,7 @@ def test_batched_llt_rejects_oversize(gpu):
 def test_batched_llt_graph_replays(gpu):
     """Replay the captured execution several times with eager work and
     synchronisations in between; every replay must equal the eager result."""
-    n, offA, A, offb, b0 = batched.packed_spd_gpu(20000, [6, 9, 12, 15, 18, 24], seed=9)
+    n, offA, A, offb, b0 = inputs.packed_spd_gpu(20000, [6, 9, 12, 15, 18, 24], seed=9)
     ref = b0.clone()
     plan = batched.BatchedPlan(n, 32)
     info = torch.empty(n.numel(), dtype=torch.int32, device="cuda")

@@ -232,7 +232,7 @@ def test_ldlt_graph_replays():
     work in between, each equal to the eager result bit for bit."""
     from eigen_amd._lib import check, lib
     from eigen_amd.dense import _p, handle
-    n, offA, A0, offb, b0 = batched.packed_spd_gpu(5000, [6, 9, 12, 15, 18, 24], seed=13)
+    n, offA, A0, offb, b0 = inputs.packed_spd_gpu(5000, [6, 9, 12, 15, 18, 24], seed=13)
     A0, b0 = A0.to(torch.float64), b0.to(torch.float64)
     offT = offb.to(torch.int64)
     count = n.numel()
