@@ -67,6 +67,7 @@ _SIGS = {
     "eamd_dllt_rank_update_batched": ([vp, c_int64, vp, vp, vp, vp, vp, vp, C.c_int32, vp], c_int),
     "eamd_dldlt_batched": ([vp, c_int64, vp, vp, C.c_int32, vp, vp, vp, vp, vp], c_int),
     "eamd_dldlt_solve_batched": ([vp, c_int64, vp, vp, C.c_int32, vp, vp, vp, vp, vp], c_int),
+    "eamd_spotrf_solve_batched_noalloc": ([vp, c_int64, vp, vp, vp, vp, vp, vp], c_int),
     "eamd_batched_plan_create": ([vp, c_int64, vp, C.c_int32, C.POINTER(vp)], c_int),
     "eamd_batched_plan_destroy": ([vp], c_int),
     "eamd_batched_plan_lanes": ([vp, C.c_int32], c_int),

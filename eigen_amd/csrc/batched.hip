@@ -484,6 +484,69 @@ int batched_plan_sort(eamd_handle* h, int64_t count, const int32_t* n, int max_n
   return kOk;
 }
 
+// ---- allocation-free batched LLT (eamd_spotrf_solve_batched_noalloc) ----
+// The plan's device buffers (the size histogram with its scatter cursors,
+// and the size-sorted order) live in one file-scope device array, one copy
+// per GPU, whose address each device looks up once (batched_noalloc_init,
+// called by eamd_create with the handle's device current).  Host code only
+// ever passes that looked-up address, never the array's name.
+constexpr int64_t kNoAllocMaxSystems = 1 << 20;
+constexpr size_t kNoAllocInts = 2 * (kMaxPlanN + 2) + (size_t)kNoAllocMaxSystems;
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int batched_noalloc_init(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) {
+    set_error("allocation-free batched solve: device %d beyond %d", device, kNoAllocMaxDevices);
+    return kErrUnsupported;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return kOk;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+  g_scratch_ptr[device] = static_cast<int*>(ptr);
+  return kOk;
+}
+
+int spotrf_solve_batched_noalloc(eamd_handle* h, int64_t count, const int64_t* offA,
+                                 const int32_t* n, const float* A, const int64_t* offb, float* b,
+                                 int32_t* info) {
+  if (count > kNoAllocMaxSystems) {
+    set_error("allocation-free batched solve: %lld systems exceed its maximum of %lld",
+              (long long)count, (long long)kNoAllocMaxSystems);
+    return kErrUnsupported;
+  }
+  int* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (h->device >= 0 && h->device < kNoAllocMaxDevices) base = g_scratch_ptr[h->device];
+  }
+  if (!base) {
+    set_error("allocation-free batched solve: no scratch address for device %d", h->device);
+    return kErrHip;
+  }
+  int* hist = base;                       // cleared by size_sort on every call
+  int* order = base + 2 * (kMaxPlanN + 2);  // fully written for this call's count
+  int bstart[kMaxPlanN + 2], bcount[kMaxPlanN + 2];
+  EAMD_CHECK(size_sort(h, count, n, kMaxBatchN, hist, order, bstart, bcount));
+  // the widths an earlier planned call tuned, else the narrowest (tuning
+  // would allocate its synthetic systems)
+  LltSched sc{};
+  for (int v = kMaxBatchN; v >= 1; --v) {
+    int path = paths_for(v).front();
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      if (g_tune[v]) path = g_tune[v] - 1;
+    }
+    sched_add(sc, v, path, bstart[v], bcount[v]);
+  }
+  return launch_sched(h, sc, order, offA, A, offb, b, b, info);
+}
+
 int batched_plan_create(eamd_handle* h, int64_t count, const int32_t* n, int max_n,
                         eamd_batched_plan** out) {
   EAMD_CHECK(batched_plan_sort(h, count, n, max_n, out));

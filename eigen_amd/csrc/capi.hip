@@ -179,6 +179,10 @@ void batched_plan_destroy(eamd_batched_plan* p);
 int batched_lanes(const eamd_batched_plan* p, int n);
 int spotrf_solve_batched_exec(eamd_handle* h, const eamd_batched_plan* p, const int64_t* offA,
                               const float* A, const int64_t* offb, float* b, int32_t* info);
+int batched_noalloc_init(int device);
+int spotrf_solve_batched_noalloc(eamd_handle* h, int64_t count, const int64_t* offA,
+                                 const int32_t* n, const float* A, const int64_t* offb, float* b,
+                                 int32_t* info);
 
 namespace {
 
@@ -267,6 +271,13 @@ int eamd_create(eamd_handle_t* handle) {
     return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
   }
   h->own_stream = true;
+  // the allocation-free entry point's scratch address on this device
+  const int st = batched_noalloc_init(dev);
+  if (st != kOk) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return st;
+  }
   *handle = h;
   return kOk;
 }
@@ -660,6 +671,21 @@ int eamd_spotrf_solve_batched(eamd_handle_t h, int64_t count, const int64_t* off
   if (!b) return -7;
   if (!info) return -8;
   return spotrf_solve_batched_impl(h, count, offA, n, A, offb, b, info);
+}
+
+int eamd_spotrf_solve_batched_noalloc(eamd_handle_t h, int64_t count, const int64_t* offA,
+                                      const int32_t* n, const float* A, const int64_t* offb,
+                                      float* b, int32_t* info) {
+  if (count < 0) return -2;
+  CHECK_HANDLE(h);
+  if (count == 0) return kOk;
+  if (!offA) return -3;
+  if (!n) return -4;
+  if (!A) return -5;
+  if (!offb) return -6;
+  if (!b) return -7;
+  if (!info) return -8;
+  return spotrf_solve_batched_noalloc(h, count, offA, n, A, offb, b, info);
 }
 
 int eamd_batched_plan_create(eamd_handle_t h, int64_t count, const int32_t* n, int32_t max_n,

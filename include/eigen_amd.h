@@ -235,6 +235,20 @@ int eamd_spotrf_solve_batched(eamd_handle_t h, int64_t count, const int64_t* off
                               const int32_t* n, const float* A, const int64_t* offb, float* b,
                               int32_t* info);
 
+/* No counterpart in the reference: eamd_spotrf_solve_batched for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and size rule (n[i] in 1..32); at most 1,048,576 systems per call (larger
+ * counts return EAMD_ERR_UNSUPPORTED).  Its plan buffers are a static
+ * device array of about 4 MiB per GPU, held until the process ends and
+ * shared by every call on that GPU, so calls that overlap on one GPU race.
+ * It synchronises the handle's stream once (the size histogram is read on
+ * the host), uses the per-size widths tuned by an earlier planned call and
+ * otherwise the narrowest, and calls no allocator. */
+int eamd_spotrf_solve_batched_noalloc(eamd_handle_t h, int64_t count, const int64_t* offA,
+                                      const int32_t* n, const float* A, const int64_t* offb,
+                                      float* b, int32_t* info);
+
 /* The same split into a plan (size bucketing of n[], one host
  * synchronisation, autotuned lanes-per-system per size) and an execution
  * that only queues kernels -- no host synchronisation, no memset -- so it can
