@@ -492,6 +492,27 @@ def batched_fields():
     return out
 
 
+def fp64_fields(A, b, n, dt):
+    """The plain FP64 GPU LU (factor + solve) on the headline input, and the
+    mixed solver's speedup over it."""
+    import torch
+
+    from eigen_amd import mixed
+    x64, rep64 = mixed.gesv_ir(A, b, precision="fp64")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    x64, rep64 = mixed.gesv_ir(A, b, precision="fp64")
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t1
+    out = {"fp64_lu_tflops": round(hpl_flops(n) / dt64 / 1e12, 3),
+           "fp64_lu_ms": round(dt64 * 1e3, 2),
+           "fp64_backward_error": rep64["backward_error"],
+           "speedup_mixed_vs_fp64": round(dt64 / dt, 3)}
+    log(f"fp64: {dt64 * 1e3:.1f} ms, {out['fp64_lu_tflops']} TFLOP/s, "
+        f"bwd err {rep64['backward_error']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,7 +521,11 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--precision", default="fp16x3")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline solve and the plain FP64 LU beside it")
+    ap.add_argument("--legs", default="fp64,householder,dense,batched",
+                    help="comma list of the reported workloads beside the headline "
+                         "(fp64, householder, dense, batched): a profiler run can take one")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, to "
                          "calibrate hardware byte counters) and exit")
@@ -564,22 +589,13 @@ def main():
         f"bwd err {worst_be:.2e}, phases(ms/step) {phases}")
 
     extras = {}
-    # plain FP64 GPU LU (factor + solve) on the same input
-    x64, rep64 = mixed.gesv_ir(A, b, precision="fp64")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    x64, rep64 = mixed.gesv_ir(A, b, precision="fp64")
-    torch.cuda.synchronize()
-    dt64 = time.perf_counter() - t1
-    extras["fp64_lu_tflops"] = round(hpl_flops(n) / dt64 / 1e12, 3)
-    extras["fp64_lu_ms"] = round(dt64 * 1e3, 2)
-    extras["fp64_backward_error"] = rep64["backward_error"]
-    extras["speedup_mixed_vs_fp64"] = round(dt64 / dt, 3)
-    log(f"fp64: {dt64 * 1e3:.1f} ms, {extras['fp64_lu_tflops']} TFLOP/s, "
-        f"bwd err {rep64['backward_error']:.2e}")
-    del x64
+    legs = set(args.legs.split(",")) if args.legs else set()
+    if args.no_extras:
+        legs &= {"fp64"}
+    if "fp64" in legs:
+        extras.update(fp64_fields(A, b, n, dt))
 
-    if not args.no_extras:
+    if "householder" in legs:
         try:
             Ah = householder_matrix_gpu(n, 64, 5)
             torch.cuda.synchronize()
@@ -596,10 +612,12 @@ def main():
             del Ah, xh
         except Exception as e:  # reported, not fatal for the headline
             extras["householder_kappa1e6"] = {"error": str(e)[:200]}
+    if "dense" in legs:
         try:
             extras.update(dense_fields())
         except Exception as e:
             extras["dense"] = {"error": str(e)[:200]}
+    if "batched" in legs:
         extras.update(batched_fields())
 
     ach_gbs = upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
