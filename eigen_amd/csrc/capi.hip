@@ -146,10 +146,14 @@ int env_int(const char* name, int dflt) {
   return v && *v ? atoi(v) : dflt;
 }
 
-int lu_block_width(int64_t n) {
+int lu_block_width(int64_t n, bool fp64) {
   int nb = env_int("EAMD_LU_NB", 0);
   if (nb > 0) return nb;
   if (n <= 2048) return 64;
+  // the FP64 factorization's trailing updates gain more from K = 256 than
+  // its panels lose (n = 16384: 224.7 ms at 256, 244.0 ms at 128); the
+  // mixed solver's FP32 panel is fastest at 128 (67.3 vs 72.0 ms at 96)
+  if (fp64 && n >= 8192) return 256;
   return 128;
 }
 
@@ -459,7 +463,7 @@ int eamd_dgetrf(eamd_handle_t h, int64_t m, int64_t n, double* A, int64_t lda, i
   // W (row-major, m x ncols) = A
   EAMD_CHECK(transpose_copy<double, double>(s, A, lda, M, NC, W, ldw));
   int64_t fz = -1;
-  EAMD_CHECK(dgetrf_rowmajor(h, W, ldw, M, NC, transp, &fz, lu_block_width(size)));
+  EAMD_CHECK(dgetrf_rowmajor(h, W, ldw, M, NC, transp, &fz, lu_block_width(size, true)));
   EAMD_CHECK(transpose_copy<double, double>(s, W, ldw, NC, M, A, lda));
   void* cnt = nullptr;
   EAMD_CHECK(h->ws.get(kWsAux2, 256, &cnt));

@@ -15,6 +15,8 @@
 // Workgroups are remapped so that consecutive tiles share an XCD (the
 // dispatcher deals workgroups round-robin over the 8 XCDs, each with its own
 // L2) and swizzled in groups of 8 tile-rows for L2 reuse of B.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -39,6 +41,30 @@ __device__ __forceinline__ void tile_coords(int bid, int nblocks, int tiles_m, i
   int r = logical - g * group;
   tm = first_m + r % gm;
   tn = r / gm;
+}
+
+// Tile `idx` of the lower trapezoid {(tr, tc): tc <= tr < tr_n, tc < tc_n} of
+// a tr_n x tc_n tile grid, row by row: the triangle tr < T = min(tr_n, tc_n)
+// first, then full rows of tc_n tiles.
+__device__ __forceinline__ void tri_tile(int idx, int tr_n, int tc_n, int& tr, int& tc) {
+  const int T = min(tr_n, tc_n), tri = T * (T + 1) / 2;
+  if (idx < tri) {
+    int r = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    if ((r + 1) * (r + 2) / 2 <= idx) ++r;
+    if (r * (r + 1) / 2 > idx) --r;
+    tr = r;
+    tc = idx - r * (r + 1) / 2;
+  } else {
+    const int q = idx - tri;
+    tr = T + q / tc_n;
+    tc = q - (q / tc_n) * tc_n;
+  }
+}
+
+// Number of tiles of that trapezoid.
+inline int tri_tiles(int tr_n, int tc_n) {
+  const int T = std::min(tr_n, tc_n);
+  return T * (T + 1) / 2 + (tr_n > T ? (tr_n - T) * tc_n : 0);
 }
 
 template <bool TA, bool TB>
@@ -115,20 +141,49 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(int m, int n, int k, doub
 
   const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
   int tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  if (MODE == 0) {
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  } else {
+    // the grid holds only the tiles that meet the triangle; each XCD takes
+    // a contiguous run of them (equal shares of the work)
+    const int bid = blockIdx.x, nblocks = gridDim.x;
+    const int xcd = bid & 7, loc = bid >> 3, per = nblocks >> 3, rem = nblocks & 7;
+    const int logical =
+        (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+    if (MODE == 1) tri_tile(logical, tiles_m, tiles_n, tm, tn);
+    else tri_tile(logical, tiles_n, tiles_m, tn, tm);
+  }
   const int m0 = tm * BM, n0 = tn * BN;
-  if (MODE == 1 && m0 + BM <= n0) return;  // tile entirely above the diagonal
-  if (MODE == 2 && n0 + BN <= m0) return;  // tile entirely below the diagonal
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int l16 = lane & 15, lk = lane >> 4;
 
+  // With beta == 1 and alpha == +-1 (the trailing updates of the blocked
+  // factorizations) the accumulators start from C / alpha, loaded before
+  // the first K slab so the C tile's latency overlaps the operand loads
+  // instead of following the last MFMA; the epilogue then only stores
+  // alpha * acc (both scalings exact).
+  const bool cinit = beta == 1.0 && (alpha == 1.0 || alpha == -1.0);
   v4d acc[4][4];
+  if (cinit) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+      const int gi = m0 + wm + i * 16 + l16;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn + j * 16 + lk + 4 * r;
+          acc[i][j][r] = (gi < m && gj < n) ? alpha * C[(int64_t)gj * ldc + gi] : 0.0;
+        }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  }
 
   const int ktiles = (k + BK - 1) / BK;
   double ra[8], rb[8];
@@ -178,7 +233,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(int m, int n, int k, doub
         if (MODE == 2 && gi > gj) continue;
         double* c = C + (int64_t)gj * ldc + gi;
         double v = alpha * acc[i][j][r];
-        if (beta != 0.0) v += beta * *c;
+        if (!cinit && beta != 0.0) v += beta * *c;
         *c = v;
       }
   }
@@ -188,7 +243,8 @@ template <bool TA, bool TB, int MODE>
 void launch_t(hipStream_t s, int m, int n, int k, double alpha, const double* A, int64_t lda,
               const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
               const int* abort_flag) {
-  int tiles = ceil_div(m, BM) * ceil_div(n, BN);
+  const int tm = ceil_div(m, BM), tn = ceil_div(n, BN);
+  const int tiles = MODE == 0 ? tm * tn : (MODE == 1 ? tri_tiles(tm, tn) : tri_tiles(tn, tm));
   hipLaunchKernelGGL((dgemm_kernel<TA, TB, MODE>), dim3(tiles), dim3(256), 0, s, m, n, k, alpha,
                      A, lda, B, ldb, beta, C, ldc, abort_flag);
 }

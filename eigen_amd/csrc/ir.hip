@@ -31,7 +31,7 @@ int residual_dd(eamd_handle* h, int n, int nrhs, const double* A, int64_t lda, c
 int matrix_inf_norm(eamd_handle* h, int n, const double* A, int64_t lda, double* out_dev);
 int inf_norms(eamd_handle* h, int n, int nrhs, const double* R, int64_t ldr, const double* X,
               int64_t ldx, const double* B, int64_t ldb, double* out_dev);
-int lu_block_width(int64_t n);
+int lu_block_width(int64_t n, bool fp64);
 
 namespace {
 
@@ -295,7 +295,7 @@ int dgesv_ir_impl(eamd_handle* h, int64_t n64, int64_t nrhs64, const double* A, 
   hipStream_t s = h->stream;
   const bool fp64 = o->factor_precision == EAMD_PREC_FP64;
   const int64_t ldw = ((n + 63) / 64) * 64;
-  const int nb = o->nb > 0 ? o->nb : lu_block_width(n);
+  const int nb = o->nb > 0 ? o->nb : lu_block_width(n, fp64);
   auto t0 = std::chrono::steady_clock::now();
   EAMD_HIP(hipStreamSynchronize(s));
   t0 = std::chrono::steady_clock::now();

@@ -1,13 +1,19 @@
 // llt.hip -- blocked right-looking Cholesky (lower) in FP64.
 //
 // Replaces llt_inplace<Scalar,Lower>::blocked (Eigen/src/Cholesky/LLT.h:330-360)
-// and ::unblocked (:304-328).  Per block column of width nb:
-//   potf2_kernel   the nb x nb diagonal block is factored by one workgroup in
-//                  LDS (x <= 0 fails at that column, NaN passes, as in the
-//                  reference's `if (x <= RealScalar(0)) return k`);
-//   trsm_rlt_kernel A21 <- A21 * L11^{-T}, one workgroup per 32 rows;
-//   dgemm (lower)  A22 -= A21 * A21^T on MFMA, tiles above the diagonal
-//                  skipped (the SYRK "bottleneck" of LLT.h:357).
+// and ::unblocked (:304-328).  Per block column of width nb (<= 128):
+//   potf2_inv_kernel   the nb x nb diagonal block is factored by one
+//                      workgroup in LDS (x <= 0 fails at that column, NaN
+//                      passes, as in the reference's
+//                      `if (x <= RealScalar(0)) return k`), which also forms
+//                      L11^-1;
+//   skinny_nt_kernel<0> A21 <- A21 * L11^{-T} as the MFMA product A21 L11^-T
+//                      (16 rows per workgroup);
+//   skinny_nt_kernel<1> the next block column's A22 -= A21 A21^T (the look-
+//                      ahead's critical path), and
+//   dgemm (lower)      the rest of A22 -= A21 * A21^T on MFMA, only the tiles
+//                      that meet the lower triangle launched (the SYRK
+//                      "bottleneck" of LLT.h:357).
 // A failure is recorded on the device; later kernels of the same call see
 // it and return early, so no host synchronisation per block is needed.
 #include "common.h"
@@ -19,65 +25,114 @@ namespace {
 
 constexpr int kMaxNb = 128;
 
+// 1 / d from the hardware reciprocal and two Newton steps (within an ulp
+// of the division for finite d > 0; d = inf gives 0 and NaN gives NaN, as
+// the division; d = 0 only occurs after the column has already failed).
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return __builtin_isinf(d) ? 0.0 : r;
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const long long bits = __double_as_longlong(x);
+  return __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(bits >> 32), l) << 32) |
+                              (unsigned)__builtin_amdgcn_readlane((int)bits, l));
+}
+
+// One 16 x 16 f64 MFMA tile product over LDS operands, accumulated into
+// acc: lane (lk = lane / 16, l16 = lane % 16) supplies F(l16, k) and
+// G(k, l16) for k = 4 q + lk, and register r of acc holds D(lk + 4 r, l16)
+// (the v_mfma_f64_16x16x4f64 layout).
+// kdim is a multiple of 8; two accumulator chains halve the dependent
+// MFMA latency.
+template <typename FF, typename GG>
+__device__ __forceinline__ void tile16(v4d& acc, int kdim, int l16, int lk, FF F, GG G) {
+  v4d acc2 = {0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < kdim; q += 8) {
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(F(l16, q + lk), G(q + lk, l16), acc, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(F(l16, q + 4 + lk), G(q + 4 + lk, l16), acc2, 0,
+                                                0, 0);
+  }
+  acc += acc2;
+}
+
 // Factor the bs x bs (bs <= 128) lower block at A (column-major, lda) in
-// LDS, in 32-column sub-blocks: wave 0 factors the 32 x 32 diagonal
-// sub-block with one row per lane in registers (broadcasts by readlane, no
-// barrier), the rows below solve against it one row per thread, and all
-// threads update the trailing lower part.  fail[0] keeps the first failing
-// absolute column (k0 + local) or -1: a diagonal x <= 0 fails, a NaN passes,
-// as in the reference's `if (x <= RealScalar(0)) return k` (LLT.h:316).
-__global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int64_t lda, int bs,
-                                                    int k0, int* __restrict__ fail) {
+// LDS and also form L^-1 there; L goes back to A, L^-1 (bsp x bsp, zero
+// above the diagonal, identity on the padding) to Li (column-major, ld
+// kMaxNb).  Factorization in 32-column sub-blocks: wave 0 factors the 32 x 32
+// diagonal sub-block with one row per lane in registers (each column is
+// broadcast through LDS, one division per column: the column is scaled by
+// 1 / L_cc), one thread per row below solves against it, and the trailing
+// lower part takes the rank-32 update as 16 x 16 MFMA tiles.  The inverse:
+// the four 32 x 32 diagonal blocks are inverted by one wave each (lane j
+// forms column j), then block row i of L^-1 is X_ij = -X_ii (sum over
+// k = j..i-1 of L_ik X_kj), block rows in order, each product on MFMA.
+// fail[0] keeps the first failing absolute column (k0 + local) or -1: a
+// diagonal x <= 0 fails, a NaN passes, as in the reference's
+// `if (x <= RealScalar(0)) return k` (LLT.h:316).
+__global__ __launch_bounds__(256) void potf2_inv_kernel(double* __restrict__ A, int64_t lda,
+                                                        int bs, int k0, int* __restrict__ fail,
+                                                        double* __restrict__ Li) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* S = reinterpret_cast<double*>(smem_raw);  // bsp x (bsp+1), column-major, padded
+  // column-major, leading dimension a compile-time constant so every LDS
+  // offset of the unrolled loops is an immediate
+  constexpr int ld = kMaxNb + 1;
+  double* S = reinterpret_cast<double*>(smem_raw);  // kMaxNb x ld
   const int bsp = (bs + 31) & ~31;
-  const int ld = bsp + 1;
+  double* Tm = S + kMaxNb * ld;  // up to three 32 x 32 temporaries, ld 33
   __shared__ int s_fail;
-  __shared__ double s_dinv[32];  // 1 / L_cc of the current diagonal sub-block
+  __shared__ double s_dinv[kMaxNb];  // 1 / L_cc
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lk = lane >> 4;
   if (*fail >= 0) return;
-  // padding: identity on the diagonal, zero elsewhere (factors to itself)
-  for (int e = tid; e < bsp * bsp; e += 256) {
-    const int j = e / bsp, i = e - j * bsp;
-    double v = 0.0;
-    if (i < bs && j < bs) v = (i >= j) ? A[i + (int64_t)j * lda] : 0.0;
-    else if (i == j) v = 1.0;
-    S[i + j * ld] = v;
+  auto Sx = [&](int r, int c) -> double& { return S[r + c * ld]; };
+  {
+    // thread: row tid % 128 of columns tid / 128 + 2 q; every load is
+    // requested before the first store (one memory round trip)
+    const int i = tid & (kMaxNb - 1), jb = tid >> 7;
+    double v[kMaxNb / 2];
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 2; ++q) {
+      const int j = jb + 2 * q;
+      v[q] = (i < bs && j < bs) ? (i >= j ? A[i + (int64_t)j * lda] : 0.0) : (i == j ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 2; ++q) Sx(i, jb + 2 * q) = v[q];
   }
   if (tid == 0) s_fail = -1;
   __syncthreads();
   for (int kb = 0; kb < bsp; kb += 32) {
-    // ---- 32 x 32 diagonal sub-block on wave 0, row kb+lane in x[]
     if (wave == 0) {
+      const int li = lane & 31, i = kb + li;
       double x[32];
-      const int i = kb + (lane & 31);
 #pragma unroll
-      for (int c = 0; c < 32; ++c) x[c] = S[i + (kb + c) * ld];
+      for (int c = 0; c < 32; ++c) x[c] = Sx(i, kb + c);
       int f = -1;
+      double myr = 0.0;  // 1 / L_cc of this lane's row c = li
+      double diag = readlane_f64(x[0], 0);
 #pragma unroll
       for (int c = 0; c < 32; ++c) {
-        const double diag = __longlong_as_double(
-            ((long long)__builtin_amdgcn_readlane((int)(__double_as_longlong(x[c]) >> 32), c) << 32) |
-            (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(x[c]), c));
-        if (f < 0 && diag <= 0.0) f = c;
-        const double d = sqrt(diag);
-        x[c] = (lane & 31) == c ? d : ((lane & 31) > c ? x[c] / d : x[c]);
+        f = (f < 0 && diag <= 0.0) ? c : f;
+        const double d = sqrt(diag), r = recip(d);
+        x[c] = li == c ? d : (li > c ? x[c] * r : x[c]);
+        myr = li == c ? r : myr;
+        if (c < 31) {
+          // the next column first (its diagonal is the next step's input),
+          // then the rest of the rank-1 update, column c read lane by lane
+          // (lanes above the diagonal carry values that are never stored)
+          x[c + 1] -= x[c] * readlane_f64(x[c], c + 1);
+          diag = readlane_f64(x[c + 1], c + 1);
 #pragma unroll
-        for (int j = c + 1; j < 32; ++j) {
-          const long long bits = __double_as_longlong(x[c]);
-          const double ljc = __longlong_as_double(
-              ((long long)__builtin_amdgcn_readlane((int)(bits >> 32), j) << 32) |
-              (unsigned)__builtin_amdgcn_readlane((int)bits, j));
-          if ((lane & 31) >= j) x[j] -= x[c] * ljc;
+          for (int j = c + 2; j < 32; ++j) x[j] -= x[c] * readlane_f64(x[c], j);
         }
       }
       if (lane < 32) {
+        s_dinv[kb + li] = myr;
 #pragma unroll
         for (int c = 0; c < 32; ++c)
-          if (c <= lane) S[i + (kb + c) * ld] = x[c];
-#pragma unroll
-        for (int c = 0; c < 32; ++c)
-          if (c == lane) s_dinv[c] = 1.0 / x[c];
+          if (c <= li) Sx(i, kb + c) = x[c];
       }
       if (lane == 0 && f >= 0) s_fail = kb + f;
     }
@@ -85,57 +140,48 @@ __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int6
     if (s_fail >= 0) break;
     const int below = bsp - kb - 32;
     if (below <= 0) break;
-    // ---- rows below: y L11^T = s, one row per thread
     if (tid < below) {
       const int i = kb + 32 + tid;
       double y[32];
 #pragma unroll
-      for (int c = 0; c < 32; ++c) y[c] = S[i + (kb + c) * ld];
+      for (int c = 0; c < 32; ++c) y[c] = Sx(i, kb + c);
+      // row c of L is read one step ahead (the LDS reads of a step are in
+      // flight while the previous step computes); four partial sums, the
+      // newest term (q = c - 1) last, so only it waits for y[c - 1]
+      double lr[2][32];
 #pragma unroll
       for (int c = 0; c < 32; ++c) {
-        double v = y[c];
+        if (c == 0) {
 #pragma unroll
-        for (int q = 0; q < c; ++q) v -= y[q] * S[kb + c + (kb + q) * ld];
-        y[c] = v * s_dinv[c];
+          for (int q = 0; q < 1; ++q) lr[1][q] = Sx(kb + 1, kb + q);
+        } else if (c < 31) {
+#pragma unroll
+          for (int q = 0; q <= c; ++q) lr[(c + 1) & 1][q] = Sx(kb + c + 1, kb + q);
+        }
+        double v[4] = {y[c], 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < c; ++q) v[(c - 1 - q) & 3] -= y[q] * lr[c & 1][q];
+        y[c] = ((v[0] + v[1]) + (v[2] + v[3])) * s_dinv[kb + c];
       }
 #pragma unroll
-      for (int c = 0; c < 32; ++c) S[i + (kb + c) * ld] = y[c];
+      for (int c = 0; c < 32; ++c) Sx(i, kb + c) = y[c];
     }
     __syncthreads();
-    // ---- trailing lower update: S[i][j] -= sum_q S[i][kb+q] S[j][kb+q], in
-    // 4 x 4 blocks (bi >= bj) so each loaded value serves four products
-    const int t0 = kb + 32, nb4 = below / 4;
-    for (int e = tid; e < nb4 * (nb4 + 1) / 2; e += 256) {
-      // e -> (bi, bj) with bi >= bj: bi(bi+1)/2 <= e < (bi+1)(bi+2)/2
-      int bi = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-      if ((bi + 1) * (bi + 2) / 2 <= e) ++bi;
-      if (bi * (bi + 1) / 2 > e) --bi;
-      const int bj = e - bi * (bi + 1) / 2;
-      const int i0 = t0 + 4 * bi, j0 = t0 + 4 * bj;
-      double acc[4][4];
+    // trailing lower update on 16 x 16 tiles (ti >= tj), one wave per tile
+    const int t0 = kb + 32, T16 = below / 16;
+    for (int t = wave; t < T16 * (T16 + 1) / 2; t += 4) {
+      int ti = 0;
+      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+      const int tj = t - ti * (ti + 1) / 2;
+      const int i0 = t0 + 16 * ti, j0 = t0 + 16 * tj;
+      v4d acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = Sx(i0 + lk + 4 * r, j0 + l16);
+      tile16(acc, 32, l16, lk, [&](int a, int k) { return -Sx(i0 + a, kb + k); },
+             [&](int k, int b) { return Sx(j0 + b, kb + k); });
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-#pragma unroll 4
-      for (int q = 0; q < 32; ++q) {
-        const double* col = S + (kb + q) * ld;
-        double a[4], b[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          a[r] = col[i0 + r];
-          b[r] = col[j0 + r];
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (bi > bj || r >= c) S[i0 + r + (j0 + c) * ld] -= acc[r][c];
+        if (ti > tj || lk + 4 * r >= l16) Sx(i0 + lk + 4 * r, j0 + l16) = acc[r];
     }
     __syncthreads();
   }
@@ -143,74 +189,154 @@ __global__ __launch_bounds__(256) void potf2_kernel(double* __restrict__ A, int6
     if (tid == 0) *fail = k0 + s_fail;
     return;
   }
-  for (int e = tid; e < bs * bs; e += 256) {
-    const int j = e / bs, i = e - j * bs;
-    if (i >= j) A[i + (int64_t)j * lda] = S[i + j * ld];
-  }
-}
-
-// X (m x bs, column-major) <- X * L^{-T}, L lower bs x bs (column-major):
-// row r of X solves y L^T = x_r.  One workgroup per 32 rows, X block in LDS;
-// per 32-column block of L: one thread per row solves the block from
-// registers (L broadcast from LDS), then 8 threads per row update the later
-// columns.
-__global__ __launch_bounds__(256) void trsm_rlt_kernel(const double* __restrict__ L, int64_t ldl,
-                                                       int bs, double* __restrict__ X,
-                                                       int64_t ldx, int m,
-                                                       const int* __restrict__ fail) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr int RB = 32;
-  const int bsp = (bs + 31) & ~31;
-  double* Xs = reinterpret_cast<double*>(smem_raw);  // [col][row], bsp x (RB+1)
-  double* Lp = Xs + (size_t)bsp * (RB + 1);          // [row j][q], bsp x 33
-  const int tid = threadIdx.x;
-  if (*fail >= 0) return;
-  const int r0 = blockIdx.x * RB;
-  const int rr = min(RB, m - r0);
-  const int r = tid & (RB - 1), qd = tid >> 5;  // 8 threads per row
-  for (int e = tid; e < RB * bsp; e += 256) {
-    const int c = e / RB, rw = e - c * RB;
-    Xs[rw + c * (RB + 1)] = (rw < rr && c < bs) ? X[(r0 + rw) + (int64_t)c * ldx] : 0.0;
-  }
-  for (int b = 0; b < bsp; b += 32) {
-    const int w = min(32, bs - b);
-    const int rest = bsp - b;  // rows j of the L panel (b .. bsp-1)
-    __syncthreads();
-    for (int e = tid; e < rest * 32; e += 256) {
-      const int j = e >> 5, q = e & 31;  // Lp[j][q] = L(b + j, b + q); padding: identity
-      double v = 0.0;
-      if (b + j < bs && q < w) v = L[(b + j) + (int64_t)(b + q) * ldl];
-      else if (j == q) v = 1.0;
-      Lp[j * 33 + q] = v;
+  {
+    const int i = tid & (kMaxNb - 1), jb = tid >> 7;
+    double v[kMaxNb / 2];
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 2; ++q) v[q] = Sx(i, jb + 2 * q);
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 2; ++q) {
+      const int j = jb + 2 * q;
+      if (i < bs && j < bs && i >= j) A[i + (int64_t)j * lda] = v[q];
     }
-    __syncthreads();
-    if (qd == 0) {
-      double y[32];
+  }
+  // ---- L^-1: diagonal 32 x 32 blocks, lane j of wave b forms column j of
+  // block b (entries above the block's diagonal become zero)
+  const int nbk = bsp / 32;
+  if (wave < nbk) {
+    const int b0 = wave * 32, j = lane & 31;
+    double x[32];
 #pragma unroll
-      for (int c = 0; c < 32; ++c) y[c] = Xs[r + (b + c) * (RB + 1)];
+    for (int i = 0; i < 32; ++i) {
+      double sp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int c = 0; c < 32; ++c) {
-        double v = y[c];
-#pragma unroll
-        for (int q = 0; q < c; ++q) v -= y[q] * Lp[c * 33 + q];
-        y[c] = v / Lp[c * 33 + c];
-      }
-#pragma unroll
-      for (int c = 0; c < 32; ++c) Xs[r + (b + c) * (RB + 1)] = y[c];
+      for (int k = 0; k < i; ++k) sp[(i - 1 - k) & 3] += Sx(b0 + i, b0 + k) * x[k];
+      const double sacc = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+      const double di = s_dinv[b0 + i];
+      x[i] = i < j ? 0.0 : (i == j ? di : -sacc * di);
+      // keeps the next row's LDS reads from being hoisted (register bound)
+      asm volatile("" ::: "memory");
     }
-    __syncthreads();
-    for (int j = 32 + qd; j < rest; j += 8) {
-      double acc = 0.0;
-#pragma unroll 8
-      for (int q = 0; q < 32; ++q) acc += Xs[r + (b + q) * (RB + 1)] * Lp[j * 33 + q];
-      Xs[r + (b + j) * (RB + 1)] -= acc;
+    asm volatile("" ::: "memory");
+    if (lane < 32) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) Sx(b0 + i, b0 + j) = x[i];
     }
   }
   __syncthreads();
-  for (int e = tid; e < RB * bs; e += 256) {
-    const int c = e / RB, rw = e - c * RB;
-    if (rw < rr) X[(r0 + rw) + (int64_t)c * ldx] = Xs[rw + c * (RB + 1)];
+  // ---- off-diagonal blocks, block row by block row
+  for (int bi = 1; bi < nbk; ++bi) {
+    // T_j = sum_{k=j}^{bi-1} L_{bi,k} X_{k,j}, j < bi: 4 tiles per block
+    for (int t = wave; t < 4 * bi; t += 4) {
+      const int bj = t >> 2, ti = (t >> 1) & 1, tj = t & 1;
+      const int r0 = 32 * bi + 16 * ti, c0 = 32 * bj + 16 * tj, kk0 = 32 * bj;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      tile16(acc, 32 * bi - kk0, l16, lk, [&](int a, int k) { return Sx(r0 + a, kk0 + k); },
+             [&](int k, int b) { return Sx(kk0 + k, c0 + b); });
+      double* T = Tm + bj * 32 * 33;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) T[(16 * ti + lk + 4 * r) + (16 * tj + l16) * 33] = acc[r];
+    }
+    __syncthreads();
+    // X_{bi,j} = -X_{bi,bi} T_j
+    for (int t = wave; t < 4 * bi; t += 4) {
+      const int bj = t >> 2, ti = (t >> 1) & 1, tj = t & 1;
+      const double* T = Tm + bj * 32 * 33;
+      const int d0 = 32 * bi;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      tile16(acc, 32, l16, lk, [&](int a, int k) { return -Sx(d0 + 16 * ti + a, d0 + k); },
+             [&](int k, int b) { return T[k + (16 * tj + b) * 33]; });
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Sx(d0 + 16 * ti + lk + 4 * r, 32 * bj + 16 * tj + l16) = acc[r];
+    }
+    __syncthreads();
   }
+  {
+    const int i = tid & (kMaxNb - 1), jb = tid >> 7;
+    double v[kMaxNb / 2];
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 2; ++q) v[q] = i >= jb + 2 * q ? Sx(i, jb + 2 * q) : 0.0;
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 2; ++q) Li[i + (jb + 2 * q) * kMaxNb] = v[q];
+  }
+}
+
+// Products of a 16-row block of an m-row panel with the transpose of a
+// K x nc block B (K, nc <= 128), one workgroup per 16 rows:
+//   MODE 0, the panel TRSM X <- X L^{-T}: out(r, c) = sum_k X(r, k) Li(c, k)
+//     with Li = L^{-1} from potf2_inv_kernel (ld kMaxNb, zero above the
+//     diagonal), written over X;
+//   MODE 1, the next block column's update A22 -= A21 A21^T:
+//     C(r, c) -= sum_k A(r, k) A(c, k) on the lower part (r >= c) only.
+// The workgroup's 16 x K rows are staged in LDS in one round trip; wave w
+// computes columns [32 w, 32 w + 32) as two MFMA tiles with B read from
+// memory (L2), consecutive lanes on consecutive rows.  17 KB of LDS and
+// under 100 VGPRs, so a workgroup fits on a CU beside two GEMM workgroups
+// of the concurrent trailing update.
+template <int MODE>
+__global__ __launch_bounds__(256) void skinny_nt_kernel(const double* __restrict__ B, int64_t ldb,
+                                                        int K, const double* __restrict__ Ain,
+                                                        int64_t lda, double* __restrict__ C,
+                                                        int64_t ldc, int m, int nc,
+                                                        const int* __restrict__ fail) {
+  constexpr int RB = 16, LDA_S = RB + 1;
+  __shared__ double As[kMaxNb * LDA_S];  // As[k * 17 + r]
+  if (*fail >= 0) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * RB, c0 = wave * 32;
+  {
+    // thread: row tid % 16 of columns tid / 16 + 16 q
+    const int r = tid & (RB - 1), kb = tid >> 4;
+    const bool rin = r0 + r < m;
+    double v[kMaxNb / 16];
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 16; ++q) {
+      const int k = kb + 16 * q;
+      v[q] = (rin && k < K) ? Ain[(r0 + r) + (int64_t)k * lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxNb / 16; ++q) As[(kb + 16 * q) * LDA_S + r] = v[q];
+  }
+  const bool active = c0 < nc;
+  const int row = r0 + l16;
+  v4d acc[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      // MODE 1 starts from -C (the result is -acc)
+      const int col = c0 + 16 * u + lk + 4 * e;
+      acc[u][e] = (MODE == 1 && active && row < m && col < nc) ? -C[row + (int64_t)col * ldc]
+                                                                 : 0.0;
+    }
+  __syncthreads();
+  if (!active) return;
+  const double* b0p = B + min(c0 + l16, nc - 1);
+  const double* b1p = B + min(c0 + 16 + l16, nc - 1);
+  // K in chunks of 32 (eight MFMA steps, their 16 loads issued together)
+  for (int q0 = 0; q0 < K; q0 += 32) {
+#pragma unroll
+    for (int q = q0; q < q0 + 32; q += 4) {
+      const int k = q + lk;
+      const bool kin = k < K;
+      const double b0 = kin ? b0p[(int64_t)k * ldb] : 0.0;
+      const double b1 = kin ? b1p[(int64_t)k * ldb] : 0.0;
+      const double a0 = As[k * LDA_S + l16];
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1], 0, 0, 0);
+    }
+  }
+  // acc[u][e] = out(r0 + l16, c0 + 16 u + lk + 4 e)
+  if (row >= m) return;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int col = c0 + 16 * u + lk + 4 * e;
+      if (col >= nc || (MODE == 1 && row < col)) continue;
+      C[row + (int64_t)col * ldc] = MODE == 1 ? -acc[u][e] : acc[u][e];
+    }
 }
 
 __global__ void set_int_kernel(int* p, int v) {
@@ -359,12 +485,13 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
   hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, s, fail, -1);
   static bool attr = false;
   if (!attr) {
-    EAMD_HIP(hipFuncSetAttribute((const void*)potf2_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    EAMD_HIP(hipFuncSetAttribute((const void*)trsm_rlt_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    EAMD_HIP(hipFuncSetAttribute((const void*)potf2_inv_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024));
     attr = true;
   }
+  void* lip = nullptr;
+  EAMD_CHECK(h->ws.get(kWsInv, sizeof(double) * kMaxNb * kMaxNb, &lip));
+  double* Li = static_cast<double*>(lip);
   // One-panel look-ahead on two streams (no pivoting, so nothing waits on
   // a grid-wide exchange): the handle's stream factors panel k (POTF2 of the
   // diagonal block, then A21 L11^-T), updates the next panel's columns only
@@ -385,13 +512,13 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
     const int rs = n - k - bs;
     double* A11 = A + k + (int64_t)k * lda;
     const int bsp = (bs + 31) & ~31;
-    size_t lds = sizeof(double) * (size_t)bsp * (bsp + 1);
-    hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(256), lds, s, A11, lda, bs, k, fail);
+    const size_t lds = sizeof(double) * (kMaxNb * (kMaxNb + 1) + 3 * 32 * 33);
+    hipLaunchKernelGGL(potf2_inv_kernel, dim3(1), dim3(256), lds, s, A11, lda, bs, k, fail, Li);
     if (rs > 0) {
       double* A21 = A11 + bs;
-      size_t lds2 = sizeof(double) * (size_t)bsp * (32 + 1) * 2;
-      hipLaunchKernelGGL(trsm_rlt_kernel, dim3(ceil_div(rs, 32)), dim3(256), lds2, s, A11, lda,
-                         bs, A21, lda, rs, fail);
+      hipLaunchKernelGGL(skinny_nt_kernel<0>, dim3(ceil_div(rs, 16)), dim3(256), 0, s,
+                         (const double*)Li, (int64_t)kMaxNb, bs, (const double*)A21, lda, A21,
+                         lda, rs, bs, (const int*)fail);
       double* A22 = A21 + (int64_t)bs * lda;
       if (!sb) {
         EAMD_CHECK(dgemm_device(s, false, true, rs, rs, bs, -1.0, A21, lda, A21, lda, 1.0, A22,
@@ -403,8 +530,9 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
       // the next panel's columns were last written by the previous panel's
       // second-stream update
       if (it > 0) EAMD_HIP(hipStreamWaitEvent(s, evB[par ^ 1], 0));
-      EAMD_CHECK(dgemm_device(s, false, true, rs, n1, bs, -1.0, A21, lda, A21, lda, 1.0, A22,
-                              lda, 1, fail));
+      hipLaunchKernelGGL(skinny_nt_kernel<1>, dim3(ceil_div(rs, 16)), dim3(256), 0, s,
+                         (const double*)A21, lda, bs, (const double*)A21, lda, A22, lda, rs, n1,
+                         (const int*)fail);
       if (n2 > 0) {
         EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));
         side_used = true;

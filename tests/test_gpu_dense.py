@@ -52,16 +52,19 @@ def test_dgemm_shapes(gpu, m, n, k):
     assert np.all(np.abs(c - ref) <= bound)
 
 
+@pytest.mark.parametrize("alpha,beta", [(-1.5, 0.25), (-1.0, 1.0), (1.0, 1.0)])
 @pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
-def test_dgemm_transposes_alpha_beta(gpu, ta, tb):
+def test_dgemm_transposes_alpha_beta(gpu, ta, tb, alpha, beta):
+    """All transposes; beta = 1 with alpha = -1 or 1 takes the path whose
+    accumulators start from C (the factorizations' trailing updates)."""
     m, n, k = 300, 200, 150
     a = inputs.normal(1, *(m, k) if ta == "N" else (k, m))
     b = inputs.normal(2, *(k, n) if tb == "N" else (n, k))
     c0 = inputs.normal(3, m, n)
-    out = host(ea.gemm(dev(a), dev(b), dev(c0), alpha=-1.5, beta=0.25, transa=ta, transb=tb))
+    out = host(ea.gemm(dev(a), dev(b), dev(c0), alpha=alpha, beta=beta, transa=ta, transb=tb))
     opa = a if ta == "N" else a.T
     opb = b if tb == "N" else b.T
-    np.testing.assert_allclose(out, -1.5 * opa @ opb + 0.25 * c0, atol=1e-12)
+    np.testing.assert_allclose(out, alpha * opa @ opb + beta * c0, atol=1e-12)
 
 
 def test_dgemm_reference_fixture(gpu):
@@ -200,6 +203,29 @@ def test_llt_against_oracle(gpu, n):
     b = inputs.normal(4000 + n, n, 2)
     x = host(llt.solve(dev(b)))
     assert po.backward_error(a, x, b) <= 1e-14
+
+
+@pytest.mark.parametrize("kappa", [1e6, 1e12])
+def test_llt_ill_conditioned(gpu, kappa):
+    """A = Q diag(lambda) Q^T with lambda log-spaced from 1 to 1/kappa (n =
+    1000, eight block columns): the factor's residual ||A - L L^T||_F /
+    ||A||_F stays within 4x the oracle's (a backward-stable Cholesky reaches
+    a few 1e-16 whatever kappa), and the solve's backward error <= 1e-14."""
+    n = 1000
+    rng = np.random.default_rng(int(np.log10(kappa)))
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    a = (q * np.logspace(0, -np.log10(kappa), n)) @ q.T
+    a = np.asfortranarray((a + a.T) / 2)
+    llt = ea.LLT(dev(a))
+    assert llt.info() == ea.Success
+    L = np.tril(host(llt.matrixLLT()))
+    Lo, _ = po.llt(a)
+    Lo = np.tril(Lo)
+    res = np.linalg.norm(a - L @ L.T) / np.linalg.norm(a)
+    res_o = np.linalg.norm(a - Lo @ Lo.T) / np.linalg.norm(a)
+    assert res <= max(4 * res_o, 1e-15), (res, res_o)
+    b = inputs.normal(4100, n)
+    assert po.backward_error(a, host(llt.solve(dev(b))), b) <= 1e-14
 
 
 @pytest.mark.parametrize("nrhs", [5, 16, 21])
