@@ -131,42 +131,39 @@ __device__ __forceinline__ double block_max(double v, double* red, int lane, int
 }
 
 // Forward (unit lower, FORWARD=true) or backward (upper) substitution of the
-// nrhs columns y[r * kMaxRhs + q] against the factors in the row-major W.
-// Rows are processed in blocks of DB: the off-block part as row-times-vector
-// products (a wave per row, lanes over the row: coalesced), the diagonal
-// triangle by wave 0 with lanes over rows, staged through LDS.
+// nrhs columns y[r * kMaxRhs + q] against the factors in the row-major W,
+// right-looking in blocks of DB rows: the diagonal triangle is staged in
+// LDS (every load requested before the first store) and solved by wave 0
+// with lanes over rows; then every row beyond the block (after it going
+// forward, before it going backward) takes y[r] -= W[r][block] y[block],
+// one row per thread with the row segment's loads issued together.
 template <typename TF, int NT, bool FORWARD>
-__device__ void tri_solve(const TF* __restrict__ W, int n, int nrhs, TF* y, TF* diag, int tid) {
-  constexpr int NW = NT / 64;
+__device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int nrhs, TF* y,
+                                          TF* diag, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
   const int nblk = (n + DB - 1) / DB;
   for (int bb = 0; bb < nblk; ++bb) {
     const int blk = FORWARD ? bb : nblk - 1 - bb;
     const int b0 = blk * DB, bn = min(DB, n - b0);
-    // off-block part: columns [0, b0) (forward) or [b0 + bn, n) (backward)
-    const int t0 = FORWARD ? 0 : b0 + bn, t1 = FORWARD ? b0 : n;
-    for (int rr = wave; rr < bn; rr += NW) {
-      const int r = b0 + rr;
-      double acc[kMaxRhs];
+    {
+      // chunks of at most 16 loads per thread, each chunk's loads issued
+      // together (register pressure)
+      constexpr int PER = DB * DB / NT, CH = PER < 16 ? PER : 16;
 #pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
-      for (int t = t0 + lane; t < t1; t += 64) {
-        const double a = (double)W[(int64_t)r * n + t];
+      for (int c0 = 0; c0 < PER; c0 += CH) {
+        TF v[CH];
 #pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q)
-          if (q < nrhs) acc[q] += a * (double)y[t * kMaxRhs + q];
-      }
-#pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q)
-        if (q < nrhs) {
-          const double s = wave_sum(acc[q]);
-          if (lane == 0) y[r * kMaxRhs + q] -= (TF)s;
+        for (int q = 0; q < CH; ++q) {
+          const int e = tid + (c0 + q) * NT, r = e / DB, t = e - r * DB;
+          v[q] = (r < bn && t < bn) ? W[(int64_t)(b0 + r) * n + b0 + t] : TF(0);
         }
-    }
-    // stage the diagonal triangle: diag[r][t], r,t < bn
-    for (int e = tid; e < bn * bn; e += NT) {
-      const int r = e / bn, t = e - r * bn;
-      diag[r * (DB + 1) + t] = W[(int64_t)(b0 + r) * n + b0 + t];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int e = tid + (c0 + q) * NT, r = e / DB, t = e - r * DB;
+          diag[r * (DB + 1) + t] = v[q];
+        }
+        asm volatile("" ::: "memory");
+      }
     }
     __syncthreads();
     if (wave == 0) {
@@ -210,6 +207,35 @@ __device__ void tri_solve(const TF* __restrict__ W, int n, int nrhs, TF* y, TF* 
       }
     }
     __syncthreads();
+    const int r_lo = FORWARD ? b0 + bn : 0, r_hi = FORWARD ? n : b0;
+#pragma unroll 1
+    for (int r = r_lo + tid; r < r_hi; r += NT) {
+      const TF* __restrict__ wr = W + (int64_t)r * n + b0;
+      double acc[kMaxRhs];
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
+      // the segment in quarters of 16 (register pressure): a quarter's
+      // loads are issued together, the next quarter's after its use
+#pragma unroll
+      for (int h = 0; h < DB; h += 16) {
+        TF wv[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) wv[t] = h + t < bn ? wr[h + t] : TF(0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          if (h + t < bn) {
+#pragma unroll
+            for (int q = 0; q < kMaxRhs; ++q)
+              if (q < nrhs) acc[q] += (double)wv[t] * (double)y[(b0 + h + t) * kMaxRhs + q];
+          }
+          if ((t & 3) == 3) asm volatile("" ::: "memory");
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kMaxRhs; ++q)
+        if (q < nrhs) y[r * kMaxRhs + q] -= (TF)acc[q];
+    }
+    __syncthreads();
   }
 }
 
@@ -225,7 +251,7 @@ __device__ void tri_solve(const TF* __restrict__ W, int n, int nrhs, TF* y, TF* 
 // starts at wbase[n] + (pos - bstart[n]) n^2 (the plan lays the copies out
 // by size).
 template <typename TF, int NT, int NMAX>
-__global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
+__global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
     int count, int pos0, const int* __restrict__ list, const int* __restrict__ list_count,
     const int* __restrict__ order, const int64_t* __restrict__ wbase,
     const int* __restrict__ bstart, const int64_t* __restrict__ offA,
@@ -238,6 +264,7 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
   __shared__ TF smem[LuSmem<TF, NMAX>::kWords];
   __shared__ int piv[NMAX];
   __shared__ int perm[NMAX + 1];
+  __shared__ int s_tr[2 * PB], s_src[2 * PB], s_ntr;
   __shared__ double red_d[NW];
   __shared__ int red_i[NW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -254,20 +281,32 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
     TF* __restrict__ W =
         static_cast<TF*>(Wbase) + wbase[n] + (int64_t)(pos - bstart[n]) * n * n;
 
-    // ---- 1. W = A in row-major, TF (32x32 tiles through LDS)
+    // ---- 1. W = A in row-major, TF (64 x 64 tiles through LDS, a tile's
+    // loads all requested before its first store)
     {
-      TF* tile = smem;  // 32 x 33
-      const int nt = (n + 31) / 32;
+      constexpr int TS = NT >= 256 ? 64 : 32, PER = TS * TS / NT;
+      TF* tile = smem;  // TS x (TS + 1)
+      const int nt = (n + TS - 1) / TS;
       for (int tt = 0; tt < nt * nt; ++tt) {
         const int tr = tt / nt, tc = tt - tr * nt;
-        for (int e = tid; e < 1024; e += NT) {
-          const int i = e & 31, j = e >> 5, r = tr * 32 + i, c = tc * 32 + j;
-          tile[j * 33 + i] = (r < n && c < n) ? (TF)As[r + (int64_t)c * n] : TF(0);
+        TF v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * NT, i = e & (TS - 1), j = e / TS;
+          const int r = tr * TS + i, c = tc * TS + j;
+          v[q] = (r < n && c < n) ? (TF)As[r + (int64_t)c * n] : TF(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * NT;
+          tile[(e / TS) * (TS + 1) + (e & (TS - 1))] = v[q];
         }
         __syncthreads();
-        for (int e = tid; e < 1024; e += NT) {
-          const int j = e & 31, i = e >> 5, r = tr * 32 + i, c = tc * 32 + j;
-          if (r < n && c < n) W[(int64_t)r * n + c] = tile[j * 33 + i];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * NT, j = e & (TS - 1), i = e / TS;
+          const int r = tr * TS + i, c = tc * TS + j;
+          if (r < n && c < n) W[(int64_t)r * n + c] = tile[j * (TS + 1) + i];
         }
         __syncthreads();
       }
@@ -314,6 +353,35 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
           }
           asm volatile("" ::: "memory");
         }
+        // the panel's interchanges as one permutation of the rows they
+        // touch: slot L < pb holds row k0 + L, slot pb + c the pivot row of
+        // column c unless an earlier slot holds it; src is the row whose
+        // old content the slot's row receives
+        int row = lane < pb ? k0 + lane : (lane < 2 * pb ? piv[k0 + lane - pb] : -1);
+        int first = lane;
+        for (int j = 0; j < 2 * PB; ++j) {
+          const int rj = __builtin_amdgcn_readlane(row, j);
+          if (j < lane && rj == row && first == lane) first = j;
+        }
+        const bool live = row >= 0 && first == lane;
+        int src = row;
+        for (int c = 0; c < pb; ++c) {
+          const int b = piv[k0 + c];
+          const int sb = (int)__builtin_ctzll(__ballot(live && row == b));
+          const int va = __builtin_amdgcn_readlane(src, c);
+          const int vb = __builtin_amdgcn_readlane(src, sb);
+          if (lane == c) src = vb;
+          if (lane == sb) src = va;
+        }
+        const bool moved = live && src != row;
+        const unsigned long long mm = __ballot(moved);
+        const int idx = __builtin_amdgcn_mbcnt_hi((unsigned)(mm >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)mm, 0u));
+        if (moved) {
+          s_tr[idx] = row;
+          s_src[idx] = src;
+        }
+        if (lane == 0) s_ntr = __popcll(mm);
       }
       __syncthreads();
       // panel back to W
@@ -321,23 +389,23 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
         const int r = e / pb, c = e - r * pb;
         W[(int64_t)(k0 + r) * n + k0 + c] = Pn[r * (PB + 1) + c];
       }
-      // the panel's interchanges on the other columns; column n is the
-      // permutation vector
-      for (int col = tid; col <= n; col += NT) {
-        if (col >= k0 && col < k0 + pb) continue;
-        for (int c = 0; c < pb; ++c) {
-          const int a = k0 + c, b = piv[a];
-          if (b == a) continue;
-          if (col == n) {
-            const int t = perm[a];
-            perm[a] = perm[b];
-            perm[b] = t;
-          } else {
-            const TF t = W[(int64_t)a * n + col];
-            W[(int64_t)a * n + col] = W[(int64_t)b * n + col];
-            W[(int64_t)b * n + col] = t;
-          }
+      // the permutation on the other columns (every touched row read
+      // before any is written, one column per thread) and on perm
+      {
+        const int ntr = s_ntr;
+        for (int col = tid; col < n; col += NT) {
+          if (col >= k0 && col < k0 + pb) continue;
+          TF v[2 * PB];
+#pragma unroll
+          for (int q = 0; q < 2 * PB; ++q)
+            if (q < ntr) v[q] = W[(int64_t)s_src[q] * n + col];
+#pragma unroll
+          for (int q = 0; q < 2 * PB; ++q)
+            if (q < ntr) W[(int64_t)s_tr[q] * n + col] = v[q];
         }
+        const int pv = tid < ntr ? perm[s_src[tid]] : 0;
+        __syncthreads();
+        if (tid < ntr) perm[s_tr[tid]] = pv;
       }
       __syncthreads();
       const int T0 = k0 + pb, nr = n - T0;
@@ -358,31 +426,47 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
         }
       }
       __syncthreads();
-      // A22 -= L21 U12 on 16x16 tiles, one wave per tile
+      // A22 -= L21 U12 on 16x16 tiles: wave w takes tile rows w, w + NW,
+      // ...; a row's tiles in groups of TB, all their loads issued first
       const int l16 = lane & 15, lk = lane >> 4;
       const int tn = (nr + 15) / 16;
-      for (int tile = wave; tile < tn * tn; tile += NW) {
-        const int ti = tile / tn, tj = tile - ti * tn;
-        const int r0 = T0 + ti * 16, c0 = T0 + tj * 16;
-        const int col = c0 + l16;
-        AccT acc;
+      constexpr int TB = sizeof(TF) == 8 ? 4 : 8;
+      for (int ti = wave; ti < tn; ti += NW) {
+        const int r0 = T0 + ti * 16, arow = r0 + l16;
+        TF a[PB / 4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = r0 + acc_row<TF>(lane, r);
-          acc[r] = (row < n && col < n) ? W[(int64_t)row * n + col] : TF(0);
-        }
-        const int arow = r0 + l16;
+        for (int kk = 0; kk < PB / 4; ++kk)
+          a[kk] = arow < n ? -Pn[(arow - k0) * (PB + 1) + kk * 4 + lk] : TF(0);
+        for (int tj0 = 0; tj0 < tn; tj0 += TB) {
+          AccT acc[TB];
 #pragma unroll
-        for (int kk = 0; kk < PB / 4; ++kk) {
-          const int k = kk * 4 + lk;
-          const TF a = arow < n ? -Pn[(arow - k0) * (PB + 1) + k] : TF(0);
-          const TF b = col < n ? Us[k * (NMAX + 1) + col - T0] : TF(0);
-          acc = mfma16x4(a, b, acc);
-        }
+          for (int j = 0; j < TB; ++j) {
+            const int col = T0 + (tj0 + j) * 16 + l16;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = r0 + acc_row<TF>(lane, r);
-          if (row < n && col < n) W[(int64_t)row * n + col] = acc[r];
+            for (int r = 0; r < 4; ++r) {
+              const int row = r0 + acc_row<TF>(lane, r);
+              acc[j][r] = (tj0 + j < tn && row < n && col < n) ? W[(int64_t)row * n + col] : TF(0);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < TB; ++j) {
+            const int col = T0 + (tj0 + j) * 16 + l16;
+#pragma unroll
+            for (int kk = 0; kk < PB / 4; ++kk) {
+              const TF b = (tj0 + j < tn && col < n) ? Us[(kk * 4 + lk) * (NMAX + 1) + col - T0]
+                                                     : TF(0);
+              acc[j] = mfma16x4(a[kk], b, acc[j]);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < TB; ++j) {
+            const int col = T0 + (tj0 + j) * 16 + l16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = r0 + acc_row<TF>(lane, r);
+              if (tj0 + j < tn && row < n && col < n) W[(int64_t)row * n + col] = acc[j][r];
+            }
+          }
         }
       }
       __syncthreads();
@@ -392,27 +476,8 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
     TF* diag = smem;
     TF* y = smem + LuSmem<TF, NMAX>::kDiag;
     TF* rv = y + LuSmem<TF, NMAX>::kVec;
-    for (int e = tid; e < n * nrhs; e += NT) {
-      const int r = e / nrhs, q = e - r * nrhs;
-      y[r * kMaxRhs + q] = (TF)Bs[perm[r] + (int64_t)q * n];
-    }
-    __syncthreads();
-    tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
-    tri_solve<TF, NT, false>(W, n, nrhs, y, diag, tid);
-    for (int e = tid; e < n * nrhs; e += NT) {
-      const int q = e / n, r = e - q * n;
-      Xs[r + (int64_t)q * n] = (double)y[r * kMaxRhs + q];
-    }
-    __syncthreads();
-
-    // ||A||_inf and ||B_q||_inf
-    double arow = 0.0;
-    for (int r = tid; r < n; r += NT) {
-      double sacc = 0.0;
-      for (int t = 0; t < n; ++t) sacc += fabs(As[r + (int64_t)t * n]);
-      arow = fmax(arow, absinf(sacc));
-    }
-    const double anorm = block_max<NW>(arow, red_d, lane, wave);
+    // ||B_q||_inf; ||A||_inf comes with the first residual
+    double anorm = 0.0;
     double bnorm[kMaxRhs];
 #pragma unroll
     for (int q = 0; q < kMaxRhs; ++q) {
@@ -425,28 +490,69 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
     int it = 0, converged = 0;
     double prev = INFINITY;
     for (;; ++it) {
-      // R = B - A X in FP64 from the original A
+      // the first pass solves P B, the later ones the correction P R:
+      // X = U^-1 L^-1 P B, then X += U^-1 L^-1 P R (one call site of each
+      // triangular solve)
+      for (int e = tid; e < n * nrhs; e += NT) {
+        const int r = e / nrhs, q = e - r * nrhs;
+        y[r * kMaxRhs + q] = it == 0 ? (TF)Bs[perm[r] + (int64_t)q * n] : rv[perm[r] * kMaxRhs + q];
+      }
+      __syncthreads();
+      tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
+      tri_solve<TF, NT, false>(W, n, nrhs, y, diag, tid);
+      for (int e = tid; e < n * nrhs; e += NT) {
+        const int q = e / n, r = e - q * n;
+        const double d = (double)y[r * kMaxRhs + q];
+        Xs[r + (int64_t)q * n] = it == 0 ? d : Xs[r + (int64_t)q * n] + d;
+      }
+      __syncthreads();
+      // X staged in LDS as FP64 (in the diagonal-block area, which the
+      // solves do not use meanwhile)
+      double* xs = reinterpret_cast<double*>(diag);
+      for (int e = tid; e < n * nrhs; e += NT) {
+        const int q = e / n, r = e - q * n;
+        xs[r * kMaxRhs + q] = Xs[r + (int64_t)q * n];
+      }
+      __syncthreads();
+      // R = B - A X in FP64 from the original A, one row per thread, 8
+      // columns of A per step with their loads issued together; the first
+      // pass also sums |A| per row for ||A||_inf
       double rmax[kMaxRhs], xmax[kMaxRhs];
 #pragma unroll
       for (int q = 0; q < kMaxRhs; ++q) { rmax[q] = 0.0; xmax[q] = 0.0; }
+      double arow = 0.0;
+#pragma unroll 1
       for (int r = tid; r < n; r += NT) {
         double acc[kMaxRhs];
 #pragma unroll
         for (int q = 0; q < kMaxRhs; ++q) acc[q] = q < nrhs ? Bs[r + (int64_t)q * n] : 0.0;
-        for (int t = 0; t < n; ++t) {
-          const double a = As[r + (int64_t)t * n];
+        double asum = 0.0;
+        for (int t0 = 0; t0 < n; t0 += 8) {
+          double av[8];
 #pragma unroll
-          for (int q = 0; q < kMaxRhs; ++q)
-            if (q < nrhs) acc[q] -= a * Xs[t + (int64_t)q * n];
+          for (int j = 0; j < 8; ++j) av[j] = t0 + j < n ? As[r + (int64_t)(t0 + j) * n] : 0.0;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            if (t0 + j < n) {
+              asum += fabs(av[j]);
+#pragma unroll
+              for (int q = 0; q < kMaxRhs; ++q)
+                if (q < nrhs) acc[q] -= av[j] * xs[(t0 + j) * kMaxRhs + q];
+            }
+            // bounds the X values read ahead (register pressure)
+            if ((j & 3) == 3) asm volatile("" ::: "memory");
+          }
         }
+        arow = fmax(arow, absinf(asum));
 #pragma unroll
         for (int q = 0; q < kMaxRhs; ++q)
           if (q < nrhs) {
             rv[r * kMaxRhs + q] = (TF)acc[q];
             rmax[q] = fmax(rmax[q], absinf(acc[q]));
-            xmax[q] = fmax(xmax[q], absinf(Xs[r + (int64_t)q * n]));
+            xmax[q] = fmax(xmax[q], absinf(xs[r * kMaxRhs + q]));
           }
       }
+      if (it == 0) anorm = block_max<NW>(arow, red_d, lane, wave);
       double worst = 0.0;
       bool bad = false;
 #pragma unroll
@@ -463,19 +569,6 @@ __global__ __launch_bounds__(NT) void lu_ir_batch_kernel(
       // stop on the iteration limit, or when a step gains less than 2x
       if (it >= kMaxIter || !(worst < 0.5 * prev)) break;
       prev = worst;
-      // correction: dX = U^-1 L^-1 P R, X += dX
-      for (int e = tid; e < n * nrhs; e += NT) {
-        const int r = e / nrhs, q = e - r * nrhs;
-        y[r * kMaxRhs + q] = rv[perm[r] * kMaxRhs + q];
-      }
-      __syncthreads();
-      tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
-      tri_solve<TF, NT, false>(W, n, nrhs, y, diag, tid);
-      for (int e = tid; e < n * nrhs; e += NT) {
-        const int q = e / n, r = e - q * n;
-        Xs[r + (int64_t)q * n] += (double)y[r * kMaxRhs + q];
-      }
-      __syncthreads();
     }
     if (tid == 0) {
       if (converged) {
