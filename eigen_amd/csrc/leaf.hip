@@ -307,6 +307,7 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   __shared__ T rowjL[2][LW];  // row j, when this workgroup owns it
   __shared__ unsigned stage[STAGE];
   __shared__ int s_abort;
+  __shared__ int s_win[2];  // the exchange's winning row, by column parity
   // bookkeeping (workgroup 0): the leaf's pivots and the panel list's rows
   __shared__ int s_piv[LW];
   __shared__ __align__(16) int s_pp[256 + 4];
@@ -448,23 +449,31 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
         if (tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return false;
       }
-      // ---- resolve the winner (every thread, G headers from LDS)
-      double wb = -2.0;
-      int wi = 0x7fffffff, wslot = 0;
-      for (int s2 = 0; s2 < G; ++s2) {
-        const unsigned* hh = &stage[s2 * SW];
-        const unsigned idx = hh[2];
-        if (idx == 0xffffffffu) continue;
-        const double val = get_value<double>(hh);
-        const double sc = score_of(val, (int)idx == j);
-        if (beats(sc, (int)idx, wb, wi)) { wb = sc; wi = (int)idx; wslot = s2; }
-      }
-      p = wi;
-      if (tid < LW) {
-        urow[par][tid] = get_value<T>(&stage[wslot * SW + 4 + tid * NW]);
-        rowj[par][tid] = get_value<T>(&stage[G * SW + 4 + tid * NW]);
+      // ---- resolve the winner: wave 0, lane s holding slot s's candidate;
+      // the highest score, then the lowest row among those (the `beats`
+      // order), by two wave reductions
+      if (wave == 0) {
+        double sc = -2.0;
+        int idx = 0x7fffffff;
+        if (lane < G) {
+          const unsigned* hh = &stage[lane * SW];
+          const unsigned id = hh[2];
+          if (id != 0xffffffffu) {
+            sc = score_of(get_value<double>(hh), (int)id == j);
+            idx = (int)id;
+          }
+        }
+        const double mx = wave_max(sc);
+        const int wi = (int)-wave_max(sc == mx ? -(double)idx : -4294967296.0);
+        const int wslot = __ffsll((long long)__ballot(sc == mx && idx == wi)) - 1;
+        if (lane < LW) {
+          urow[par][lane] = get_value<T>(&stage[wslot * SW + 4 + lane * NW]);
+          rowj[par][lane] = get_value<T>(&stage[G * SW + 4 + lane * NW]);
+        }
+        if (lane == 0) s_win[par] = wi;
       }
       __syncthreads();
+      p = s_win[par];
       up = urow[par];
       rp = rowj[par];
     }
