@@ -314,17 +314,38 @@ __global__ __launch_bounds__(256) void skinny_nt_kernel(const double* __restrict
   if (!active) return;
   const double* b0p = B + min(c0 + l16, nc - 1);
   const double* b1p = B + min(c0 + 16 + l16, nc - 1);
-  // K in chunks of 32 (eight MFMA steps, their 16 loads issued together)
-  for (int q0 = 0; q0 < K; q0 += 32) {
+  if constexpr (MODE == 0) {
+    // every B value of the wave (2 x K / 4 per lane, from the 128 KB L^-1)
+    // requested at once: one memory round trip (14 vs 19 us per launch)
+    double bv[2][kMaxNb / 4];
 #pragma unroll
-    for (int q = q0; q < q0 + 32; q += 4) {
-      const int k = q + lk;
+    for (int s = 0; s < kMaxNb / 4; ++s) {
+      const int k = 4 * s + lk;
       const bool kin = k < K;
-      const double b0 = kin ? b0p[(int64_t)k * ldb] : 0.0;
-      const double b1 = kin ? b1p[(int64_t)k * ldb] : 0.0;
-      const double a0 = As[k * LDA_S + l16];
-      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1], 0, 0, 0);
+      bv[0][s] = kin ? b0p[(int64_t)k * ldb] : 0.0;
+      bv[1][s] = kin ? b1p[(int64_t)k * ldb] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxNb / 4; ++s) {
+      const double a0 = As[(4 * s + lk) * LDA_S + l16];
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[0][s], a0, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[1][s], a0, acc[1], 0, 0, 0);
+    }
+  } else {
+    // K in chunks of 32 (eight MFMA steps, their 16 loads issued together):
+    // fewer registers, so the workgroup still fits beside the concurrent
+    // SYRK's (the all-at-once form ran slower here, 52 vs 45 us)
+    for (int q0 = 0; q0 < K; q0 += 32) {
+#pragma unroll
+      for (int q = q0; q < q0 + 32; q += 4) {
+        const int k = q + lk;
+        const bool kin = k < K;
+        const double b0 = kin ? b0p[(int64_t)k * ldb] : 0.0;
+        const double b1 = kin ? b1p[(int64_t)k * ldb] : 0.0;
+        const double a0 = As[k * LDA_S + l16];
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1], 0, 0, 0);
+      }
     }
   }
   // acc[u][e] = out(r0 + l16, c0 + 16 u + lk + 4 e)
