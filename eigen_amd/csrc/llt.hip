@@ -278,7 +278,8 @@ __global__ __launch_bounds__(256) void skinny_nt_kernel(const double* __restrict
                                                         int K, const double* __restrict__ Ain,
                                                         int64_t lda, double* __restrict__ C,
                                                         int64_t ldc, int m, int nc,
-                                                        const int* __restrict__ fail) {
+                                                        const int* __restrict__ fail,
+                                                        double* __restrict__ Bc) {
   constexpr int RB = 16, LDA_S = RB + 1;
   __shared__ double As[kMaxNb * LDA_S];  // As[k * 17 + r]
   if (*fail >= 0) return;
@@ -357,6 +358,10 @@ __global__ __launch_bounds__(256) void skinny_nt_kernel(const double* __restrict
       const int col = c0 + 16 * u + lk + 4 * e;
       if (col >= nc || (MODE == 1 && row < col)) continue;
       C[row + (int64_t)col * ldc] = MODE == 1 ? -acc[u][e] : acc[u][e];
+      // MODE 0: the first kMaxNb rows of the result also go to the compact
+      // Bc (ld kMaxNb), the B operand of the next block column's update (a
+      // 128 KB block instead of 128 columns of the matrix)
+      if (MODE == 0 && row < kMaxNb) Bc[row + col * kMaxNb] = acc[u][e];
     }
 }
 
@@ -511,8 +516,9 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
     attr = true;
   }
   void* lip = nullptr;
-  EAMD_CHECK(h->ws.get(kWsInv, sizeof(double) * kMaxNb * kMaxNb, &lip));
-  double* Li = static_cast<double*>(lip);
+  EAMD_CHECK(h->ws.get(kWsInv, 2 * sizeof(double) * kMaxNb * kMaxNb, &lip));
+  double* Li = static_cast<double*>(lip);       // L11^-1 of the current block column
+  double* Bc = Li + kMaxNb * kMaxNb;            // its first 128 rows of A21 L11^-T
   // One-panel look-ahead on two streams (no pivoting, so nothing waits on
   // a grid-wide exchange): the handle's stream factors panel k (POTF2 of the
   // diagonal block, then A21 L11^-T), updates the next panel's columns only
@@ -539,7 +545,7 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
       double* A21 = A11 + bs;
       hipLaunchKernelGGL(skinny_nt_kernel<0>, dim3(ceil_div(rs, 16)), dim3(256), 0, s,
                          (const double*)Li, (int64_t)kMaxNb, bs, (const double*)A21, lda, A21,
-                         lda, rs, bs, (const int*)fail);
+                         lda, rs, bs, (const int*)fail, Bc);
       double* A22 = A21 + (int64_t)bs * lda;
       if (!sb) {
         EAMD_CHECK(dgemm_device(s, false, true, rs, rs, bs, -1.0, A21, lda, A21, lda, 1.0, A22,
@@ -552,8 +558,8 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
       // second-stream update
       if (it > 0) EAMD_HIP(hipStreamWaitEvent(s, evB[par ^ 1], 0));
       hipLaunchKernelGGL(skinny_nt_kernel<1>, dim3(ceil_div(rs, 16)), dim3(256), 0, s,
-                         (const double*)A21, lda, bs, (const double*)A21, lda, A22, lda, rs, n1,
-                         (const int*)fail);
+                         (const double*)Bc, (int64_t)kMaxNb, bs, (const double*)A21, lda, A22,
+                         lda, rs, n1, (const int*)fail, (double*)nullptr);
       if (n2 > 0) {
         EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));
         side_used = true;
