@@ -155,7 +155,7 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int n
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
           const int e = tid + (c0 + q) * NT, r = e / DB, t = e - r * DB;
-          v[q] = (r < bn && t < bn) ? W[(int64_t)(b0 + r) * n + b0 + t] : TF(0);
+          v[q] = (r < bn && t < bn) ? W[(b0 + r) * n + b0 + t] : TF(0);
         }
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
@@ -210,7 +210,7 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int n
     const int r_lo = FORWARD ? b0 + bn : 0, r_hi = FORWARD ? n : b0;
 #pragma unroll 1
     for (int r = r_lo + tid; r < r_hi; r += NT) {
-      const TF* __restrict__ wr = W + (int64_t)r * n + b0;
+      const TF* __restrict__ wr = W + r * n + b0;
       double acc[kMaxRhs];
 #pragma unroll
       for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
@@ -251,7 +251,13 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int n
 // starts at wbase[n] + (pos - bstart[n]) n^2 (the plan lays the copies out
 // by size).
 template <typename TF, int NT, int NMAX>
-__global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
+// Occupancy: the kernel is latency-bound per system, so throughput follows
+// the systems in flight.  The 256-thread FP32 instances are held to 128
+// VGPRs (four workgroups per CU): C5 84 ms against 110 ms at 160 VGPRs and
+// 132 ms at 256; at 128 the compiler keeps about 84 bytes of loop-invariant
+// values in scratch, reloaded a few times per system.  The others keep
+// their register budgets (no scratch).
+__global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 128 ? 1 : 2))) void lu_ir_batch_kernel(
     int count, int pos0, const int* __restrict__ list, const int* __restrict__ list_count,
     const int* __restrict__ order, const int64_t* __restrict__ wbase,
     const int* __restrict__ bstart, const int64_t* __restrict__ offA,
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
         for (int q = 0; q < PER; ++q) {
           const int e = tid + q * NT, i = e & (TS - 1), j = e / TS;
           const int r = tr * TS + i, c = tc * TS + j;
-          v[q] = (r < n && c < n) ? (TF)As[r + (int64_t)c * n] : TF(0);
+          v[q] = (r < n && c < n) ? (TF)As[r + c * n] : TF(0);
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -306,7 +312,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
         for (int q = 0; q < PER; ++q) {
           const int e = tid + q * NT, j = e & (TS - 1), i = e / TS;
           const int r = tr * TS + i, c = tc * TS + j;
-          if (r < n && c < n) W[(int64_t)r * n + c] = tile[j * (TS + 1) + i];
+          if (r < n && c < n) W[r * n + c] = tile[j * (TS + 1) + i];
         }
         __syncthreads();
       }
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       const int pb = min(PB, n - k0), R = n - k0;
       for (int e = tid; e < R * PB; e += NT) {
         const int r = e / PB, c = e - r * PB;
-        Pn[r * (PB + 1) + c] = c < pb ? W[(int64_t)(k0 + r) * n + k0 + c] : TF(0);
+        Pn[r * (PB + 1) + c] = c < pb ? W[(k0 + r) * n + k0 + c] : TF(0);
       }
       __syncthreads();
       // the panel is factored by wave 0 alone: wave reductions and the
@@ -387,21 +393,29 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       // panel back to W
       for (int e = tid; e < R * pb; e += NT) {
         const int r = e / pb, c = e - r * pb;
-        W[(int64_t)(k0 + r) * n + k0 + c] = Pn[r * (PB + 1) + c];
+        W[(k0 + r) * n + k0 + c] = Pn[r * (PB + 1) + c];
       }
       // the permutation on the other columns (every touched row read
       // before any is written, one column per thread) and on perm
       {
         const int ntr = s_ntr;
-        for (int col = tid; col < n; col += NT) {
+        // the first PB values in registers, the rest in this thread's slots
+        // of the U12 area (written only after this step): fewer registers
+        // (loads without branches: entries past ntr re-read the last source)
+        TF* lds = Us + tid;
+        for (int col = tid; ntr > 0 && col < n; col += NT) {
           if (col >= k0 && col < k0 + pb) continue;
-          TF v[2 * PB];
+          TF v[PB];
 #pragma unroll
-          for (int q = 0; q < 2 * PB; ++q)
-            if (q < ntr) v[q] = W[(int64_t)s_src[q] * n + col];
+          for (int q = 0; q < PB; ++q) v[q] = W[s_src[min(q, ntr - 1)] * n + col];
 #pragma unroll
-          for (int q = 0; q < 2 * PB; ++q)
-            if (q < ntr) W[(int64_t)s_tr[q] * n + col] = v[q];
+          for (int q = 0; q < PB; ++q) lds[q * NT] = W[s_src[min(PB + q, ntr - 1)] * n + col];
+#pragma unroll
+          for (int q = 0; q < PB; ++q)
+            if (q < ntr) W[s_tr[q] * n + col] = v[q];
+#pragma unroll
+          for (int q = 0; q < PB; ++q)
+            if (PB + q < ntr) W[s_tr[PB + q] * n + col] = lds[q * NT];
         }
         const int pv = tid < ntr ? perm[s_src[tid]] : 0;
         __syncthreads();
@@ -414,14 +428,14 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       for (int col = T0 + tid; col < n; col += NT) {
         TF u[PB];
 #pragma unroll
-        for (int t = 0; t < PB; ++t) u[t] = W[(int64_t)(k0 + t) * n + col];
+        for (int t = 0; t < PB; ++t) u[t] = W[(k0 + t) * n + col];
 #pragma unroll
         for (int t = 1; t < PB; ++t)
 #pragma unroll
           for (int q = 0; q < t; ++q) u[t] -= Pn[t * (PB + 1) + q] * u[q];
 #pragma unroll
         for (int t = 0; t < PB; ++t) {
-          W[(int64_t)(k0 + t) * n + col] = u[t];
+          W[(k0 + t) * n + col] = u[t];
           Us[t * (NMAX + 1) + col - T0] = u[t];
         }
       }
@@ -445,7 +459,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int row = r0 + acc_row<TF>(lane, r);
-              acc[j][r] = (tj0 + j < tn && row < n && col < n) ? W[(int64_t)row * n + col] : TF(0);
+              acc[j][r] = (tj0 + j < tn && row < n && col < n) ? W[row * n + col] : TF(0);
             }
           }
 #pragma unroll
@@ -464,7 +478,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int row = r0 + acc_row<TF>(lane, r);
-              if (tj0 + j < tn && row < n && col < n) W[(int64_t)row * n + col] = acc[j][r];
+              if (tj0 + j < tn && row < n && col < n) W[row * n + col] = acc[j][r];
             }
           }
         }
@@ -483,7 +497,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
     for (int q = 0; q < kMaxRhs; ++q) {
       double v = 0.0;
       if (q < nrhs)
-        for (int r = tid; r < n; r += NT) v = fmax(v, absinf(Bs[r + (int64_t)q * n]));
+        for (int r = tid; r < n; r += NT) v = fmax(v, absinf(Bs[r + q * n]));
       bnorm[q] = q < nrhs ? block_max<NW>(v, red_d, lane, wave) : 0.0;
     }
 
@@ -495,7 +509,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       // triangular solve)
       for (int e = tid; e < n * nrhs; e += NT) {
         const int r = e / nrhs, q = e - r * nrhs;
-        y[r * kMaxRhs + q] = it == 0 ? (TF)Bs[perm[r] + (int64_t)q * n] : rv[perm[r] * kMaxRhs + q];
+        y[r * kMaxRhs + q] = it == 0 ? (TF)Bs[perm[r] + q * n] : rv[perm[r] * kMaxRhs + q];
       }
       __syncthreads();
       tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
@@ -503,7 +517,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       for (int e = tid; e < n * nrhs; e += NT) {
         const int q = e / n, r = e - q * n;
         const double d = (double)y[r * kMaxRhs + q];
-        Xs[r + (int64_t)q * n] = it == 0 ? d : Xs[r + (int64_t)q * n] + d;
+        Xs[r + q * n] = it == 0 ? d : Xs[r + q * n] + d;
       }
       __syncthreads();
       // X staged in LDS as FP64 (in the diagonal-block area, which the
@@ -511,7 +525,7 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       double* xs = reinterpret_cast<double*>(diag);
       for (int e = tid; e < n * nrhs; e += NT) {
         const int q = e / n, r = e - q * n;
-        xs[r * kMaxRhs + q] = Xs[r + (int64_t)q * n];
+        xs[r * kMaxRhs + q] = Xs[r + q * n];
       }
       __syncthreads();
       // R = B - A X in FP64 from the original A, one row per thread, 8
@@ -525,12 +539,12 @@ __global__ __launch_bounds__(NT, NT == 128 ? 1 : 2) void lu_ir_batch_kernel(
       for (int r = tid; r < n; r += NT) {
         double acc[kMaxRhs];
 #pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q) acc[q] = q < nrhs ? Bs[r + (int64_t)q * n] : 0.0;
+        for (int q = 0; q < kMaxRhs; ++q) acc[q] = q < nrhs ? Bs[r + q * n] : 0.0;
         double asum = 0.0;
         for (int t0 = 0; t0 < n; t0 += 8) {
           double av[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) av[j] = t0 + j < n ? As[r + (int64_t)(t0 + j) * n] : 0.0;
+          for (int j = 0; j < 8; ++j) av[j] = t0 + j < n ? As[r + (t0 + j) * n] : 0.0;
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
             if (t0 + j < n) {
