@@ -130,19 +130,92 @@ __device__ __forceinline__ double block_max(double v, double* red, int lane, int
   return m;
 }
 
+// x made opaque to the optimiser: index arithmetic derived from it is
+// recomputed where it is used instead of hoisted out of the loops around and
+// kept live (register pressure; without it the 256-thread instances spill)
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+// four consecutive values from a 4-byte (float) or 8-byte (double) aligned
+// address: 16-byte global loads (global memory takes unaligned vector loads)
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+__device__ __forceinline__ void load4(const float* p, float* v) {
+  const f4u x = *reinterpret_cast<const f4u*>(p);
+  v[0] = x[0];
+  v[1] = x[1];
+  v[2] = x[2];
+  v[3] = x[3];
+}
+__device__ __forceinline__ void load4(const double* p, double* v) {
+  const d2u x = *reinterpret_cast<const d2u*>(p);
+  const d2u z = *reinterpret_cast<const d2u*>(p + 2);
+  v[0] = x[0];
+  v[1] = x[1];
+  v[2] = z[0];
+  v[3] = z[1];
+}
+// sum over the 16 lanes of a DPP row (quad xor-1, xor-2, half-row and row
+// mirrors); every lane of the row receives it
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float sum16(float v) {
+  v += dpp_f<0xB1>(v);
+  v += dpp_f<0x4E>(v);
+  v += dpp_f<0x141>(v);
+  return v + dpp_f<0x140>(v);
+}
+__device__ __forceinline__ double sum16(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  return v + dpp_d<0x140>(v);
+}
+__device__ __forceinline__ float bcast(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double bcast(double x, int l) { return readlane_d(x, l); }
+
+// RC rows of the block's column segment per 16-lane group: row rb + g + k NG
+// (k < RC), lane columns b0 + cb .. b0 + cb + 3 (zeros outside the rows and
+// the block), every load requested before the first use
+template <typename TF, int RC, int NG>
+__device__ __forceinline__ void load_rows(const TF* __restrict__ W, int n, int b0, int bn, int cb,
+                                          int g, int rb, int r_hi, TF (&w)[RC][4]) {
+#pragma unroll
+  for (int k = 0; k < RC; ++k) {
+    const int r = rb + g + k * NG;
+    const TF* p = W + r * n + b0 + cb;
+    if (r < r_hi && cb + 4 <= bn) {
+      load4(p, w[k]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) w[k][e] = (r < r_hi && cb + e < bn) ? p[e] : TF(0);
+    }
+  }
+}
+
 // Forward (unit lower, FORWARD=true) or backward (upper) substitution of the
-// nrhs columns y[r * kMaxRhs + q] against the factors in the row-major W,
-// right-looking in blocks of DB rows: the diagonal triangle is staged in
-// LDS (every load requested before the first store) and solved by wave 0
-// with lanes over rows; then every row beyond the block (after it going
-// forward, before it going backward) takes y[r] -= W[r][block] y[block],
-// one row per thread with the row segment's loads issued together.
-template <typename TF, int NT, bool FORWARD>
-__device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int nrhs, TF* y,
-                                          TF* diag, int tid) {
-  const int lane = tid & 63, wave = tid >> 6;
+// NR (>= nrhs) columns y[r * kMaxRhs + q] against the factors in the
+// row-major W, right-looking in blocks of DB rows.  The block's strict
+// triangle is staged in LDS (zero-padded to DB, with U's reciprocal diagonal
+// in column DB) and solved by wave 0, lanes over rows, the DB steps unrolled
+// (broadcasts by readlane).  Every row beyond the block (after it going
+// forward, before it going backward) then takes y[r] -= W[r][block] y[block]:
+// 16 lanes per row, each lane four consecutive columns by one 16-byte load,
+// the products summed over the 16 lanes; the first rows' loads are requested
+// before the wait for the diagonal solve.
+template <typename TF, int NT, bool FORWARD, int NR>
+__device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, TF* y, TF* diag,
+                                          int tid0) {
+  constexpr int LD = DB + 1, NG = NT / 16, RC = sizeof(TF) == 4 ? 8 : 4;
   const int nblk = (n + DB - 1) / DB;
   for (int bb = 0; bb < nblk; ++bb) {
+    const int tid = opaque(tid0), lane = tid & 63, wave = tid >> 6;
+    const int g = tid >> 4, c16 = tid & 15, cb = 4 * c16;
     const int blk = FORWARD ? bb : nblk - 1 - bb;
     const int b0 = blk * DB, bn = min(DB, n - b0);
     {
@@ -155,85 +228,88 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int n
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
           const int e = tid + (c0 + q) * NT, r = e / DB, t = e - r * DB;
-          v[q] = (r < bn && t < bn) ? W[(b0 + r) * n + b0 + t] : TF(0);
+          const bool keep = r < bn && t < bn && (FORWARD ? t < r : t >= r);
+          v[q] = keep ? W[(b0 + r) * n + b0 + t] : TF(0);
         }
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
           const int e = tid + (c0 + q) * NT, r = e / DB, t = e - r * DB;
-          diag[r * (DB + 1) + t] = v[q];
+          if (!FORWARD && r == t) {
+            diag[r * LD + DB] = r < bn ? TF(1) / v[q] : TF(1);
+            diag[r * LD + t] = TF(0);
+          } else {
+            diag[r * LD + t] = v[q];
+          }
         }
         asm volatile("" ::: "memory");
       }
     }
+    const int r_lo = FORWARD ? b0 + bn : 0, r_hi = FORWARD ? n : b0;
+    TF w[RC][4];
+    load_rows<TF, RC, NG>(W, n, b0, bn, cb, g, r_lo, r_hi, w);
     __syncthreads();
     if (wave == 0) {
-      TF acc[kMaxRhs];
+      TF acc[NR];
 #pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q) acc[q] = (lane < bn && q < nrhs) ? y[(b0 + lane) * kMaxRhs + q] : TF(0);
+      for (int q = 0; q < NR; ++q) acc[q] = lane < bn ? y[(b0 + lane) * kMaxRhs + q] : TF(0);
       if (FORWARD) {
-        for (int t = 0; t < bn; ++t) {
-          const TF l = (lane > t && lane < bn) ? diag[lane * (DB + 1) + t] : TF(0);
+        // l = 0 on and above the diagonal and in the padding
 #pragma unroll
-          for (int q = 0; q < kMaxRhs; ++q)
-            if (q < nrhs) {
-              const TF yt = sizeof(TF) == 8 ? (TF)readlane_d((double)acc[q], t)
-                                            : __int_as_float(__builtin_amdgcn_readlane(
-                                                  __float_as_int((float)acc[q]), t));
-              acc[q] -= l * yt;
-            }
+        for (int t = 0; t < DB - 1; ++t) {
+          const TF l = diag[lane * LD + t];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] -= l * bcast(acc[q], t);
+          // bounds the LDS reads hoisted ahead (register pressure)
+          if ((t & 7) == 7) asm volatile("" ::: "memory");
         }
       } else {
-        for (int t = bn - 1; t >= 0; --t) {
-          const TF d = diag[t * (DB + 1) + t];
-          if (lane == t) {
+        // u = 0 on and below the diagonal; row t is final once scaled by
+        // its reciprocal pivot (1 in the padding)
+        const TF rd = diag[lane * LD + DB];
 #pragma unroll
-            for (int q = 0; q < kMaxRhs; ++q) acc[q] = acc[q] / d;
+        for (int t = DB - 1; t >= 0; --t) {
+          const TF u = diag[lane * LD + t];
+          const TF rt = bcast(rd, t);
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
+            const TF yt = bcast(acc[q], t) * rt;
+            acc[q] = lane == t ? yt : acc[q] - u * yt;
           }
-          const TF u = (lane < t) ? diag[lane * (DB + 1) + t] : TF(0);
-#pragma unroll
-          for (int q = 0; q < kMaxRhs; ++q)
-            if (q < nrhs) {
-              const TF yt = sizeof(TF) == 8 ? (TF)readlane_d((double)acc[q], t)
-                                            : __int_as_float(__builtin_amdgcn_readlane(
-                                                  __float_as_int((float)acc[q]), t));
-              acc[q] -= u * yt;
-            }
+          if ((t & 7) == 0) asm volatile("" ::: "memory");
         }
       }
       if (lane < bn) {
 #pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q)
-          if (q < nrhs) y[(b0 + lane) * kMaxRhs + q] = acc[q];
+        for (int q = 0; q < NR; ++q) y[(b0 + lane) * kMaxRhs + q] = acc[q];
       }
     }
     __syncthreads();
-    const int r_lo = FORWARD ? b0 + bn : 0, r_hi = FORWARD ? n : b0;
-#pragma unroll 1
-    for (int r = r_lo + tid; r < r_hi; r += NT) {
-      const TF* __restrict__ wr = W + r * n + b0;
-      double acc[kMaxRhs];
+    if (r_lo < r_hi) {
+      TF yv[4][NR];
 #pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q) acc[q] = 0.0;
-      // the segment in quarters of 16 (register pressure): a quarter's
-      // loads are issued together, the next quarter's after its use
+      for (int e = 0; e < 4; ++e)
 #pragma unroll
-      for (int h = 0; h < DB; h += 16) {
-        TF wv[16];
+        for (int q = 0; q < NR; ++q)
+          yv[e][q] = cb + e < bn ? y[(b0 + cb + e) * kMaxRhs + q] : TF(0);
+      for (int rb = r_lo;;) {
 #pragma unroll
-        for (int t = 0; t < 16; ++t) wv[t] = h + t < bn ? wr[h + t] : TF(0);
+        for (int k = 0; k < RC; ++k) {
+          const int r = rb + g + k * NG;
+          TF s[NR];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          if (h + t < bn) {
+          for (int q = 0; q < NR; ++q)
+            s[q] = sum16(w[k][0] * yv[0][q] + w[k][1] * yv[1][q] +
+                         (w[k][2] * yv[2][q] + w[k][3] * yv[3][q]));
+          if (r < r_hi) {
 #pragma unroll
-            for (int q = 0; q < kMaxRhs; ++q)
-              if (q < nrhs) acc[q] += (double)wv[t] * (double)y[(b0 + h + t) * kMaxRhs + q];
+            for (int q = 0; q < NR; ++q)
+              if (c16 == q) y[r * kMaxRhs + q] -= s[q];
           }
-          if ((t & 3) == 3) asm volatile("" ::: "memory");
         }
+        rb += NG * RC;
+        if (rb >= r_hi) break;
+        load_rows<TF, RC, NG>(W, n, b0, bn, cb, g, rb, r_hi, w);
       }
-#pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q)
-        if (q < nrhs) y[r * kMaxRhs + q] -= (TF)acc[q];
     }
     __syncthreads();
   }
@@ -253,10 +329,9 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, int n
 template <typename TF, int NT, int NMAX>
 // Occupancy: the kernel is latency-bound per system, so throughput follows
 // the systems in flight.  The 256-thread FP32 instances are held to 128
-// VGPRs (four workgroups per CU): C5 84 ms against 110 ms at 160 VGPRs and
-// 132 ms at 256; at 128 the compiler keeps about 84 bytes of loop-invariant
-// values in scratch, reloaded a few times per system.  The others keep
-// their register budgets (no scratch).
+// VGPRs (four workgroups per CU; 84 ms for 65,536 systems of sizes 16..256
+// against 110 ms at 160 VGPRs and 132 ms at 256), which they fit without
+// scratch once the loops re-read the thread index through opaque().
 __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 128 ? 1 : 2))) void lu_ir_batch_kernel(
     int count, int pos0, const int* __restrict__ list, const int* __restrict__ list_count,
     const int* __restrict__ order, const int64_t* __restrict__ wbase,
@@ -272,12 +347,14 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
   __shared__ int perm[NMAX + 1];
   __shared__ int s_tr[2 * PB], s_src[2 * PB], s_ntr;
   __shared__ double red_d[NW];
+  __shared__ double s_bnorm[kMaxRhs];
   __shared__ int red_i[NW];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int total = list_count ? *list_count : count;
   constexpr bool kRefactor = sizeof(TF) == 8;
 
   for (int qi = blockIdx.x; qi < total; qi += gridDim.x) {
+    // the thread index re-read through opaque() here and in the inner loops
+    const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     const int pos = list ? list[qi] : pos0 + qi;
     const int s = order[pos];
     const int n = nsz[s];
@@ -323,6 +400,7 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
     TF* Pn = smem;                          // panel: [row][PB + 1]
     TF* Us = smem + LuSmem<TF, NMAX>::kPanel;     // U12: [PB][NMAX + 1]
     for (int k0 = 0; k0 < n; k0 += PB) {
+      const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
       const int pb = min(PB, n - k0), R = n - k0;
       for (int e = tid; e < R * PB; e += NT) {
         const int r = e / PB, c = e - r * PB;
@@ -491,29 +569,36 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
     TF* y = smem + LuSmem<TF, NMAX>::kDiag;
     TF* rv = y + LuSmem<TF, NMAX>::kVec;
     // ||B_q||_inf; ||A||_inf comes with the first residual
+    // (held in LDS: registers are the scarce resource here)
     double anorm = 0.0;
-    double bnorm[kMaxRhs];
-#pragma unroll
-    for (int q = 0; q < kMaxRhs; ++q) {
+#pragma unroll 1
+    for (int q = 0; q < nrhs; ++q) {
       double v = 0.0;
-      if (q < nrhs)
-        for (int r = tid; r < n; r += NT) v = fmax(v, absinf(Bs[r + q * n]));
-      bnorm[q] = q < nrhs ? block_max<NW>(v, red_d, lane, wave) : 0.0;
+      for (int r = tid; r < n; r += NT) v = fmax(v, absinf(Bs[r + q * n]));
+      v = block_max<NW>(v, red_d, lane, wave);
+      if (tid == 0) s_bnorm[q] = v;
     }
 
     int it = 0, converged = 0;
     double prev = INFINITY;
     for (;; ++it) {
+      const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
       // the first pass solves P B, the later ones the correction P R:
-      // X = U^-1 L^-1 P B, then X += U^-1 L^-1 P R (one call site of each
-      // triangular solve)
-      for (int e = tid; e < n * nrhs; e += NT) {
-        const int r = e / nrhs, q = e - r * nrhs;
-        y[r * kMaxRhs + q] = it == 0 ? (TF)Bs[perm[r] + q * n] : rv[perm[r] * kMaxRhs + q];
+      // X = U^-1 L^-1 P B, then X += U^-1 L^-1 P R (the columns past nrhs
+      // zero; one call site of each triangular solve per column count)
+      for (int e = tid; e < n * kMaxRhs; e += NT) {
+        const int r = e / kMaxRhs, q = e - r * kMaxRhs;
+        y[e] = q >= nrhs ? TF(0)
+                         : (it == 0 ? (TF)Bs[perm[r] + q * n] : rv[perm[r] * kMaxRhs + q]);
       }
       __syncthreads();
-      tri_solve<TF, NT, true>(W, n, nrhs, y, diag, tid);
-      tri_solve<TF, NT, false>(W, n, nrhs, y, diag, tid);
+      if (nrhs <= 4) {
+        tri_solve<TF, NT, true, 4>(W, n, y, diag, tid);
+        tri_solve<TF, NT, false, 4>(W, n, y, diag, tid);
+      } else {
+        tri_solve<TF, NT, true, kMaxRhs>(W, n, y, diag, tid);
+        tri_solve<TF, NT, false, kMaxRhs>(W, n, y, diag, tid);
+      }
       for (int e = tid; e < n * nrhs; e += NT) {
         const int q = e / n, r = e - q * n;
         const double d = (double)y[r * kMaxRhs + q];
@@ -574,7 +659,7 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
         if (q < nrhs) {
           const double rn = block_max<NW>(rmax[q], red_d, lane, wave);
           const double xn = block_max<NW>(xmax[q], red_d, lane, wave);
-          const double den = anorm * xn + bnorm[q];
+          const double den = anorm * xn + s_bnorm[q];
           const double be = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
           if (!(be <= kTol)) bad = true;
           worst = (be == be) ? fmax(worst, be) : INFINITY;

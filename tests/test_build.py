@@ -1,8 +1,8 @@
 """CPU checks of the built gfx950 code objects: every kernel of
 libeigen_amd.so runs without scratch memory (register spills or
 runtime-indexed private arrays would be served from it, at HBM latency),
-but for one bounded, measured exception below, read from the kernel
-descriptors' metadata of the library's own embedded code objects."""
+read from the kernel descriptors' metadata of the library's own embedded
+code objects."""
 import os
 import re
 import shutil
@@ -41,22 +41,11 @@ def kernel_metadata(tmp_path):
     return out
 
 
-# The one bounded exception: the batched LU's 256-thread FP32 instances are
-# held to 128 VGPRs for occupancy (twice the systems in flight: 84 vs 132 ms
-# on the 65,536-system workload) and keep a few loop-invariant values in
-# scratch, reloaded a few times per system.
-SCRATCH_ALLOWED = {"_ZN4eamd12_GLOBAL__N_118lu_ir_batch_kernelIfLi256E": 128}
-
-
 def test_no_kernel_uses_scratch(tmp_path):
     meta = kernel_metadata(tmp_path)
     assert len(meta) >= 40, f"only {len(meta)} kernels found"
-    def allowed(n):
-        return max([v for k, v in SCRATCH_ALLOWED.items() if n.startswith(k)] or [0])
-    spilling = [(n, s) for n, s, _ in meta if s > allowed(n)]
+    spilling = [(n, s) for n, s, _ in meta if s]
     assert not spilling, spilling
-    # the exception is used by exactly the three NMAX instances it names
-    assert sum(1 for n, s, _ in meta if s and allowed(n)) <= 3
     # the batched kernels are all there (the check above covers them)
     names = " ".join(n for n, _, _ in meta)
     for k in ("llt_batch_persistent", "lu_ir_batch_kernel", "ldlt_batch_kernel",
