@@ -180,20 +180,24 @@ __device__ __forceinline__ float bcast(float x, int l) {
 __device__ __forceinline__ double bcast(double x, int l) { return readlane_d(x, l); }
 
 // RC rows of the block's column segment per 16-lane group: row rb + g + k NG
-// (k < RC), lane columns b0 + cb .. b0 + cb + 3 (zeros outside the rows and
-// the block), every load requested before the first use
+// (k < RC), lane columns b0 + cb .. b0 + cb + 3, every load requested before
+// the first use.  Branch-free: rows past the matrix read its last row (their
+// sums are not stored), columns past the block read its last column and are
+// zeroed.
 template <typename TF, int RC, int NG>
 __device__ __forceinline__ void load_rows(const TF* __restrict__ W, int n, int b0, int bn, int cb,
-                                          int g, int rb, int r_hi, TF (&w)[RC][4]) {
+                                          int g, int rb, TF (&w)[RC][4]) {
 #pragma unroll
   for (int k = 0; k < RC; ++k) {
-    const int r = rb + g + k * NG;
-    const TF* p = W + r * n + b0 + cb;
-    if (r < r_hi && cb + 4 <= bn) {
-      load4(p, w[k]);
+    const int r = min(rb + g + k * NG, n - 1);
+    if (bn == DB) {
+      load4(W + r * n + b0 + cb, w[k]);
     } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) w[k][e] = (r < r_hi && cb + e < bn) ? p[e] : TF(0);
+      for (int e = 0; e < 4; ++e) {
+        const TF x = W[r * n + b0 + min(cb + e, bn - 1)];
+        w[k][e] = cb + e < bn ? x : TF(0);
+      }
     }
   }
 }
@@ -246,7 +250,7 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, TF* y
     }
     const int r_lo = FORWARD ? b0 + bn : 0, r_hi = FORWARD ? n : b0;
     TF w[RC][4];
-    load_rows<TF, RC, NG>(W, n, b0, bn, cb, g, r_lo, r_hi, w);
+    load_rows<TF, RC, NG>(W, n, b0, bn, cb, g, r_lo, w);
     __syncthreads();
     if (wave == 0) {
       TF acc[NR];
@@ -308,7 +312,7 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, TF* y
         }
         rb += NG * RC;
         if (rb >= r_hi) break;
-        load_rows<TF, RC, NG>(W, n, b0, bn, cb, g, rb, r_hi, w);
+        load_rows<TF, RC, NG>(W, n, b0, bn, cb, g, rb, w);
       }
     }
     __syncthreads();
@@ -370,21 +374,29 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
       constexpr int TS = NT >= 256 ? 64 : 32, PER = TS * TS / NT;
       TF* tile = smem;  // TS x (TS + 1)
       const int nt = (n + TS - 1) / TS;
-      for (int tt = 0; tt < nt * nt; ++tt) {
+      double v[PER];
+      // tile tt's loads (branch-free: entries past the matrix read its last
+      // row or column and are never stored); the next tile's are requested
+      // before this tile's stores, so they are in flight meanwhile
+      auto load_tile = [&](int tt) {
         const int tr = tt / nt, tc = tt - tr * nt;
-        TF v[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
           const int e = tid + q * NT, i = e & (TS - 1), j = e / TS;
-          const int r = tr * TS + i, c = tc * TS + j;
-          v[q] = (r < n && c < n) ? (TF)As[r + c * n] : TF(0);
+          const int r = min(tr * TS + i, n - 1), c = min(tc * TS + j, n - 1);
+          v[q] = As[r + c * n];
         }
+      };
+      load_tile(0);
+      for (int tt = 0; tt < nt * nt; ++tt) {
+        const int tr = tt / nt, tc = tt - tr * nt;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
           const int e = tid + q * NT;
-          tile[(e / TS) * (TS + 1) + (e & (TS - 1))] = v[q];
+          tile[(e / TS) * (TS + 1) + (e & (TS - 1))] = (TF)v[q];
         }
         __syncthreads();
+        if (tt + 1 < nt * nt) load_tile(tt + 1);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
           const int e = tid + q * NT, j = e & (TS - 1), i = e / TS;
@@ -402,9 +414,38 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
     for (int k0 = 0; k0 < n; k0 += PB) {
       const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
       const int pb = min(PB, n - k0), R = n - k0;
-      for (int e = tid; e < R * PB; e += NT) {
-        const int r = e / PB, c = e - r * PB;
-        Pn[r * (PB + 1) + c] = c < pb ? W[(k0 + r) * n + k0 + c] : TF(0);
+      {
+        // 4 lanes per row, four consecutive columns each (one 16-byte load),
+        // a lane's CHR rows requested together
+        constexpr int RPI = NT / 4, CHR = NT >= 256 ? 4 : 8;
+        const int c4 = 4 * (tid & 3), rr = tid >> 2;
+#pragma unroll 1
+        for (int r0 = 0; r0 < R; r0 += RPI * CHR) {
+          TF v[CHR][4];
+#pragma unroll
+          for (int k = 0; k < CHR; ++k) {
+            // rows past the matrix read its last row (not stored)
+            const int r = min(r0 + rr + k * RPI, R - 1);
+            const TF* p = W + (k0 + r) * n + k0;
+            if (pb == PB) {
+              load4(p + c4, v[k]);
+            } else {
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const TF x = p[min(c4 + e, pb - 1)];
+                v[k][e] = c4 + e < pb ? x : TF(0);
+              }
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < CHR; ++k) {
+            const int r = r0 + rr + k * RPI;
+            if (r < R) {
+#pragma unroll
+              for (int e = 0; e < 4; ++e) Pn[r * (PB + 1) + c4 + e] = v[k][e];
+            }
+          }
+        }
       }
       __syncthreads();
       // the panel is factored by wave 0 alone: wave reductions and the
@@ -586,10 +627,35 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
       // the first pass solves P B, the later ones the correction P R:
       // X = U^-1 L^-1 P B, then X += U^-1 L^-1 P R (the columns past nrhs
       // zero; one call site of each triangular solve per column count)
-      for (int e = tid; e < n * kMaxRhs; e += NT) {
-        const int r = e / kMaxRhs, q = e - r * kMaxRhs;
-        y[e] = q >= nrhs ? TF(0)
-                         : (it == 0 ? (TF)Bs[perm[r] + q * n] : rv[perm[r] * kMaxRhs + q]);
+#pragma unroll 1
+      for (int e0 = tid; e0 < n * kMaxRhs; e0 += 8 * NT) {
+        // eight entries per thread, their loads requested together
+        // (branch-free: entries past the matrix or nrhs read a valid one and
+        // are zeroed)
+        double v[8];
+        if (it == 0) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const int e = min(e0 + k * NT, n * kMaxRhs - 1), r = e / kMaxRhs,
+                      q = min(e - r * kMaxRhs, nrhs - 1);
+            v[k] = Bs[perm[r] + q * n];
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const int e = min(e0 + k * NT, n * kMaxRhs - 1), r = e / kMaxRhs,
+                      q = min(e - r * kMaxRhs, nrhs - 1);
+            v[k] = (double)rv[perm[r] * kMaxRhs + q];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int e = e0 + k * NT;
+          if ((e & (kMaxRhs - 1)) >= nrhs) v[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (e0 + k * NT < n * kMaxRhs) y[e0 + k * NT] = (TF)v[k];
       }
       __syncthreads();
       if (nrhs <= 4) {
@@ -599,23 +665,35 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
         tri_solve<TF, NT, true, kMaxRhs>(W, n, y, diag, tid);
         tri_solve<TF, NT, false, kMaxRhs>(W, n, y, diag, tid);
       }
-      for (int e = tid; e < n * nrhs; e += NT) {
-        const int q = e / n, r = e - q * n;
-        const double d = (double)y[r * kMaxRhs + q];
-        Xs[r + q * n] = it == 0 ? d : Xs[r + q * n] + d;
-      }
-      __syncthreads();
-      // X staged in LDS as FP64 (in the diagonal-block area, which the
-      // solves do not use meanwhile)
+      // X = the solution (first pass) or X + the correction, written to X and
+      // staged in LDS as FP64 for the residual (in the diagonal-block area,
+      // which the solves no longer use); eight entries per thread, the old
+      // values' loads requested together
       double* xs = reinterpret_cast<double*>(diag);
-      for (int e = tid; e < n * nrhs; e += NT) {
-        const int q = e / n, r = e - q * n;
-        xs[r * kMaxRhs + q] = Xs[r + q * n];
+#pragma unroll 1
+      for (int e0 = tid; e0 < n * nrhs; e0 += 8 * NT) {
+        double xo[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int e = e0 + k * NT;
+          const double x = Xs[min(e, n * nrhs - 1)];
+          xo[k] = it > 0 ? x : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int e = e0 + k * NT, q = e / n, r = e - q * n;
+          if (e < n * nrhs) {
+            const double xn = xo[k] + (double)y[r * kMaxRhs + q];
+            Xs[e] = xn;
+            xs[r * kMaxRhs + q] = xn;
+          }
+        }
       }
       __syncthreads();
       // R = B - A X in FP64 from the original A, one row per thread, 8
-      // columns of A per step with their loads issued together; the first
-      // pass also sums |A| per row for ||A||_inf
+      // columns of A per step, the next step's loads requested before this
+      // step's use (two steps in flight; the last reads column n - 1 again);
+      // the first pass also sums |A| per row for ||A||_inf
       double rmax[kMaxRhs], xmax[kMaxRhs];
 #pragma unroll
       for (int q = 0; q < kMaxRhs; ++q) { rmax[q] = 0.0; xmax[q] = 0.0; }
@@ -624,12 +702,18 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
       for (int r = tid; r < n; r += NT) {
         double acc[kMaxRhs];
 #pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q) acc[q] = q < nrhs ? Bs[r + q * n] : 0.0;
+        for (int q = 0; q < kMaxRhs; ++q) {
+          const double b = Bs[r + min(q, nrhs - 1) * n];
+          acc[q] = q < nrhs ? b : 0.0;
+        }
         double asum = 0.0;
-        for (int t0 = 0; t0 < n; t0 += 8) {
-          double av[8];
+        double av[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) av[j] = t0 + j < n ? As[r + (t0 + j) * n] : 0.0;
+        for (int j = 0; j < 8; ++j) av[j] = As[r + min(j, n - 1) * n];
+        for (int t0 = 0; t0 < n; t0 += 8) {
+          double an[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) an[j] = As[r + min(t0 + 8 + j, n - 1) * n];
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
             if (t0 + j < n) {
@@ -641,6 +725,8 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
             // bounds the X values read ahead (register pressure)
             if ((j & 3) == 3) asm volatile("" ::: "memory");
           }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) av[j] = an[j];
         }
         arow = fmax(arow, absinf(asum));
 #pragma unroll
