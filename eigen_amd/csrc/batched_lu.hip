@@ -95,9 +95,10 @@ __device__ __forceinline__ int wave_min_i(int x) {
 
 // Eigen's pivot score: |a|; the strict '>' scan makes a NaN win only from
 // the first position (UnaryFunctors.h:63-66, Visitor.h:179)
-__device__ __forceinline__ double score_of(double v, bool first) {
-  double s = fabs(v);
-  if (s != s) s = first ? INFINITY : -1.0;
+template <typename T>
+__device__ __forceinline__ T score_t(T v, bool first) {
+  T s = fabs(v);
+  if (s != s) s = first ? T(INFINITY) : T(-1);
   return s;
 }
 
@@ -178,6 +179,13 @@ __device__ __forceinline__ float bcast(float x, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 __device__ __forceinline__ double bcast(double x, int l) { return readlane_d(x, l); }
+__device__ __forceinline__ float wave_max(float s) {
+  s = fmaxf(s, dpp_f<0xB1>(s));
+  s = fmaxf(s, dpp_f<0x4E>(s));
+  s = fmaxf(s, dpp_f<0x141>(s));
+  s = fmaxf(s, dpp_f<0x140>(s));
+  return fmaxf(fmaxf(bcast(s, 0), bcast(s, 16)), fmaxf(bcast(s, 32), bcast(s, 48)));
+}
 
 // RC rows of the block's column segment per 16-lane group: row rb + g + k NG
 // (k < RC), lane columns b0 + cb .. b0 + cb + 3, every load requested before
@@ -319,6 +327,92 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, TF* y
   }
 }
 
+// u[j - C] = P[ks][j] of lane pl for j = C..PB-1, ks uniform: one readlane
+// per column in the branch of slot ks
+template <typename TF, int NS, int C, int K>
+struct PivotRow {
+  __device__ __forceinline__ static void get(TF (&P)[NS][PB], int ks, int pl, TF* u) {
+    if (ks == K) {
+#pragma unroll
+      for (int j = C; j < PB; ++j) u[j - C] = bcast(P[K][j], pl);
+    } else {
+      PivotRow<TF, NS, C, K + 1>::get(P, ks, pl, u);
+    }
+  }
+};
+template <typename TF, int NS, int C>
+struct PivotRow<TF, NS, C, NS> {
+  __device__ __forceinline__ static void get(TF (&)[NS][PB], int, int, TF*) {}
+};
+
+// One column step C of the panel factorization in wave 0's registers.
+// Lane l holds the physical panel rows l + 64 k in P[k] (NS = rows per
+// lane); lg[k] is that row's current position in the interchanged order, so
+// an interchange swaps two positions and moves no data.  The pivot is
+// Eigen's first maximum of |a| over positions C..R-1 (the smallest position
+// among equal maxima), the pivot row's columns C.. are broadcast by readlane,
+// and the rows below take l = a * (1 / pivot) (one division per column; the
+// reference divides each entry, which differs by at most an ulp in l) and the
+// rank-1 update in registers (Eigen/src/LU/PartialPivLU.h:365-407).
+template <typename TF, int NS, int C>
+__device__ __forceinline__ void panel_col(TF (&P)[NS][PB], int (&lg)[NS], int lane, int R, int k0,
+                                          int* piv) {
+  TF sc = TF(-1);
+  int ri = 0x7fffffff;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const int g = lg[k];
+    const TF v = score_t(P[k][C], g == C);
+    if (g >= C && g < R && (v > sc || (v == sc && g < ri))) {
+      sc = v;
+      ri = g;
+    }
+  }
+  const TF M = wave_max(sc);
+  const int p = wave_min_i(sc == M ? ri : 0x7fffffff);
+  // the pivot row: lane pl, slot ks (uniform), its columns C.. broadcast
+  int kown = -1;
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+    if (lg[k] == p) kown = k;
+  const int pl = (int)__builtin_ctzll(__ballot(kown >= 0));
+  const int ks = __builtin_amdgcn_readlane(kown, pl);
+  TF u[PB - C];
+  PivotRow<TF, NS, C, 0>::get(P, ks, pl, u);
+  if (lane == 0) piv[k0 + C] = k0 + p;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) lg[k] = lg[k] == C ? p : (lg[k] == p ? C : lg[k]);
+  const TF pv = u[0];
+  if (pv != TF(0)) {
+    const TF rp = TF(1) / pv;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const bool below = lg[k] > C && lg[k] < R;
+      const TF l = below ? P[k][C] * rp : TF(0);
+      if (below) P[k][C] = l;
+#pragma unroll
+      for (int j = C + 1; j < PB; ++j) P[k][j] -= l * u[j - C];
+    }
+  }
+}
+// the column steps C.. while C < pb, unrolled by template recursion (every
+// register index a constant)
+template <typename TF, int NS, int C>
+struct PanelCols {
+  __device__ __forceinline__ static void run(TF (&P)[NS][PB], int (&lg)[NS], int lane, int R,
+                                             int pb, int k0, int* piv) {
+    if (C < pb) {
+      panel_col<TF, NS, C>(P, lg, lane, R, k0, piv);
+      PanelCols<TF, NS, C + 1>::run(P, lg, lane, R, pb, k0, piv);
+    }
+  }
+};
+template <typename TF, int NS>
+struct PanelCols<TF, NS, PB> {
+  __device__ __forceinline__ static void run(TF (&)[NS][PB], int (&)[NS], int, int, int, int,
+                                             int*) {}
+};
+
 // One workgroup per system.  Pass 1 (TF = float): systems list[0..count);
 // systems that do not reach the tolerance are appended to the refactor
 // queue.  Pass 2 (TF = double): the systems of the queue (its length read
@@ -448,36 +542,26 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
         }
       }
       __syncthreads();
-      // the panel is factored by wave 0 alone: wave reductions and the
-      // wave's in-order LDS accesses need no workgroup barrier per column
+      // the panel is factored by wave 0 alone, in its registers (lane l
+      // holds panel rows l, l + 64, ...; see PanelCols)
       if (wave == 0) {
-        for (int c = 0; c < pb; ++c) {
-          double sc = -1.0;
-          int ri = 0x7fffffff;
-          for (int r = c + lane; r < R; r += 64) {
-            const double v = score_of((double)Pn[r * (PB + 1) + c], r == c);
-            if (v > sc) { sc = v; ri = r; }
-          }
-          const double M = wave_max(sc);
-          const int p = wave_min_i(sc == M ? ri : 0x7fffffff);
-          const TF pv = Pn[p * (PB + 1) + c];
-          if (lane == 0) piv[k0 + c] = k0 + p;
-          if (p != c && lane < PB) {
-            const TF t = Pn[c * (PB + 1) + lane];
-            Pn[c * (PB + 1) + lane] = Pn[p * (PB + 1) + lane];
-            Pn[p * (PB + 1) + lane] = t;
-          }
-          asm volatile("" ::: "memory");
-          if (pv != TF(0)) {
-            for (int r = c + 1 + lane; r < R; r += 64) {
-              const TF l = Pn[r * (PB + 1) + c] / pv;
-              Pn[r * (PB + 1) + c] = l;
-              for (int cc = c + 1; cc < pb; ++cc)
-                Pn[r * (PB + 1) + cc] -= l * Pn[c * (PB + 1) + cc];
-            }
-          }
-          asm volatile("" ::: "memory");
+        constexpr int NS = NMAX / 64;
+        TF P[NS][PB];
+        int lg[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          lg[k] = lane + 64 * k;
+#pragma unroll
+          for (int j = 0; j < PB; ++j) P[k][j] = Pn[(lane + 64 * k) * (PB + 1) + j];
         }
+        PanelCols<TF, NS, 0>::run(P, lg, lane, R, pb, k0, piv);
+        // each row to its final position
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+          if (lg[k] < R) {
+#pragma unroll
+            for (int j = 0; j < PB; ++j) Pn[lg[k] * (PB + 1) + j] = P[k][j];
+          }
         // the panel's interchanges as one permutation of the rows they
         // touch: slot L < pb holds row k0 + L, slot pb + c the pivot row of
         // column c unless an earlier slot holds it; src is the row whose
@@ -563,7 +647,7 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
       // ...; a row's tiles in groups of TB, all their loads issued first
       const int l16 = lane & 15, lk = lane >> 4;
       const int tn = (nr + 15) / 16;
-      constexpr int TB = sizeof(TF) == 8 ? 4 : 8;
+      constexpr int TB = sizeof(TF) == 8 ? 4 : (NMAX >= 128 ? 8 : 4);
       for (int ti = wave; ti < tn; ti += NW) {
         const int r0 = T0 + ti * 16, arow = r0 + l16;
         TF a[PB / 4];
