@@ -8,15 +8,21 @@
 // descending size from the plan's order so the largest start first.  The
 // workgroup converts its FP64 column-major A_i into a row-major working copy
 // W_i (row interchanges are then contiguous), and runs a right-looking
-// blocked LU with 16-column panels: the panel is factored in LDS (pivot
-// search by DPP wave reductions), the interchanges are applied to the rest
-// of the rows, U12 is solved against the unit lower L11, and the trailing
-// update A22 -= L21 U12 runs as 16x16x4 MFMA tiles (FP32, or FP64 for the
-// refactor pass).  The solves and the refinement steps (residual B - A X in
-// FP64 from the original A) stay in the same workgroup.  The workgroup size
-// (64, 128 or 256 threads) is chosen per 16-wide size bucket by timing on
-// synthetic systems, once per process.
+// blocked LU with 16-column panels.  A panel is factored by wave 0 in its
+// registers (pivot search by DPP wave reductions, interchanges as swapped
+// row positions, the pivot row broadcast by readlane); the interchanges are
+// applied to the other columns, U12 is solved against the unit lower L11,
+// and the trailing update A22 -= L21 U12 runs as 16x16x4 MFMA tiles (FP32,
+// transposed so a lane holds four adjacent columns of a row; FP64 for the
+// refactor pass).  With more than one wave the next panel's columns are
+// updated first and wave 0 factors that panel while the other waves update
+// the rest.  The solves (right-looking in 64-row blocks, 16 lanes per row)
+// and the refinement steps (residual B - A X in FP64 from the original A)
+// stay in the same workgroup.  The workgroup size (64, 128 or 256 threads)
+// is chosen per 16-wide size bucket by timing on synthetic systems, once per
+// process.
 #include <algorithm>
+#include <type_traits>
 #include <mutex>
 
 #include "common.h"
@@ -138,6 +144,17 @@ __device__ __forceinline__ int opaque(int x) {
   asm volatile("" : "+v"(x));
   return x;
 }
+// the thread index rebuilt from the wave index (a scalar, which is also the
+// `wave` used for branches: uniform, so a wave's branch holds no registers
+// through the other waves' branch) and the lane
+// (mbcnt over a full mask, so exec does not enter; in a volatile asm block so
+// it is recomputed where it is used): nothing vector-valued stays live for it
+// between uses
+__device__ __forceinline__ int thread_index(int swave) {
+  int lane;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+  return swave * 64 + lane;
+}
 // four consecutive values from a 4-byte (float) or 8-byte (double) aligned
 // address: 16-byte global loads (global memory takes unaligned vector loads)
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
@@ -156,6 +173,13 @@ __device__ __forceinline__ void load4(const double* p, double* v) {
   v[1] = x[1];
   v[2] = z[0];
   v[3] = z[1];
+}
+__device__ __forceinline__ void store4(float* p, const float* v) {
+  *reinterpret_cast<f4u*>(p) = f4u{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void store4(double* p, const double* v) {
+  *reinterpret_cast<d2u*>(p) = d2u{v[0], v[1]};
+  *reinterpret_cast<d2u*>(p + 2) = d2u{v[2], v[3]};
 }
 // sum over the 16 lanes of a DPP row (quad xor-1, xor-2, half-row and row
 // mirrors); every lane of the row receives it
@@ -226,7 +250,9 @@ __device__ __forceinline__ void tri_solve(const TF* __restrict__ W, int n, TF* y
   constexpr int LD = DB + 1, NG = NT / 16, RC = sizeof(TF) == 4 ? 8 : 4;
   const int nblk = (n + DB - 1) / DB;
   for (int bb = 0; bb < nblk; ++bb) {
-    const int tid = opaque(tid0), lane = tid & 63, wave = tid >> 6;
+    const int tid = opaque(tid0), lane = tid & 63;
+    // the wave index as a scalar: branches on it are uniform
+    const int wave = __builtin_amdgcn_readfirstlane(tid0 >> 6);
     const int g = tid >> 4, c16 = tid & 15, cb = 4 * c16;
     const int blk = FORWARD ? bb : nblk - 1 - bb;
     const int b0 = blk * DB, bn = min(DB, n - b0);
@@ -413,6 +439,92 @@ struct PanelCols<TF, NS, PB> {
                                              int*) {}
 };
 
+// R = B - A X in FP64 from the original A and the convergence test's norms,
+// NR (>= nrhs) columns: one row per thread, 8 columns of A per step, the
+// loads of the next RD steps in flight (the steps past n read column n - 1
+// again); the first pass also sums |A| per row for ||A||_inf.  The maxima of
+// all columns are reduced together (one LDS exchange); the normwise backward
+// error of column q is ||r_q|| / (||A|| ||x_q|| + ||b_q||) in the infinity
+// norm.
+template <typename TF, int NT, int NR>
+__device__ __forceinline__ void residual(const double* __restrict__ As,
+                                         const double* __restrict__ Bs, int n, int nrhs,
+                                         const double* xs, TF* rv, int tid, int lane, int wave,
+                                         double* red, const double* bnorm, bool first,
+                                         double& anorm, double& worst, bool& bad) {
+  constexpr int NW = NT / 64, RD = NR > 4 ? 2 : 4, NV = 2 * NR + 1;
+  double mx[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) mx[q] = 0.0;
+#pragma unroll 1
+  for (int r = tid; r < n; r += NT) {
+    double acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double b = Bs[r + min(q, nrhs - 1) * n];
+      acc[q] = q < nrhs ? b : 0.0;
+    }
+    double asum = 0.0;
+    double av[RD][8];
+#pragma unroll
+    for (int d = 0; d < RD; ++d)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) av[d][j] = As[r + min(8 * d + j, n - 1) * n];
+    for (int t0 = 0; t0 < n; t0 += 8 * RD) {
+#pragma unroll
+      for (int d = 0; d < RD; ++d) {
+        const int c0 = t0 + 8 * d;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if (c0 + j < n) {
+            asum += fabs(av[d][j]);
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] -= av[d][j] * xs[(c0 + j) * kMaxRhs + q];
+          }
+          // bounds the X values read ahead (register pressure)
+          constexpr int XM = NR > 4 ? 1 : 3;
+          if ((j & XM) == XM) asm volatile("" ::: "memory");
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) av[d][j] = As[r + min(c0 + 8 * RD + j, n - 1) * n];
+      }
+    }
+    mx[2 * NR] = fmax(mx[2 * NR], absinf(asum));
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      rv[r * kMaxRhs + q] = (TF)acc[q];
+      mx[q] = fmax(mx[q], absinf(acc[q]));
+      mx[NR + q] = fmax(mx[NR + q], absinf(xs[r * kMaxRhs + q]));
+    }
+  }
+  // workgroup maxima (a NaN counts as +infinity): the waves' maxima through
+  // LDS, one value at a time (register pressure)
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double w = wave_max(mx[q] == mx[q] ? mx[q] : INFINITY);
+    if (lane == 0) red[wave * NV + q] = w;
+    asm volatile("" ::: "memory");
+  }
+  __syncthreads();
+  auto gmax = [&](int q) {
+    double m = red[q];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) m = fmax(m, red[w * NV + q]);
+    return m;
+  };
+  if (first) anorm = gmax(2 * NR);
+  worst = 0.0;
+  bad = false;
+#pragma unroll 1
+  for (int q = 0; q < nrhs; ++q) {
+    const double rn = gmax(q), den = anorm * gmax(NR + q) + bnorm[q];
+    const double be = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
+    if (!(be <= kTol)) bad = true;
+    worst = (be == be) ? fmax(worst, be) : INFINITY;
+  }
+  __syncthreads();
+}
+
 // One workgroup per system.  Pass 1 (TF = float): systems list[0..count);
 // systems that do not reach the tolerance are appended to the refactor
 // queue.  Pass 2 (TF = double): the systems of the queue (its length read
@@ -429,7 +541,7 @@ template <typename TF, int NT, int NMAX>
 // the systems in flight.  The 256-thread FP32 instances are held to 128
 // VGPRs (four workgroups per CU; 84 ms for 65,536 systems of sizes 16..256
 // against 110 ms at 160 VGPRs and 132 ms at 256), which they fit without
-// scratch once the loops re-read the thread index through opaque().
+// scratch once the loops rebuild the thread index (thread_index()).
 __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 128 ? 1 : 2))) void lu_ir_batch_kernel(
     int count, int pos0, const int* __restrict__ list, const int* __restrict__ list_count,
     const int* __restrict__ order, const int64_t* __restrict__ wbase,
@@ -444,15 +556,16 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
   __shared__ int piv[NMAX];
   __shared__ int perm[NMAX + 1];
   __shared__ int s_tr[2 * PB], s_src[2 * PB], s_ntr;
-  __shared__ double red_d[NW];
+  __shared__ double red_d[NW * (2 * kMaxRhs + 1)];
   __shared__ double s_bnorm[kMaxRhs];
   __shared__ int red_i[NW];
   const int total = list_count ? *list_count : count;
   constexpr bool kRefactor = sizeof(TF) == 8;
+  const int swave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
   for (int qi = blockIdx.x; qi < total; qi += gridDim.x) {
-    // the thread index re-read through opaque() here and in the inner loops
-    const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    // the thread index rebuilt (thread_index) here and in the inner loops
+    const int tid = thread_index(swave), lane = tid & 63, wave = swave;
     const int pos = list ? list[qi] : pos0 + qi;
     const int s = order[pos];
     const int n = nsz[s];
@@ -505,99 +618,189 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
     // ---- 2. blocked LU with partial pivoting
     TF* Pn = smem;                          // panel: [row][PB + 1]
     TF* Us = smem + LuSmem<TF, NMAX>::kPanel;     // U12: [PB][NMAX + 1]
-    for (int k0 = 0; k0 < n; k0 += PB) {
-      const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
-      const int pb = min(PB, n - k0), R = n - k0;
-      {
-        // 4 lanes per row, four consecutive columns each (one 16-byte load),
-        // a lane's CHR rows requested together
-        constexpr int RPI = NT / 4, CHR = NT >= 256 ? 4 : 8;
-        const int c4 = 4 * (tid & 3), rr = tid >> 2;
-#pragma unroll 1
-        for (int r0 = 0; r0 < R; r0 += RPI * CHR) {
-          TF v[CHR][4];
+    // Look-ahead: the next panel's columns (tile column 0 of the trailing
+    // matrix) are updated first; then wave 0 loads and factors the next
+    // panel in its registers while the other waves update the rest of the
+    // trailing matrix, and writes it out after they are done.
+    constexpr int NS = NMAX / 64;
+    // wave 0: panel rows kp.. (columns kp .. kp + 15) from W into registers
+    // (lane l holds rows kp + l + 64 k in P[k]), factored (PanelCols)
+    auto factor_panel = [&](int kp, TF (&P)[NS][PB], int (&lg)[NS], int lane) {
+      const int pbp = min(PB, n - kp), Rp = n - kp;
 #pragma unroll
-          for (int k = 0; k < CHR; ++k) {
-            // rows past the matrix read its last row (not stored)
-            const int r = min(r0 + rr + k * RPI, R - 1);
-            const TF* p = W + (k0 + r) * n + k0;
-            if (pb == PB) {
-              load4(p + c4, v[k]);
-            } else {
+      for (int k = 0; k < NS; ++k) {
+        lg[k] = lane + 64 * k;
+        // rows past the matrix read its last row (never stored)
+        const TF* p = W + (kp + min(lane + 64 * k, Rp - 1)) * n + kp;
+        if (pbp == PB) {
 #pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                const TF x = p[min(c4 + e, pb - 1)];
-                v[k][e] = c4 + e < pb ? x : TF(0);
+          for (int j = 0; j < PB; j += 4) load4(p + j, &P[k][j]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < PB; ++j) {
+            const TF x = p[min(j, pbp - 1)];
+            P[k][j] = j < pbp ? x : TF(0);
+          }
+        }
+      }
+      PanelCols<TF, NS, 0>::run(P, lg, lane, Rp, pbp, kp, piv);
+    };
+    // wave 0: the factored panel to Pn and to W's panel columns in its final
+    // row order, and the panel's interchanges as one permutation of the rows
+    // they touch (s_tr / s_src / s_ntr): slot L < pb holds row kp + L, slot
+    // pb + c the pivot row of column c unless an earlier slot holds it; src
+    // is the row whose old content the slot's row receives
+    auto write_panel = [&](int kp, TF (&P)[NS][PB], int (&lg)[NS], int lane) {
+      const int pbp = min(PB, n - kp), Rp = n - kp;
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        if (lg[k] < Rp) {
+#pragma unroll
+          for (int j = 0; j < PB; ++j) Pn[lg[k] * (PB + 1) + j] = P[k][j];
+          TF* w = W + (kp + lg[k]) * n + kp;
+#pragma unroll
+          for (int j = 0; j < PB; ++j)
+            if (j < pbp) w[j] = P[k][j];
+        }
+      int row = lane < pbp ? kp + lane : (lane < 2 * pbp ? piv[kp + lane - pbp] : -1);
+      int first = lane;
+      for (int j = 0; j < 2 * PB; ++j) {
+        const int rj = __builtin_amdgcn_readlane(row, j);
+        if (j < lane && rj == row && first == lane) first = j;
+      }
+      const bool live = row >= 0 && first == lane;
+      int src = row;
+      for (int c = 0; c < pbp; ++c) {
+        const int b = piv[kp + c];
+        const int sb = (int)__builtin_ctzll(__ballot(live && row == b));
+        const int va = __builtin_amdgcn_readlane(src, c);
+        const int vb = __builtin_amdgcn_readlane(src, sb);
+        if (lane == c) src = vb;
+        if (lane == sb) src = va;
+      }
+      const bool moved = live && src != row;
+      const unsigned long long mm = __ballot(moved);
+      const int idx = __builtin_amdgcn_mbcnt_hi((unsigned)(mm >> 32),
+                                                __builtin_amdgcn_mbcnt_lo((unsigned)mm, 0u));
+      if (moved) {
+        s_tr[idx] = row;
+        s_src[idx] = src;
+      }
+      if (lane == 0) s_ntr = __popcll(mm);
+    };
+    // A22 += L21 (-U12) on the 16x16 tiles (ti, tj_begin .. tj_end - 1), in
+    // groups of TB with all a group's loads issued first.  The MFMA A operand
+    // is L21 of the tile row (panel at k0, trailing rows from T0), read from
+    // the panel area or (A_FROM_W, while wave 0 rewrites the panel area)
+    // from W's panel columns; rows past the matrix only reach products that
+    // are never stored.  B is -U12 from the U12 area.
+    constexpr int TB = sizeof(TF) == 8 ? 4 : (NMAX >= 128 ? 8 : 4);
+    auto update_row = [&](auto a_from_w, int ti, int tj_begin, int tj_end, int k0, int T0,
+                          int lane0) {
+      const int lane = opaque(lane0), l16 = lane & 15, lk = lane >> 4;
+      const int r0 = T0 + ti * 16, arow = r0 + l16;
+      TF a[PB / 4];
+      if constexpr (decltype(a_from_w)::value) {
+        const TF* ap = W + min(arow, n - 1) * n + k0 + lk;
+#pragma unroll
+        for (int kk = 0; kk < PB / 4; ++kk) a[kk] = ap[kk * 4];
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < PB / 4; ++kk)
+          a[kk] = arow < n ? Pn[(arow - k0) * (PB + 1) + kk * 4 + lk] : TF(0);
+      }
+      if constexpr (sizeof(TF) == 4) {
+        // FP32: the transposed tile C^T += (-U12)^T L21^T (the operands
+        // swapped), whose accumulator gives lane l row r0 + l % 16 and the
+        // four consecutive columns 4 (l / 16) .. of the tile: one 16-byte
+        // load and store per tile (rows past the matrix read its last row
+        // and are not stored)
+        const int row = r0 + l16;
+        TF* wr = W + min(row, n - 1) * n + T0 + 4 * lk;
+        for (int tj0 = tj_begin; tj0 < tj_end; tj0 += TB) {
+          AccT acc[TB];
+#pragma unroll
+          for (int j = 0; j < TB; ++j)
+            if (tj0 + j < tj_end) {
+              const int c0 = (tj0 + j) * 16;
+              if (T0 + c0 + 16 <= n) {
+                acc[j] = *reinterpret_cast<const f4u*>(wr + c0);
+              } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  acc[j][r] = T0 + c0 + 4 * lk + r < n ? wr[c0 + r] : TF(0);
               }
             }
-          }
 #pragma unroll
-          for (int k = 0; k < CHR; ++k) {
-            const int r = r0 + rr + k * RPI;
-            if (r < R) {
+          for (int j = 0; j < TB; ++j)
+            if (tj0 + j < tj_end) {
+              const int c0 = (tj0 + j) * 16;
 #pragma unroll
-              for (int e = 0; e < 4; ++e) Pn[r * (PB + 1) + c4 + e] = v[k][e];
+              for (int kk = 0; kk < PB / 4; ++kk)
+                acc[j] = mfma16x4(Us[(kk * 4 + lk) * (NMAX + 1) + c0 + l16], a[kk], acc[j]);
             }
+#pragma unroll
+          for (int j = 0; j < TB; ++j)
+            if (tj0 + j < tj_end && row < n) {
+              const int c0 = (tj0 + j) * 16;
+              if (T0 + c0 + 16 <= n) {
+                *reinterpret_cast<f4u*>(wr + c0) = acc[j];
+              } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  if (T0 + c0 + 4 * lk + r < n) wr[c0 + r] = acc[j][r];
+              }
+            }
+        }
+        return;
+      }
+      for (int tj0 = tj_begin; tj0 < tj_end; tj0 += TB) {
+        AccT acc[TB];
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+          const int col = T0 + (tj0 + j) * 16 + l16;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + acc_row<TF>(lane, r);
+            acc[j][r] = (tj0 + j < tj_end && row < n && col < n) ? W[row * n + col] : TF(0);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+          const int col = T0 + (tj0 + j) * 16 + l16;
+#pragma unroll
+          for (int kk = 0; kk < PB / 4; ++kk) {
+            const TF bv = (tj0 + j < tj_end && col < n)
+                              ? Us[(kk * 4 + lk) * (NMAX + 1) + col - T0]
+                              : TF(0);
+            acc[j] = mfma16x4(a[kk], bv, acc[j]);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+          const int col = T0 + (tj0 + j) * 16 + l16;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + acc_row<TF>(lane, r);
+            if (tj0 + j < tj_end && row < n && col < n) W[row * n + col] = acc[j][r];
           }
         }
       }
-      __syncthreads();
-      // the panel is factored by wave 0 alone, in its registers (lane l
-      // holds panel rows l, l + 64, ...; see PanelCols)
+    };
+
+    {
+      const int tid = thread_index(swave), lane = tid & 63, wave = swave;
       if (wave == 0) {
-        constexpr int NS = NMAX / 64;
         TF P[NS][PB];
         int lg[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-          lg[k] = lane + 64 * k;
-#pragma unroll
-          for (int j = 0; j < PB; ++j) P[k][j] = Pn[(lane + 64 * k) * (PB + 1) + j];
-        }
-        PanelCols<TF, NS, 0>::run(P, lg, lane, R, pb, k0, piv);
-        // each row to its final position
-#pragma unroll
-        for (int k = 0; k < NS; ++k)
-          if (lg[k] < R) {
-#pragma unroll
-            for (int j = 0; j < PB; ++j) Pn[lg[k] * (PB + 1) + j] = P[k][j];
-          }
-        // the panel's interchanges as one permutation of the rows they
-        // touch: slot L < pb holds row k0 + L, slot pb + c the pivot row of
-        // column c unless an earlier slot holds it; src is the row whose
-        // old content the slot's row receives
-        int row = lane < pb ? k0 + lane : (lane < 2 * pb ? piv[k0 + lane - pb] : -1);
-        int first = lane;
-        for (int j = 0; j < 2 * PB; ++j) {
-          const int rj = __builtin_amdgcn_readlane(row, j);
-          if (j < lane && rj == row && first == lane) first = j;
-        }
-        const bool live = row >= 0 && first == lane;
-        int src = row;
-        for (int c = 0; c < pb; ++c) {
-          const int b = piv[k0 + c];
-          const int sb = (int)__builtin_ctzll(__ballot(live && row == b));
-          const int va = __builtin_amdgcn_readlane(src, c);
-          const int vb = __builtin_amdgcn_readlane(src, sb);
-          if (lane == c) src = vb;
-          if (lane == sb) src = va;
-        }
-        const bool moved = live && src != row;
-        const unsigned long long mm = __ballot(moved);
-        const int idx = __builtin_amdgcn_mbcnt_hi((unsigned)(mm >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((unsigned)mm, 0u));
-        if (moved) {
-          s_tr[idx] = row;
-          s_src[idx] = src;
-        }
-        if (lane == 0) s_ntr = __popcll(mm);
+        factor_panel(0, P, lg, lane);
+        write_panel(0, P, lg, lane);
       }
       __syncthreads();
-      // panel back to W
-      for (int e = tid; e < R * pb; e += NT) {
-        const int r = e / pb, c = e - r * pb;
-        W[(k0 + r) * n + k0 + c] = Pn[r * (PB + 1) + c];
-      }
+    }
+    for (int k0 = 0; k0 < n; k0 += PB) {
+      const int tid = thread_index(swave), lane = tid & 63, wave = swave;
+      const int pb = min(PB, n - k0);
       // the permutation on the other columns (every touched row read
       // before any is written, one column per thread) and on perm
       {
@@ -626,9 +829,34 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
       }
       __syncthreads();
       const int T0 = k0 + pb, nr = n - T0;
-      if (nr <= 0) continue;
-      // U12 = L11^-1 A12
-      for (int col = T0 + tid; col < n; col += NT) {
+      if (nr <= 0) break;
+      // U12 = L11^-1 A12: four adjacent columns per thread (16-byte loads
+      // and stores of W) up to a multiple of four, the last columns one per
+      // thread
+      const int nq = T0 + ((n - T0) & ~3);
+      for (int c4 = T0 + 4 * tid; c4 < nq; c4 += 4 * NT) {
+        TF u[PB][4];
+#pragma unroll
+        for (int t = 0; t < PB; ++t) load4(W + (k0 + t) * n + c4, u[t]);
+#pragma unroll
+        for (int t = 1; t < PB; ++t) {
+#pragma unroll
+          for (int q = 0; q < t; ++q) {
+            const TF l = Pn[t * (PB + 1) + q];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) u[t][e] -= l * u[q][e];
+          }
+          // bounds the L11 reads hoisted ahead (register pressure)
+          asm volatile("" ::: "memory");
+        }
+#pragma unroll
+        for (int t = 0; t < PB; ++t) {
+          store4(W + (k0 + t) * n + c4, u[t]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) Us[t * (NMAX + 1) + c4 - T0 + e] = -u[t][e];
+        }
+      }
+      for (int col = nq + tid; col < n; col += NT) {
         TF u[PB];
 #pragma unroll
         for (int t = 0; t < PB; ++t) u[t] = W[(k0 + t) * n + col];
@@ -639,51 +867,35 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
 #pragma unroll
         for (int t = 0; t < PB; ++t) {
           W[(k0 + t) * n + col] = u[t];
-          Us[t * (NMAX + 1) + col - T0] = u[t];
+          Us[t * (NMAX + 1) + col - T0] = -u[t];
         }
       }
       __syncthreads();
-      // A22 -= L21 U12 on 16x16 tiles: wave w takes tile rows w, w + NW,
-      // ...; a row's tiles in groups of TB, all their loads issued first
-      const int l16 = lane & 15, lk = lane >> 4;
       const int tn = (nr + 15) / 16;
-      constexpr int TB = sizeof(TF) == 8 ? 4 : (NMAX >= 128 ? 8 : 4);
-      for (int ti = wave; ti < tn; ti += NW) {
-        const int r0 = T0 + ti * 16, arow = r0 + l16;
-        TF a[PB / 4];
-#pragma unroll
-        for (int kk = 0; kk < PB / 4; ++kk)
-          a[kk] = arow < n ? -Pn[(arow - k0) * (PB + 1) + kk * 4 + lk] : TF(0);
-        for (int tj0 = 0; tj0 < tn; tj0 += TB) {
-          AccT acc[TB];
-#pragma unroll
-          for (int j = 0; j < TB; ++j) {
-            const int col = T0 + (tj0 + j) * 16 + l16;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int row = r0 + acc_row<TF>(lane, r);
-              acc[j][r] = (tj0 + j < tn && row < n && col < n) ? W[row * n + col] : TF(0);
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < TB; ++j) {
-            const int col = T0 + (tj0 + j) * 16 + l16;
-#pragma unroll
-            for (int kk = 0; kk < PB / 4; ++kk) {
-              const TF b = (tj0 + j < tn && col < n) ? Us[(kk * 4 + lk) * (NMAX + 1) + col - T0]
-                                                     : TF(0);
-              acc[j] = mfma16x4(a[kk], b, acc[j]);
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < TB; ++j) {
-            const int col = T0 + (tj0 + j) * 16 + l16;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int row = r0 + acc_row<TF>(lane, r);
-              if (tj0 + j < tn && row < n && col < n) W[row * n + col] = acc[j][r];
-            }
-          }
+      const std::integral_constant<bool, false> from_lds;
+      const std::integral_constant<bool, true> from_w;
+      if constexpr (NW == 1) {
+        // one wave: the whole update, then the next panel
+        for (int ti = 0; ti < tn; ++ti) update_row(from_lds, ti, 0, tn, k0, T0, lane);
+        TF P[NS][PB];
+        int lg[NS];
+        factor_panel(T0, P, lg, lane);
+        write_panel(T0, P, lg, lane);
+      } else {
+        // (a) tile column 0 (the next panel's columns) by every wave
+        for (int ti = wave; ti < tn; ti += NW) update_row(from_lds, ti, 0, 1, k0, T0, lane);
+        __syncthreads();
+        // (b) wave 0 factors the next panel and writes it out (panel area,
+        // W's columns T0 .., interchange lists); the others update tile
+        // columns 1.. with L21 from W
+        if (wave == 0) {
+          TF P[NS][PB];
+          int lg[NS];
+          factor_panel(T0, P, lg, lane);
+          write_panel(T0, P, lg, lane);
+        } else {
+          for (int ti = wave - 1; ti < tn; ti += NW - 1)
+            update_row(from_w, ti, 1, tn, k0, T0, lane);
         }
       }
       __syncthreads();
@@ -698,6 +910,7 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
     double anorm = 0.0;
 #pragma unroll 1
     for (int q = 0; q < nrhs; ++q) {
+      const int tid = thread_index(swave), lane = tid & 63, wave = swave;
       double v = 0.0;
       for (int r = tid; r < n; r += NT) v = fmax(v, absinf(Bs[r + q * n]));
       v = block_max<NW>(v, red_d, lane, wave);
@@ -707,7 +920,7 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
     int it = 0, converged = 0;
     double prev = INFINITY;
     for (;; ++it) {
-      const int tid = opaque(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+      const int tid = thread_index(swave), lane = tid & 63, wave = swave;
       // the first pass solves P B, the later ones the correction P R:
       // X = U^-1 L^-1 P B, then X += U^-1 L^-1 P R (the columns past nrhs
       // zero; one call site of each triangular solve per column count)
@@ -774,72 +987,21 @@ __global__ __launch_bounds__(NT, sizeof(TF) == 8 ? 1 : (NT == 256 ? 4 : (NT == 1
         }
       }
       __syncthreads();
-      // R = B - A X in FP64 from the original A, one row per thread, 8
-      // columns of A per step, the next step's loads requested before this
-      // step's use (two steps in flight; the last reads column n - 1 again);
-      // the first pass also sums |A| per row for ||A||_inf
-      double rmax[kMaxRhs], xmax[kMaxRhs];
-#pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q) { rmax[q] = 0.0; xmax[q] = 0.0; }
-      double arow = 0.0;
-#pragma unroll 1
-      for (int r = tid; r < n; r += NT) {
-        double acc[kMaxRhs];
-#pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q) {
-          const double b = Bs[r + min(q, nrhs - 1) * n];
-          acc[q] = q < nrhs ? b : 0.0;
-        }
-        double asum = 0.0;
-        double av[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) av[j] = As[r + min(j, n - 1) * n];
-        for (int t0 = 0; t0 < n; t0 += 8) {
-          double an[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) an[j] = As[r + min(t0 + 8 + j, n - 1) * n];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            if (t0 + j < n) {
-              asum += fabs(av[j]);
-#pragma unroll
-              for (int q = 0; q < kMaxRhs; ++q)
-                if (q < nrhs) acc[q] -= av[j] * xs[(t0 + j) * kMaxRhs + q];
-            }
-            // bounds the X values read ahead (register pressure)
-            if ((j & 3) == 3) asm volatile("" ::: "memory");
-          }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) av[j] = an[j];
-        }
-        arow = fmax(arow, absinf(asum));
-#pragma unroll
-        for (int q = 0; q < kMaxRhs; ++q)
-          if (q < nrhs) {
-            rv[r * kMaxRhs + q] = (TF)acc[q];
-            rmax[q] = fmax(rmax[q], absinf(acc[q]));
-            xmax[q] = fmax(xmax[q], absinf(xs[r * kMaxRhs + q]));
-          }
-      }
-      if (it == 0) anorm = block_max<NW>(arow, red_d, lane, wave);
+      // R = B - A X and the norms of the convergence test
       double worst = 0.0;
       bool bad = false;
-#pragma unroll
-      for (int q = 0; q < kMaxRhs; ++q)
-        if (q < nrhs) {
-          const double rn = block_max<NW>(rmax[q], red_d, lane, wave);
-          const double xn = block_max<NW>(xmax[q], red_d, lane, wave);
-          const double den = anorm * xn + s_bnorm[q];
-          const double be = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
-          if (!(be <= kTol)) bad = true;
-          worst = (be == be) ? fmax(worst, be) : INFINITY;
-        }
+      if (nrhs <= 4)
+        residual<TF, NT, 4>(As, Bs, n, nrhs, xs, rv, tid, lane, wave, red_d, s_bnorm, it == 0,
+                            anorm, worst, bad);
+      else
+        residual<TF, NT, kMaxRhs>(As, Bs, n, nrhs, xs, rv, tid, lane, wave, red_d, s_bnorm,
+                                  it == 0, anorm, worst, bad);
       if (!bad) { converged = 1; break; }
       // stop on the iteration limit, or when a step gains less than 2x
       if (it >= kMaxIter || !(worst < 0.5 * prev)) break;
       prev = worst;
     }
-    if (tid == 0) {
+    if (thread_index(swave) == 0) {
       if (converged) {
         info[s] = kRefactor ? 1 : 0;
         if (iters) iters[s] = it;
