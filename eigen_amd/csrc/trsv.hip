@@ -58,6 +58,13 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
   __shared__ T accs[BS][NRC];
   __shared__ unsigned xsw[2][CHUNK];  // staged solution of the current predecessor
   __shared__ int s_abort;
+  // FP32, one right-hand side: M = inv(T_bb) T_bl for the last predecessor l
+  // is formed while the block waits, and inv(T_bb) (B_b - sum over the other
+  // predecessors) before l's solution arrives, so the step after it is one
+  // row of M times X_l (one barrier instead of two, no reduction through LDS)
+  constexpr bool kM = NRC == 1 && sizeof(T) == 4;
+  __shared__ T Tl[kM ? BS : 1][BS + 1];
+  __shared__ unsigned xwv[kM ? NT / 64 : 1][BS];  // (kM) each wave's copy of X_l
   const int tid = threadIdx.x;
   const int nblk = (n + BS - 1) / BS;
   const int w = blockIdx.x;
@@ -66,6 +73,8 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
   const int bs = min(BS, n - rb0);
   const int nc = min(NRC, nrhs - c0);
   const int r = tid >> 2, q = tid & 3;  // my row of the block, my quarter of each column block
+  const int ndep = LOWER ? b : nblk - 1 - b;
+  const int jl = LOWER ? b - 1 : b + 1;  // the last predecessor (ndep > 0)
 
   // ---- diagonal block into LDS and its inverse (independent of the
   // predecessors: overlaps their solves)
@@ -73,6 +82,10 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
     const int rr = e / BS, cc = e - rr * BS;
     Tb[rr][cc] = (rr < bs && cc < bs) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + rr, rb0 + cc) : T(0);
     Dinv[rr][cc] = T(0);  // the padding of a partial last block stays exactly zero
+    if constexpr (kM)
+      Tl[rr][cc] = (ndep > 0 && rr < bs && jl * BS + cc < n)
+                       ? *tptr<T, ROWMAJOR>(A, lda, rb0 + rr, jl * BS + cc)
+                       : T(0);
   }
   if (tid == 0) s_abort = 0;
   __syncthreads();
@@ -104,11 +117,33 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
   // workgroup stages its solution from the exchange words into LDS (one
   // barrier); the matrix values, which do not depend on X, are prefetched
   // one predecessor ahead.
+  // (kM) this thread's quarter of row r of M
+  T mrow[kM ? SEG : 1];
+  if constexpr (kM) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SEG; ++i) mrow[i] = T(0);
+    if (ndep > 0) {
+#pragma unroll 4
+      for (int k = 0; k < BS; ++k) {
+        const T dk = Dinv[r][k];
+#pragma unroll
+        for (int i = 0; i < SEG; ++i) mrow[i] += dk * Tl[k][q * SEG + i];
+      }
+    }
+  }
   T acc[NRC];
 #pragma unroll
   for (int c = 0; c < NRC; ++c) acc[c] = T(0);
-  const int ndep = LOWER ? b : nblk - 1 - b;
+  // predecessors taken in the loop (kM: all but the last)
+  const int nloop = kM ? max(ndep - 1, 0) : ndep;
   const bool rvalid = r < bs;
+  // the block's right-hand sides, requested now: read after the loop they
+  // would put a memory round trip on every link of the chain
+  T bval[NRC];
+#pragma unroll
+  for (int c = 0; c < NRC; ++c)
+    bval[c] = (q == 0 && rvalid && c < nc) ? X[(int64_t)(c0 + c) * ldx + rb0 + r] : T(0);
   // a row-major segment is contiguous: 16-byte loads when aligned (a quarter
   // or half of the address work of element loads, which the CU's address
   // unit otherwise limits)
@@ -140,11 +175,11 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
         tv[i] = (rvalid && cb + i < n) ? *tptr<T, ROWMAJOR>(A, lda, rb0 + r, cb + i) : T(0);
     }
   };
-  T tcur[SEG], tnext[SEG];
-  if (ndep > 0) load_seg(0, tcur);
   int bad = 0;
-  for (int d = 0; d < ndep && !bad; ++d) {
-    if (d + 1 < ndep) load_seg(d + 1, tnext);
+  // the solution of predecessor step d, from its exchange words into
+  // xsw[d & 1] (my words all requested together, then only the words not
+  // yet published read again), then one barrier
+  auto stage = [&](int d) {
     const int jb = LOWER ? d : nblk - 1 - d;
     const int par = d & 1;
     // my words of the predecessor's solution (row jr, column c, half h): all
@@ -193,7 +228,16 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
     }
     if (bad) s_abort = 1;
     __syncthreads();
-    if (s_abort) { bad = 1; break; }
+    if (s_abort) bad = 1;
+  };
+  T tcur[SEG], tnext[SEG];
+  if (nloop > 0) load_seg(0, tcur);
+  for (int d = 0; d < nloop && !bad; ++d) {
+    if (d + 1 < nloop) load_seg(d + 1, tnext);
+    const int jb = LOWER ? d : nblk - 1 - d;
+    const int par = d & 1;
+    stage(d);
+    if (bad) break;
     const int cb = jb * BS + q * SEG;
 #pragma unroll
     for (int c = 0; c < NRC; ++c) {
@@ -228,7 +272,7 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
   if (q == 0) {
 #pragma unroll
     for (int c = 0; c < NRC; ++c)
-      accs[r][c] = (rvalid && c < nc) ? X[(int64_t)(c0 + c) * ldx + rb0 + r] - acc[c] : T(0);
+      accs[r][c] = (rvalid && c < nc) ? bval[c] - acc[c] : T(0);
   }
   __syncthreads();
   if (s_abort) return;
@@ -245,6 +289,57 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
   for (int c = 0; c < NRC; ++c) {
     xs[c] += __shfl_xor(xs[c], 1);
     xs[c] += __shfl_xor(xs[c], 2);
+  }
+  if constexpr (kM) {
+    // X_b -= M X_l once l's solution is here: every wave polls X_l's BS
+    // words itself (lane l: words l, l + 64) and publishes its own rows, so
+    // no workgroup barrier sits on the chain
+    if (ndep > 0) {
+      constexpr int WL = BS / 64;
+      const int lane = tid & 63, wv = tid >> 6;
+      const u64* src[WL];
+      bool need[WL];
+      u64 w[WL];
+#pragma unroll
+      for (int k = 0; k < WL; ++k) {
+        const int row = jl * BS + lane + 64 * k;
+        need[k] = row < n;
+        src[k] = xw + (size_t)row;
+        w[k] = need[k] ? __hip_atomic_load(const_cast<u64*>(src[k]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)
+                       : (u64)epoch << 32;
+      }
+      auto missing = [&]() {
+        bool m = false;
+#pragma unroll
+        for (int k = 0; k < WL; ++k) m = m || (unsigned)(w[k] >> 32) != epoch;
+        return m;
+      };
+      for (int spins = 0; __ballot(missing()) != 0ull;) {
+        if ((++spins & 255) == 0 &&
+            (spins > kSpin ||
+             __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int k = 0; k < WL; ++k)
+          if ((unsigned)(w[k] >> 32) != epoch)
+            w[k] = __hip_atomic_load(const_cast<u64*>(src[k]), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // the wave's own LDS copy (a wave's LDS accesses execute in order)
+#pragma unroll
+      for (int k = 0; k < WL; ++k) xwv[wv][lane + 64 * k] = (unsigned)w[k];
+      asm volatile("" ::: "memory");
+      T m = T(0);
+#pragma unroll
+      for (int i = 0; i < SEG; ++i) m += mrow[i] * __uint_as_float(xwv[wv][q * SEG + i]);
+      m += __shfl_xor(m, 1);
+      m += __shfl_xor(m, 2);
+      xs[0] -= m;
+    }
   }
   // publish: the result, and its exchange words for the successors
   if (q == 0 && rvalid) {
@@ -294,11 +389,13 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
                 (double)n * n * nrhs);
   if (nrhs == 1) {
     // one right-hand side: one exchange word (per 32 payload bits) per row
-    // (128-row blocks measured slower: 6.5 vs 5.8 ms of solves per step at
+    // (128-row blocks measured slower: 7.4 vs 3.7 ms of solves per step at
     // n = 16384)
+    constexpr int BS1 = 64;
+    const int nblk1 = ceil_div(n, BS1);
     const unsigned epoch = (unsigned)++h->trsv_epoch;
-#define EAMD_TRSV1(RM, LO, UN)                                                                 \
-  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, 1>), dim3(nblk), dim3(4 * BS), 0,   \
+#define EAMD_TRSV1(RM, LO, UN)                                                                   \
+  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS1, 1>), dim3(nblk1), dim3(4 * BS1), 0, \
                      h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV1(true, true, true); else EAMD_TRSV1(true, true, false); }
