@@ -291,6 +291,7 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     int* __restrict__ pperm_n, int* __restrict__ ppos, int* __restrict__ psrc, int row0,
     int pnb, int* __restrict__ zero_col, u64* __restrict__ xbuf, int* __restrict__ err,
     unsigned epoch_base, int piv_offset, int* __restrict__ fz_global) {
+  __builtin_amdgcn_s_setprio(3);  // on the critical path (see tsl_kernel)
   constexpr int NW = Words<T>::n;
   constexpr int SW = slot_words<T>();
   constexpr int NWAVE = BLOCK / 64;
@@ -659,6 +660,9 @@ __global__ __launch_bounds__(64 * NW) void tsl_kernel(
     int* __restrict__ lsrc, int* __restrict__ pperm_n, int* __restrict__ ppos,
     int* __restrict__ psrc, int row0, int pnb, int* __restrict__ zero_col, int piv_offset,
     int* __restrict__ fz_global, int fuse, float* __restrict__ xtop) {
+  // the panel is the factorization's critical path while the trailing
+  // update's waves share its SIMDs: its waves take issue priority
+  __builtin_amdgcn_s_setprio(3);
   constexpr int R = kTslR;
   static_assert(MODE != 2 || NW == 1, "merge rounds run one wave per workgroup");
   // slot ids in the key's low KB bits: 2^KB = 2 * NW * 256 keeps every
@@ -1010,6 +1014,7 @@ template <bool FULL>
 __global__ __launch_bounds__(256) void tsl_apply_kernel(float* __restrict__ P, int64_t ldp, int m,
                                                         int w, const float* __restrict__ saved,
                                                         const int* __restrict__ moved) {
+  __builtin_amdgcn_s_setprio(3);  // on the critical path (see tsl_kernel)
   if (FULL) w = LW;
   __shared__ float U[LW][LW + 1];
   __shared__ int mpos[LW], msrc[LW];
@@ -1091,6 +1096,7 @@ __global__ __launch_bounds__(256) void tsl_apply_rest_kernel(float* __restrict__
                                                              const int* __restrict__ moved,
                                                              const float* __restrict__ xtop,
                                                              int rest) {
+  __builtin_amdgcn_s_setprio(3);  // on the critical path (see tsl_kernel)
   extern __shared__ float s_u12[];  // LW x rest
   __shared__ float U[LW][LW + 1];
   __shared__ float dinv[LW];

@@ -159,6 +159,9 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
                                                          const int* __restrict__ pos,
                                                          const int* __restrict__ src) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  // (LBLK: the next panel's interchanges and U rows, on the critical path
+  // beside the trailing update's waves: issue priority)
+  if (LBLK) __builtin_amdgcn_s_setprio(3);
   constexpr int XP = CB + 1;
   const int nbp = (nb + 31) & ~31, LP = nbp + 1, nbb = nbp / 32;
   // full: nbp x LP; blocks: (bi, bj), bi >= bj, 32 x 33 each at bi(bi+1)/2 + bj

@@ -433,6 +433,9 @@ __global__ __launch_bounds__(256) void f32_update_narrow_kernel(int M, int N, in
                                                                 const float* __restrict__ A12,
                                                                 const float* __restrict__ A21,
                                                                 int64_t ld, float* __restrict__ C) {
+  // the next panel's columns: on the critical path beside the trailing
+  // update's waves (issue priority)
+  __builtin_amdgcn_s_setprio(3);
   constexpr int NT = 32, KH = 64;
   __shared__ float As[KH][128 + 4];  // A12: k x column
   __shared__ float Bs[KH][NT + 4];   // A21 transposed: k x row
