@@ -280,6 +280,8 @@ __global__ __launch_bounds__(256) void skinny_nt_kernel(const double* __restrict
                                                         int64_t ldc, int m, int nc,
                                                         const int* __restrict__ fail,
                                                         double* __restrict__ Bc) {
+  // the factorization's chain, beside the SYRK's waves: issue priority
+  __builtin_amdgcn_s_setprio(3);
   constexpr int RB = 16, LDA_S = RB + 1;
   __shared__ double As[kMaxNb * LDA_S];  // As[k * 17 + r]
   if (*fail >= 0) return;
