@@ -222,7 +222,25 @@ __global__ __launch_bounds__(256) void laswp_trsm_kernel(const float* __restrict
   }
   __syncthreads();
   for (int b = 0; b < nb; b += 32) {
-    if (tid < CB) {
+    if constexpr (LBLK) {
+      // (the critical-path instance) the diagonal block by substitution with
+      // lanes over rows: a wave takes two columns (lanes 0..31 and 32..63,
+      // row i = lane % 32), each step's solved value broadcast by readlane
+      // (the same operations in the same order as the per-column loop below,
+      // without its dependent chain of 496 FMAs)
+      for (int cp = 2 * (tid >> 6); cp < CB; cp += 2 * 4) {
+        const int lane = tid & 63, i = lane & 31, c = cp + (lane >> 5);
+        float x = c < CB ? Xs[(b + i) * XP + c] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 31; ++t) {
+          const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), t));
+          const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32 + t));
+          const float xt = lane < 32 ? lo : hi;
+          if (i > t) x -= lat(b + i, b + t) * xt;
+        }
+        if (c < CB) Xs[(b + i) * XP + c] = x;
+      }
+    } else if (tid < CB) {
       float x[32];
 #pragma unroll
       for (int i = 0; i < 32; ++i) x[i] = Xs[(b + i) * XP + tid];
