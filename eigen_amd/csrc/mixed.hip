@@ -30,7 +30,15 @@ namespace {
 
 constexpr int MT = 128;   // C tile (M and N), also the scale block
 constexpr int KS = 32;    // K per MFMA slab (v_mfma_f32_16x16x32_{f16,bf16})
-constexpr int LROW = 40;  // LDS row length in halves (64 B of data + 16 B pad)
+constexpr int LROW = 32;  // LDS row length in halves (64 B, four 16-byte chunks)
+
+// 16-byte chunk c of LDS row r is stored at chunk c ^ lds_swz(r).  Without
+// padding, the 16 rows that one MFMA operand read touches then fall on
+// distinct 16-byte bank groups, for 8 lanes per LDS cycle on 32 banks and 16
+// lanes on 64 banks alike.  Unpadded rows keep the two buffers of the four
+// arrays at 64 KB per workgroup: two workgroups leave 32 KB of a CU's LDS to
+// the panel kernels that run beside the update.
+__device__ __forceinline__ int lds_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
 enum { kModeF16 = 0, kModeF16x3 = 1, kModeBF16 = 2 };
 
@@ -229,6 +237,9 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
   // staging: each thread moves 32 bytes of each array per slab:
   // row = tid / 2, byte half = (tid & 1) * 32 of the 64-byte K slab
   const int srow = tid >> 1, shalf = (tid & 1) * 16;  // in halves
+  const int swz = lds_swz(srow);
+  // the operand rows below are 16-aligned bases + l16: their swizzle is l16's
+  const int rch = (lk ^ lds_swz(l16)) * 8;
   const unsigned short* src[4] = {Pah, Pbh, Pal, Pbl};
   uint4 stg[4][2];
   auto gload = [&](int k0) {
@@ -251,9 +262,9 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int a = 0; a < NARR; ++a) {
-      uint4* d = reinterpret_cast<uint4*>(&lds[buf][a][srow * LROW + shalf]);
-      d[0] = stg[a][0];
-      d[1] = stg[a][1];
+      unsigned short* d = &lds[buf][a][srow * LROW];
+      *reinterpret_cast<uint4*>(d + (((shalf >> 3) ^ swz) << 3)) = stg[a][0];
+      *reinterpret_cast<uint4*>(d + ((((shalf >> 3) + 1) ^ swz) << 3)) = stg[a][1];
     }
   };
 
@@ -288,15 +299,15 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
     uint4 af[4], bf[4], afl[4], bfl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      af[i] = *reinterpret_cast<const uint4*>(&lds[buf][0][(wm + i * 16 + l16) * LROW + lk * 8]);
+      af[i] = *reinterpret_cast<const uint4*>(&lds[buf][0][(wm + i * 16 + l16) * LROW + rch]);
       if (X3)
-        afl[i] = *reinterpret_cast<const uint4*>(&lds[buf][2][(wm + i * 16 + l16) * LROW + lk * 8]);
+        afl[i] = *reinterpret_cast<const uint4*>(&lds[buf][2][(wm + i * 16 + l16) * LROW + rch]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      bf[j] = *reinterpret_cast<const uint4*>(&lds[buf][1][(wn + j * 16 + l16) * LROW + lk * 8]);
+      bf[j] = *reinterpret_cast<const uint4*>(&lds[buf][1][(wn + j * 16 + l16) * LROW + rch]);
       if (X3)
-        bfl[j] = *reinterpret_cast<const uint4*>(&lds[buf][3][(wn + j * 16 + l16) * LROW + lk * 8]);
+        bfl[j] = *reinterpret_cast<const uint4*>(&lds[buf][3][(wn + j * 16 + l16) * LROW + rch]);
     }
     // small terms first, so each tile's sum ends with its largest part
     if (X3) {
@@ -427,8 +438,9 @@ __global__ __launch_bounds__(256) void f32_update_kernel(int M, int N, int K,
 // workgroup per 32 rows, each wave 32 of the columns.  Every operand word is
 // loaded up front (the whole A12, the workgroup's A21 rows and its C tile),
 // so a single memory round trip precedes the FP32 MFMA work; K passes
-// through LDS in two halves of 64.  M, K and ld are multiples of 4 and the
-// operands 16-byte aligned (checked by the caller).
+// through LDS in four quarters of 32 (21 KB of LDS: the workgroup fits on a
+// CU beside two trailing-update workgroups).  M, K and ld are multiples of 4
+// and the operands 16-byte aligned (checked by the caller).
 __global__ __launch_bounds__(256) void f32_update_narrow_kernel(int M, int N, int K,
                                                                 const float* __restrict__ A12,
                                                                 const float* __restrict__ A21,
@@ -436,7 +448,7 @@ __global__ __launch_bounds__(256) void f32_update_narrow_kernel(int M, int N, in
   // the next panel's columns: on the critical path beside the trailing
   // update's waves (issue priority)
   __builtin_amdgcn_s_setprio(3);
-  constexpr int NT = 32, KH = 64;
+  constexpr int NT = 32, KH = 32;
   __shared__ float As[KH][128 + 4];  // A12: k x column
   __shared__ float Bs[KH][NT + 4];   // A21 transposed: k x row
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -477,16 +489,15 @@ __global__ __launch_bounds__(256) void f32_update_narrow_kernel(int M, int N, in
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    if (h) __syncthreads();  // every read of the first half is done
+  for (int h = 0; h < 4; ++h) {
+    if (h) __syncthreads();  // every read of the previous quarter is done
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      *reinterpret_cast<v4f*>(&As[(tid >> 5) + 8 * t][(tid & 31) * 4]) = ra[8 * h + t];
+    for (int t = 0; t < 4; ++t)
+      *reinterpret_cast<v4f*>(&As[(tid >> 5) + 8 * t][(tid & 31) * 4]) = ra[4 * h + t];
+    {
+      const int kk = (tid & 7) * 4;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int kk = (tid & 7) * 4 + 32 * t;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) Bs[kk + e][tid >> 3] = rb[2 * h + t][e];
+      for (int e = 0; e < 4; ++e) Bs[kk + e][tid >> 3] = rb[h][e];
     }
     __syncthreads();
 #pragma unroll
