@@ -525,9 +525,14 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
   // a grid-wide exchange): the handle's stream factors panel k (POTF2 of the
   // diagonal block, then A21 L11^-T), updates the next panel's columns only
   // (rows below the panel, lower part of their diagonal block) and goes on
-  // to panel k+1; the second stream, started by an event after panel k,
-  // does the SYRK of the rest of the trailing matrix.  The handle's stream
-  // waits for it only before updating the next panel's columns again.
+  // to panel k+1; the second stream, started by an event after the next
+  // panel's columns are updated, does the SYRK of the rest of the trailing
+  // matrix.  The handle's stream waits for it only before updating the next
+  // panel's columns again.  The SYRK starts after that column update (not
+  // straight after panel k) so that panel k+1's POTF2, whose 157 KB of LDS
+  // needs a CU of its own, is dispatched before the SYRK's workgroups (two
+  // of 65 KB per CU) occupy every CU; started earlier, the SYRK made each
+  // POTF2 wait for its tail (13.9 vs 13.4 ms for n = 8192).
   static const bool lookahead = env_int("EAMD_LOOKAHEAD", 1) != 0;
   hipStream_t sb = nullptr;
   if (lookahead && n > nb) {
@@ -555,13 +560,13 @@ int dpotrf_lower(eamd_handle* h, int n, double* A, int64_t lda, int64_t* fail_co
         continue;
       }
       const int par = it & 1, n1 = min(nb, rs), n2 = rs - n1;
-      EAMD_HIP(hipEventRecord(evP, s));  // the panel is factored
       // the next panel's columns were last written by the previous panel's
       // second-stream update
       if (it > 0) EAMD_HIP(hipStreamWaitEvent(s, evB[par ^ 1], 0));
       hipLaunchKernelGGL(skinny_nt_kernel<1>, dim3(ceil_div(rs, 16)), dim3(256), 0, s,
                          (const double*)Bc, (int64_t)kMaxNb, bs, (const double*)A21, lda, A22,
                          lda, rs, n1, (const int*)fail, (double*)nullptr);
+      EAMD_HIP(hipEventRecord(evP, s));  // panel factored, next panel's columns updated
       if (n2 > 0) {
         EAMD_HIP(hipStreamWaitEvent(sb, evP, 0));
         side_used = true;
