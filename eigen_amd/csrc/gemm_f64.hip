@@ -16,6 +16,7 @@
 // dispatcher deals workgroups round-robin over the 8 XCDs, each with its own
 // L2) and swizzled in groups of 8 tile-rows for L2 reuse of B.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -127,9 +128,134 @@ __device__ __forceinline__ void store_slab(double* As, double* Bs, const double 
   }
 }
 
+// The products of one BK slab, staged in as / bs, into the 4 x 4 tiles.
+__device__ __forceinline__ void slab_products(const double* as, const double* bs, int wm, int wn,
+                                              int l16, int lk, v4d (&acc)[4][4]) {
+#pragma unroll
+  for (int kk = 0; kk < BK / 4; ++kk) {
+    const int krow = (kk * 4 + lk) * LDS_LD;
+    double af[4], bf[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) af[i] = as[krow + wm + i * 16 + l16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bf[j] = bs[krow + wn + j * 16 + l16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+  }
+}
+
+// The K loop of C += alpha op(A) op(B) (alpha = +-1) with the C tile read in
+// eight parts of eight words per lane, one part beside each of the first
+// eight slabs' operand loads and added into the accumulators one slab
+// later.  Loading the whole tile before the first slab made every
+// workgroup of a launch wait for 128 KB of C at once: with a short inner
+// dimension (the rank-128 and rank-256 trailing updates of the blocked
+// factorizations) that burst was a large share of the tile's time (LLT
+// n = 8192: 13.35 -> 12.18 ms; FP64 LU n = 16384: 195 -> 178 ms).  Part p
+// holds tiles (i, j) = (p / 2, 2 (p % 2) + {0, 1}).
+template <bool TA, bool TB>
+__device__ __forceinline__ void dgemm_body_cin(int m, int n, int k, double alpha,
+                                                const double* __restrict__ A, int64_t lda,
+                                                const double* __restrict__ B, int64_t ldb,
+                                                const double* __restrict__ C, int64_t ldc, int m0,
+                                                int n0, double (*As)[BK * LDS_LD],
+                                                double (*Bs)[BK * LDS_LD], v4d (&acc)[4][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  double cb[8];
+  auto cload = [&](auto P) {
+    constexpr int p = decltype(P)::value, i = p >> 1, j0 = (p & 1) * 2;
+    const int gi = m0 + wm + i * 16 + l16;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn + (j0 + jj) * 16 + lk + 4 * r;
+        cb[jj * 4 + r] = (gi < m && gj < n) ? C[(int64_t)gj * ldc + gi] : 0.0;
+      }
+  };
+  auto cadd = [&](auto P) {
+    constexpr int p = decltype(P)::value, i = p >> 1, j0 = (p & 1) * 2;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j0 + jj][r] += alpha * cb[jj * 4 + r];
+  };
+  const int ktiles = (k + BK - 1) / BK;
+  double ra[8], rb[8];
+  if (ktiles > 0) {
+    load_slab<TA, TB>(A, lda, B, ldb, m0, n0, 0, m, n, k, ra, rb);
+    store_slab<TA, TB>(As[0], Bs[0], ra, rb);
+  }
+  __syncthreads();
+  // slab p with C part p (a slab that does not exist only moves C)
+  auto step = [&](auto P) {
+    constexpr int p = decltype(P)::value;
+    if (p < ktiles) {
+      if (p + 1 < ktiles)
+        load_slab<TA, TB>(A, lda, B, ldb, m0, n0, (p + 1) * BK, m, n, k, ra, rb);
+      if constexpr (p > 0) cadd(std::integral_constant<int, p - 1>{});
+      cload(P);
+      slab_products(As[p & 1], Bs[p & 1], wm, wn, l16, lk, acc);
+      if (p + 1 < ktiles) store_slab<TA, TB>(As[(p + 1) & 1], Bs[(p + 1) & 1], ra, rb);
+      __syncthreads();
+    } else {
+      if constexpr (p > 0) cadd(std::integral_constant<int, p - 1>{});
+      cload(P);
+    }
+  };
+  step(std::integral_constant<int, 0>{});
+  step(std::integral_constant<int, 1>{});
+  step(std::integral_constant<int, 2>{});
+  step(std::integral_constant<int, 3>{});
+  step(std::integral_constant<int, 4>{});
+  step(std::integral_constant<int, 5>{});
+  step(std::integral_constant<int, 6>{});
+  step(std::integral_constant<int, 7>{});
+  cadd(std::integral_constant<int, 7>{});
+  // the rest of K: the plain loop (written out, not through slab_products,
+  // whose per-slab LDS address arithmetic cost the 8192^3 product 5 %)
+  for (int kt = 8; kt < ktiles; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < ktiles) load_slab<TA, TB>(A, lda, B, ldb, m0, n0, (kt + 1) * BK, m, n, k, ra, rb);
+    const double* as = As[buf];
+    const double* bs = Bs[buf];
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int krow = (kk * 4 + lk) * LDS_LD;
+      double af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = as[krow + wm + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = bs[krow + wn + j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < ktiles) store_slab<TA, TB>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+    __syncthreads();
+  }
+}
+
 // MODE 0: full C.  MODE 1: lower triangle only (i >= j), tiles strictly above
 // the diagonal are skipped.  MODE 2: upper triangle only (i <= j).
-template <bool TA, bool TB, int MODE>
+// CIN: beta == 1 and alpha == +-1 (the trailing updates of the blocked
+// factorizations): C is read during the first K slabs and added into the
+// accumulators (dgemm_body_cin); the epilogue then only stores alpha * acc
+// (alpha^2 = 1, both scalings exact).  Otherwise C is read in the epilogue.
+// The two K loops are separate instantiations, so neither constrains the
+// other's registers or code layout.
+template <bool TA, bool TB, int MODE, bool CIN>
 __global__ __launch_bounds__(256, 2) void dgemm_kernel(int m, int n, int k, double alpha,
                                                     const double* __restrict__ A, int64_t lda,
                                                     const double* __restrict__ B, int64_t ldb,
@@ -159,61 +285,44 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(int m, int n, int k, doub
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int l16 = lane & 15, lk = lane >> 4;
 
-  // With beta == 1 and alpha == +-1 (the trailing updates of the blocked
-  // factorizations) the accumulators start from C / alpha, loaded before
-  // the first K slab so the C tile's latency overlaps the operand loads
-  // instead of following the last MFMA; the epilogue then only stores
-  // alpha * acc (both scalings exact).
-  const bool cinit = beta == 1.0 && (alpha == 1.0 || alpha == -1.0);
   v4d acc[4][4];
-  if (cinit) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int gi = m0 + wm + i * 16 + l16;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gj = n0 + wn + j * 16 + lk + 4 * r;
-          acc[i][j][r] = (gi < m && gj < n) ? alpha * C[(int64_t)gj * ldc + gi] : 0.0;
-        }
-    }
+  if constexpr (CIN) {
+    dgemm_body_cin<TA, TB>(m, n, k, alpha, A, lda, B, ldb, C, ldc, m0, n0, As, Bs, acc);
   } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-  }
-
-  const int ktiles = (k + BK - 1) / BK;
-  double ra[8], rb[8];
-  if (ktiles > 0) {
-    load_slab<TA, TB>(A, lda, B, ldb, m0, n0, 0, m, n, k, ra, rb);
-    store_slab<TA, TB>(As[0], Bs[0], ra, rb);
-  }
-  __syncthreads();
-
-  for (int kt = 0; kt < ktiles; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < ktiles) load_slab<TA, TB>(A, lda, B, ldb, m0, n0, (kt + 1) * BK, m, n, k, ra, rb);
-    const double* as = As[buf];
-    const double* bs = Bs[buf];
-#pragma unroll
-    for (int kk = 0; kk < BK / 4; ++kk) {
-      const int krow = (kk * 4 + lk) * LDS_LD;
-      double af[4], bf[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = as[krow + wm + i * 16 + l16];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = bs[krow + wn + j * 16 + l16];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+    const int ktiles = (k + BK - 1) / BK;
+    double ra[8], rb[8];
+    if (ktiles > 0) {
+      load_slab<TA, TB>(A, lda, B, ldb, m0, n0, 0, m, n, k, ra, rb);
+      store_slab<TA, TB>(As[0], Bs[0], ra, rb);
     }
-    if (kt + 1 < ktiles) store_slab<TA, TB>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
     __syncthreads();
+    for (int kt = 0; kt < ktiles; ++kt) {
+      const int buf = kt & 1;
+      if (kt + 1 < ktiles)
+        load_slab<TA, TB>(A, lda, B, ldb, m0, n0, (kt + 1) * BK, m, n, k, ra, rb);
+      const double* as = As[buf];
+      const double* bs = Bs[buf];
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; ++kk) {
+        const int krow = (kk * 4 + lk) * LDS_LD;
+        double af[4], bf[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) af[i] = as[krow + wm + i * 16 + l16];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bf[j] = bs[krow + wn + j * 16 + l16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+      }
+      if (kt + 1 < ktiles) store_slab<TA, TB>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+      __syncthreads();
+    }
   }
 
   // epilogue: v_mfma_f64_16x16x4f64 leaves result row lk + 4r of the 16x16
@@ -233,7 +342,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(int m, int n, int k, doub
         if (MODE == 2 && gi > gj) continue;
         double* c = C + (int64_t)gj * ldc + gi;
         double v = alpha * acc[i][j][r];
-        if (!cinit && beta != 0.0) v += beta * *c;
+        if (!CIN && beta != 0.0) v += beta * *c;
         *c = v;
       }
   }
@@ -245,8 +354,12 @@ void launch_t(hipStream_t s, int m, int n, int k, double alpha, const double* A,
               const int* abort_flag) {
   const int tm = ceil_div(m, BM), tn = ceil_div(n, BN);
   const int tiles = MODE == 0 ? tm * tn : (MODE == 1 ? tri_tiles(tm, tn) : tri_tiles(tn, tm));
-  hipLaunchKernelGGL((dgemm_kernel<TA, TB, MODE>), dim3(tiles), dim3(256), 0, s, m, n, k, alpha,
-                     A, lda, B, ldb, beta, C, ldc, abort_flag);
+  if (beta == 1.0 && (alpha == 1.0 || alpha == -1.0))
+    hipLaunchKernelGGL((dgemm_kernel<TA, TB, MODE, true>), dim3(tiles), dim3(256), 0, s, m, n, k,
+                       alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  else
+    hipLaunchKernelGGL((dgemm_kernel<TA, TB, MODE, false>), dim3(tiles), dim3(256), 0, s, m, n,
+                       k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
 }
 
 // C = beta*C for k == 0 or alpha == 0 (BLAS semantics: beta == 0 clears C,
