@@ -19,6 +19,8 @@
 // The update is HBM-bound (C is read and written once per block step); see
 // the roofline accounting in the benchmark.
 #include "../../include/eigen_amd.h"
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -201,7 +203,7 @@ __device__ __forceinline__ v4f mfma16(const uint4& a, const uint4& b, v4f c) {
 // C (M x N column-major, ldc) -= (Pa . Pb) / (sa * sb).  Pa: M x KP
 // K-contiguous, Pb: N x KP K-contiguous.
 template <int MODE>
-__global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
+__global__ __launch_bounds__(256, 2) void mixed_update_kernel(int M, int N, int KP,
                                                            const unsigned short* __restrict__ Pah,
                                                            const unsigned short* __restrict__ Pal,
                                                            const unsigned short* __restrict__ Pbh,
@@ -270,14 +272,18 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
 
   const int kslabs = KP / KS;
   gload(0);
-  // the C tile is read while the products run.  The products are formed
-  // transposed (A-side operand first), so a lane's four results are four
-  // consecutive M indices 4*lk + r of one N index l16: one 16-byte word of
-  // the row-major C when ldc and C allow it
+  // The C tile is read in four parts of 16 rows per wave, part p beside
+  // slab p + 1's operand loads (the operand stores of a slab then wait only
+  // for their own loads), and is used only in the epilogue: a 64 KB burst of
+  // C per workgroup ahead of the first products made every workgroup of a
+  // launch wait for it at once.  The products are formed transposed (A-side
+  // operand first), so a lane's four results are four consecutive M indices
+  // 4*lk + r of one N index l16: one 16-byte word of the row-major C when
+  // ldc and C allow it.
   const bool cvec = (ldc & 3) == 0 && ((uintptr_t)C & 15) == 0;
   v4f cv[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  auto cpart = [&](auto I) {
+    constexpr int i = decltype(I)::value;
     const int gi0 = m0 + wm + i * 16 + lk * 4;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -290,12 +296,10 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
         for (int r = 0; r < 4; ++r) cv[i][j][r] = (gj < N && gi0 + r < M) ? pc[r] : 0.f;
       }
     }
-  }
-  lstore(0);
-  __syncthreads();
-  for (int ks = 0; ks < kslabs; ++ks) {
+  };
+  // the products of slab ks (staged in LDS buffer ks & 1)
+  auto slab = [&](int ks) {
     const int buf = ks & 1;
-    if (ks + 1 < kslabs) gload((ks + 1) * KS);
     uint4 af[4], bf[4], afl[4], bfl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -324,7 +328,30 @@ __global__ __launch_bounds__(256) void mixed_update_kernel(int M, int N, int KP,
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = mfma16<MODE>(af[i], bf[j], acc[i][j]);
-    if (ks + 1 < kslabs) lstore(buf ^ 1);
+  };
+  lstore(0);
+  __syncthreads();
+  // slabs 0..3 with C part p (a slab that does not exist only reads C)
+  auto step = [&](auto I) {
+    constexpr int p = decltype(I)::value;
+    if (p < kslabs) {
+      if (p + 1 < kslabs) gload((p + 1) * KS);
+      cpart(I);
+      slab(p);
+      if (p + 1 < kslabs) lstore((p + 1) & 1);
+      __syncthreads();
+    } else {
+      cpart(I);
+    }
+  };
+  step(std::integral_constant<int, 0>{});
+  step(std::integral_constant<int, 1>{});
+  step(std::integral_constant<int, 2>{});
+  step(std::integral_constant<int, 3>{});
+  for (int ks = 4; ks < kslabs; ++ks) {
+    if (ks + 1 < kslabs) gload((ks + 1) * KS);
+    slab(ks);
+    if (ks + 1 < kslabs) lstore((ks + 1) & 1);
     __syncthreads();
   }
 
