@@ -13,6 +13,10 @@
 // epoch (an aligned 8-byte store is single-copy atomic, so a word with the
 // right epoch has the right payload): consumers poll the words themselves,
 // with no flag and no cache write-back or invalidate on the critical path.
+// Several right-hand sides run as up to four independent chains of four
+// columns in one launch, their workgroups interleaved block by block.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -45,7 +49,7 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
                                                          int64_t lda, T* __restrict__ X,
                                                          int64_t ldx, int c0, int nrhs,
                                                          u64* __restrict__ xw, unsigned epoch,
-                                                         int* __restrict__ err) {
+                                                         int* __restrict__ err, int nch) {
   static_assert((BS == 64 || BS == 128) && (NRC == 16 || NRC == 4 || NRC == 1),
                 "four threads per row of a 64- or 128-row block");
   constexpr int NT = 4 * BS;   // threads
@@ -67,7 +71,13 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
   __shared__ unsigned xwv[kM ? NT / 64 : 1][BS];  // (kM) each wave's copy of X_l
   const int tid = threadIdx.x;
   const int nblk = (n + BS - 1) / BS;
-  const int w = blockIdx.x;
+  // nch independent chains of NRC columns each, interleaved block by block
+  // (workgroup w * nch + k is block w of chain k: every chain's blocks are
+  // dispatched in dependency order), each with its own exchange words
+  const int chain = blockIdx.x % nch;
+  const int w = blockIdx.x / nch;
+  c0 += chain * NRC;
+  xw += (size_t)chain * nblk * BS * NRC * NW;
   const int b = LOWER ? w : nblk - 1 - w;
   const int rb0 = b * BS;
   const int bs = min(BS, n - rb0);
@@ -370,10 +380,11 @@ template <typename T>
 int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, bool lower,
                 bool unit, T* X, int64_t ldx, int nrhs) {
   if (n <= 0 || nrhs <= 0) return kOk;
-  constexpr int BS = 64, NRC = 4, NRC16 = 16;
+  constexpr int BS = 64, NRC = 4, kChains = 4;
   const int nblk = ceil_div(n, BS);
-  // error word | exchange words (up to 16 values per row, 32 payload bits each)
-  const size_t nwords = (size_t)nblk * BS * NRC16 * (sizeof(T) / 4);
+  // error word | exchange words (kChains chains of NRC values per row, 32
+  // payload bits each)
+  const size_t nwords = (size_t)nblk * BS * NRC * kChains * (sizeof(T) / 4);
   size_t need = 256 + sizeof(unsigned long long) * nwords;
   void* base = nullptr;
   bool fresh = h->ws.bytes[kWsStatus] < need;
@@ -396,7 +407,7 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     const unsigned epoch = (unsigned)++h->trsv_epoch;
 #define EAMD_TRSV1(RM, LO, UN)                                                                   \
   hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS1, 1>), dim3(nblk1), dim3(4 * BS1), 0, \
-                     h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err)
+                     h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err, 1)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV1(true, true, true); else EAMD_TRSV1(true, true, false); }
       else { if (unit) EAMD_TRSV1(true, false, true); else EAMD_TRSV1(true, false, false); }
@@ -408,20 +419,18 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     EAMD_HIP(hipGetLastError());
     return kOk;
   }
-  // more than four right-hand sides: sixteen per chain (each chain step's
-  // latency is paid once for all of them), four for the remainder
+  // several right-hand sides: chains of four columns, up to four chains in
+  // one launch.  A chain's step costs about linearly in its columns (n =
+  // 8192: 24 us per 64-row step with sixteen, 6.5 us with four), so chains
+  // run side by side instead of one wide chain (LLT solve with 16
+  // right-hand sides: 6.3 ms with one chain of sixteen, 6.7 ms with four
+  // chains of four one after the other).
   for (int c0 = 0; c0 < nrhs;) {
     const unsigned epoch = (unsigned)++h->trsv_epoch;
-    const bool wide = nrhs - c0 > NRC;
-#define EAMD_TRSV(RM, LO, UN)                                                                   \
-  do {                                                                                          \
-    if (wide)                                                                                   \
-      hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC16>), dim3(nblk), dim3(256),  \
-                         0, h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err);            \
-    else                                                                                        \
-      hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC>), dim3(nblk), dim3(256), 0, \
-                         h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err);               \
-  } while (0)
+    const int nch = std::min(kChains, ceil_div(nrhs - c0, NRC));
+#define EAMD_TRSV(RM, LO, UN)                                                                  \
+  hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC>), dim3(nblk * nch), dim3(256), \
+                     0, h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err, nch)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV(true, true, true); else EAMD_TRSV(true, true, false); }
       else { if (unit) EAMD_TRSV(true, false, true); else EAMD_TRSV(true, false, false); }
@@ -431,7 +440,7 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     }
 #undef EAMD_TRSV
     EAMD_HIP(hipGetLastError());
-    c0 += wide ? NRC16 : NRC;
+    c0 += nch * NRC;
   }
   return kOk;
 }
