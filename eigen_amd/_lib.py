@@ -58,6 +58,11 @@ _SIGS = {
     "eamd_transpositions_to_permutation": ([vp, c_int64, vp, vp], c_int),
     "eamd_dpotrf": ([vp, c_char, c_int64, vp, c_int64, C.POINTER(c_int64)], c_int),
     "eamd_dpotrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, c_int64], c_int),
+    "eamd_sgetrf": ([vp, c_int64, c_int64, vp, c_int64, vp, C.POINTER(C.c_int32),
+                     C.POINTER(c_int64)], c_int),
+    "eamd_sgetrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, vp, c_int64], c_int),
+    "eamd_spotrf": ([vp, c_char, c_int64, vp, c_int64, C.POINTER(c_int64)], c_int),
+    "eamd_spotrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, c_int64], c_int),
     "eamd_ir_opts_default": ([C.POINTER(IrOpts)], None),
     "eamd_dgesv_ir": ([vp, c_int64, c_int64, vp, c_int64, vp, c_int64, vp, c_int64,
                        C.POINTER(IrOpts), C.POINTER(IrReport)], c_int),
@@ -80,6 +85,11 @@ _SIGS = {
     "eamd_host_dgetrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, vp, c_int64], c_int),
     "eamd_host_dpotrf": ([vp, c_char, c_int64, vp, c_int64, C.POINTER(c_int64)], c_int),
     "eamd_host_dpotrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, c_int64], c_int),
+    "eamd_host_sgetrf": ([vp, c_int64, c_int64, vp, c_int64, vp, C.POINTER(C.c_int32),
+                          C.POINTER(c_int64)], c_int),
+    "eamd_host_sgetrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, vp, c_int64], c_int),
+    "eamd_host_spotrf": ([vp, c_char, c_int64, vp, c_int64, C.POINTER(c_int64)], c_int),
+    "eamd_host_spotrs": ([vp, c_char, c_int64, c_int64, vp, c_int64, vp, c_int64], c_int),
     "eamd_host_dgemm": ([vp, c_char, c_char, c_int64, c_int64, c_int64, c_dbl, vp, c_int64, vp,
                          c_int64, c_dbl, vp, c_int64], c_int),
     "eamd_host_dsyrk": ([vp, c_char, c_char, c_int64, c_int64, c_dbl, vp, c_int64, c_dbl, vp,

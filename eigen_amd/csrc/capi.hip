@@ -228,6 +228,31 @@ bool is_diag(char c) { return c == 'N' || c == 'n' || c == 'U' || c == 'u'; }
 bool is_unit(char c) { return c == 'U' || c == 'u'; }
 bool is_side(char c) { return c == 'L' || c == 'l' || c == 'R' || c == 'r'; }
 
+// B = A converted (rows x cols, column-major, leading dimensions lda/ldb)
+template <typename Ti, typename To>
+__global__ void convert2d_kernel(int64_t rows, int64_t cols, const Ti* __restrict__ A,
+                                 int64_t lda, To* __restrict__ B, int64_t ldb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int64_t i = idx % rows, j = idx / rows;
+  B[j * ldb + i] = (To)A[j * lda + i];
+}
+
+template <typename Ti, typename To>
+int convert2d(hipStream_t s, int64_t rows, int64_t cols, const Ti* A, int64_t lda, To* B,
+              int64_t ldb) {
+  const int64_t blocks = (rows * cols + 255) / 256;
+  if (blocks <= 0) return kOk;
+  if (blocks > INT32_MAX) {
+    set_error("convert: %lld x %lld exceeds one launch", (long long)rows, (long long)cols);
+    return kErrUnsupported;
+  }
+  hipLaunchKernelGGL((convert2d_kernel<Ti, To>), dim3((unsigned)blocks), dim3(256), 0, s, rows,
+                     cols, A, lda, B, ldb);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
 __global__ void scale_kernel(int m, int n, double alpha, double* B, int64_t ldb) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)m * n) return;
@@ -563,6 +588,91 @@ int eamd_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const doubl
   return trsv_check_error(h);
 }
 
+// ------------------------------------------------- FP32 single matrices
+// FP32 storage, FP64 arithmetic: widen into the handle's conversion buffer,
+// run the FP64 routine, round the results back.  Every entry point returns
+// with its outputs complete, as the FP64 ones do.
+int eamd_sgetrf(eamd_handle_t h, int64_t m, int64_t n, float* A, int64_t lda, int32_t* transp,
+                int32_t* nb_transp, int64_t* first_zero_pivot) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max<int64_t>(1, m)) return -4;
+  CHECK_HANDLE(h);
+  if (nb_transp) *nb_transp = 0;
+  if (first_zero_pivot) *first_zero_pivot = -1;
+  if (m == 0 || n == 0) return kOk;
+  if (m > 1000000 || n > 1000000) return kErrUnsupported;
+  void* cp = nullptr;
+  EAMD_CHECK(h->ws.get(kWsConv, sizeof(double) * (size_t)m * n, &cp));
+  double* D = static_cast<double*>(cp);
+  EAMD_CHECK(convert2d<float, double>(h->stream, m, n, A, lda, D, m));
+  EAMD_CHECK(eamd_dgetrf(h, m, n, D, m, transp, nb_transp, first_zero_pivot));
+  EAMD_CHECK(convert2d<double, float>(h->stream, m, n, D, m, A, lda));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_sgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const float* LU,
+                int64_t lda, const int32_t* transp, float* B, int64_t ldb) {
+  if (!is_op(trans)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max<int64_t>(1, n)) return -5;
+  if (ldb < std::max<int64_t>(1, n)) return -8;
+  CHECK_HANDLE(h);
+  if (n == 0 || nrhs == 0) return kOk;
+  void* cp = nullptr;
+  EAMD_CHECK(h->ws.get(kWsConv, sizeof(double) * (size_t)n * (n + nrhs), &cp));
+  double* D = static_cast<double*>(cp);
+  double* DB = D + (size_t)n * n;
+  EAMD_CHECK(convert2d<float, double>(h->stream, n, n, LU, lda, D, n));
+  EAMD_CHECK(convert2d<float, double>(h->stream, n, nrhs, B, ldb, DB, n));
+  EAMD_CHECK(eamd_dgetrs(h, trans, n, nrhs, D, n, transp, DB, n));
+  EAMD_CHECK(convert2d<double, float>(h->stream, n, nrhs, DB, n, B, ldb));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_spotrf(eamd_handle_t h, char uplo, int64_t n, float* A, int64_t lda,
+                int64_t* fail_col) {
+  if (!is_uplo(uplo)) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max<int64_t>(1, n)) return -4;
+  CHECK_HANDLE(h);
+  if (fail_col) *fail_col = -1;
+  if (n == 0) return kOk;
+  void* cp = nullptr;
+  EAMD_CHECK(h->ws.get(kWsConv, sizeof(double) * (size_t)n * n, &cp));
+  double* D = static_cast<double*>(cp);
+  EAMD_CHECK(convert2d<float, double>(h->stream, n, n, A, lda, D, n));
+  EAMD_CHECK(eamd_dpotrf(h, uplo, n, D, n, fail_col));
+  // the triangle not referenced keeps its values (FP32 -> FP64 -> FP32 is exact)
+  EAMD_CHECK(convert2d<double, float>(h->stream, n, n, D, n, A, lda));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_spotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const float* A,
+                int64_t lda, float* B, int64_t ldb) {
+  if (!is_uplo(uplo)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max<int64_t>(1, n)) return -5;
+  if (ldb < std::max<int64_t>(1, n)) return -7;
+  CHECK_HANDLE(h);
+  if (n == 0 || nrhs == 0) return kOk;
+  void* cp = nullptr;
+  EAMD_CHECK(h->ws.get(kWsConv, sizeof(double) * (size_t)n * (n + nrhs), &cp));
+  double* D = static_cast<double*>(cp);
+  double* DB = D + (size_t)n * n;
+  EAMD_CHECK(convert2d<float, double>(h->stream, n, n, A, lda, D, n));
+  EAMD_CHECK(convert2d<float, double>(h->stream, n, nrhs, B, ldb, DB, n));
+  EAMD_CHECK(eamd_dpotrs(h, uplo, n, nrhs, D, n, DB, n));
+  EAMD_CHECK(convert2d<double, float>(h->stream, n, nrhs, DB, n, B, ldb));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
 // ------------------------------------------------------------ mixed / IR
 void eamd_ir_opts_default(eamd_ir_opts* o) {
   if (!o) return;
@@ -811,17 +921,19 @@ struct DevBuf {
   }
 };
 
-int up2d(eamd_handle* h, double* d, int64_t ldd, const double* src, int64_t lds, int64_t rows,
+template <typename T>
+int up2d(eamd_handle* h, T* d, int64_t ldd, const T* src, int64_t lds, int64_t rows,
          int64_t cols) {
-  EAMD_HIP(hipMemcpy2DAsync(d, ldd * sizeof(double), src, lds * sizeof(double),
-                            rows * sizeof(double), cols, hipMemcpyHostToDevice, h->stream));
+  EAMD_HIP(hipMemcpy2DAsync(d, ldd * sizeof(T), src, lds * sizeof(T), rows * sizeof(T), cols,
+                            hipMemcpyHostToDevice, h->stream));
   return kOk;
 }
 
-int down2d(eamd_handle* h, double* dst, int64_t ldd, const double* d, int64_t lds, int64_t rows,
+template <typename T>
+int down2d(eamd_handle* h, T* dst, int64_t ldd, const T* d, int64_t lds, int64_t rows,
            int64_t cols) {
-  EAMD_HIP(hipMemcpy2DAsync(dst, ldd * sizeof(double), d, lds * sizeof(double),
-                            rows * sizeof(double), cols, hipMemcpyDeviceToHost, h->stream));
+  EAMD_HIP(hipMemcpy2DAsync(dst, ldd * sizeof(T), d, lds * sizeof(T), rows * sizeof(T), cols,
+                            hipMemcpyDeviceToHost, h->stream));
   return kOk;
 }
 }  // namespace
@@ -905,6 +1017,87 @@ int eamd_host_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const 
   EAMD_CHECK(up2d(h, (double*)dB.p, n, B, ldb, n, nrhs));
   EAMD_CHECK(eamd_dpotrs(h, uplo, n, nrhs, (double*)dA.p, n, (double*)dB.p, n));
   EAMD_CHECK(down2d(h, B, ldb, (double*)dB.p, n, n, nrhs));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_host_sgetrf(eamd_handle_t h, int64_t m, int64_t n, float* A, int64_t lda,
+                     int32_t* transp, int32_t* nb_transp, int64_t* first_zero_pivot) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max<int64_t>(1, m)) return -4;
+  CHECK_HANDLE(h);
+  if (m == 0 || n == 0) return kOk;
+  DevBuf dA, dt;
+  EAMD_CHECK(dA.alloc(sizeof(float) * (size_t)m * n));
+  EAMD_CHECK(dt.alloc(sizeof(int32_t) * (size_t)std::min(m, n)));
+  EAMD_CHECK(up2d(h, (float*)dA.p, m, A, lda, m, n));
+  EAMD_CHECK(eamd_sgetrf(h, m, n, (float*)dA.p, m, (int32_t*)dt.p, nb_transp, first_zero_pivot));
+  EAMD_CHECK(down2d(h, A, lda, (float*)dA.p, m, m, n));
+  EAMD_HIP(hipMemcpyAsync(transp, dt.p, sizeof(int32_t) * (size_t)std::min(m, n),
+                          hipMemcpyDeviceToHost, h->stream));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_host_sgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const float* LU,
+                     int64_t lda, const int32_t* transp, float* B, int64_t ldb) {
+  if (!is_op(trans)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max<int64_t>(1, n)) return -5;
+  if (ldb < std::max<int64_t>(1, n)) return -8;
+  CHECK_HANDLE(h);
+  if (n == 0 || nrhs == 0) return kOk;
+  DevBuf dA, dB, dt;
+  EAMD_CHECK(dA.alloc(sizeof(float) * (size_t)n * n));
+  EAMD_CHECK(dB.alloc(sizeof(float) * (size_t)n * nrhs));
+  EAMD_CHECK(dt.alloc(sizeof(int32_t) * (size_t)n));
+  EAMD_CHECK(up2d(h, (float*)dA.p, n, LU, lda, n, n));
+  EAMD_CHECK(up2d(h, (float*)dB.p, n, B, ldb, n, nrhs));
+  EAMD_HIP(hipMemcpyAsync(dt.p, transp, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
+                          h->stream));
+  EAMD_CHECK(eamd_sgetrs(h, trans, n, nrhs, (float*)dA.p, n, (int32_t*)dt.p, (float*)dB.p, n));
+  EAMD_CHECK(down2d(h, B, ldb, (float*)dB.p, n, n, nrhs));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_host_spotrf(eamd_handle_t h, char uplo, int64_t n, float* A, int64_t lda,
+                     int64_t* fail_col) {
+  if (!is_uplo(uplo)) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max<int64_t>(1, n)) return -4;
+  CHECK_HANDLE(h);
+  if (n == 0) {
+    if (fail_col) *fail_col = -1;
+    return kOk;
+  }
+  DevBuf dA;
+  EAMD_CHECK(dA.alloc(sizeof(float) * (size_t)n * n));
+  EAMD_CHECK(up2d(h, (float*)dA.p, n, A, lda, n, n));
+  EAMD_CHECK(eamd_spotrf(h, uplo, n, (float*)dA.p, n, fail_col));
+  EAMD_CHECK(down2d(h, A, lda, (float*)dA.p, n, n, n));
+  EAMD_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int eamd_host_spotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const float* A,
+                     int64_t lda, float* B, int64_t ldb) {
+  if (!is_uplo(uplo)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max<int64_t>(1, n)) return -5;
+  if (ldb < std::max<int64_t>(1, n)) return -7;
+  CHECK_HANDLE(h);
+  if (n == 0 || nrhs == 0) return kOk;
+  DevBuf dA, dB;
+  EAMD_CHECK(dA.alloc(sizeof(float) * (size_t)n * n));
+  EAMD_CHECK(dB.alloc(sizeof(float) * (size_t)n * nrhs));
+  EAMD_CHECK(up2d(h, (float*)dA.p, n, A, lda, n, n));
+  EAMD_CHECK(up2d(h, (float*)dB.p, n, B, ldb, n, nrhs));
+  EAMD_CHECK(eamd_spotrs(h, uplo, n, nrhs, (float*)dA.p, n, (float*)dB.p, n));
+  EAMD_CHECK(down2d(h, B, ldb, (float*)dB.p, n, n, nrhs));
   EAMD_HIP(hipStreamSynchronize(h->stream));
   return kOk;
 }

@@ -136,6 +136,7 @@ enum WsSlot : int {
   kWsTsl,          // tournament-pivoting candidates and saved rows
   kWsPackL,        // packed panel columns (A21) of two consecutive panels
   kWsPackU,        // packed U rows (A12) of the next panel and of the rest
+  kWsConv,         // FP64 copies of the FP32 single-matrix entry points' operands
 };
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

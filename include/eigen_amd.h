@@ -138,6 +138,29 @@ int eamd_dpotrf(eamd_handle_t h, char uplo, int64_t n, double* A, int64_t lda,
 int eamd_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const double* A,
                 int64_t lda, double* B, int64_t ldb);
 
+/* ----------------------------------------------- FP32 single matrices -- */
+/* FP32 storage, FP64 arithmetic: the operands are widened on the device, the
+ * FP64 routine above runs, and the results are rounded back to FP32.
+ * Arguments, transpositions, first-zero-pivot and failing-column rules are
+ * those of the FP64 entry points.  The pivots are those of the FP64
+ * factorization of the FP32 values (Eigen's FP32 arithmetic can choose
+ * differently between candidates within FP32 rounding of each other), and
+ * the factors are that factorization rounded once.
+ * eamd_sgetrf/eamd_sgetrs replace partial_lu_impl<float,ColMajor,int>::
+ * blocked_lu as PartialPivLU_LAPACKE.h:75 binds it (sgetrf) and the FP32
+ * PartialPivLU::_solve_impl (PartialPivLU.h:226-271); eamd_spotrf/
+ * eamd_spotrs replace llt_inplace<float,Lower|Upper>::blocked as
+ * LLT_LAPACKE.h:91 binds it (spotrf) and the FP32 LLT::_solve_impl
+ * (LLT.h:475-505). */
+int eamd_sgetrf(eamd_handle_t h, int64_t m, int64_t n, float* A, int64_t lda, int32_t* transp,
+                int32_t* nb_transp, int64_t* first_zero_pivot);
+int eamd_sgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const float* LU,
+                int64_t lda, const int32_t* transp, float* B, int64_t ldb);
+int eamd_spotrf(eamd_handle_t h, char uplo, int64_t n, float* A, int64_t lda,
+                int64_t* fail_col);
+int eamd_spotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const float* A,
+                int64_t lda, float* B, int64_t ldb);
+
 /* --------------------------------------------------- mixed precision IR -- */
 /* No reference counterpart (Eigen has no iterative refinement): a new
  * capability behind PartialPivLU's solve() surface. */
@@ -303,6 +326,14 @@ int eamd_host_dpotrf(eamd_handle_t h, char uplo, int64_t n, double* A, int64_t l
                      int64_t* fail_col);
 int eamd_host_dpotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const double* A,
                      int64_t lda, double* B, int64_t ldb);
+int eamd_host_sgetrf(eamd_handle_t h, int64_t m, int64_t n, float* A, int64_t lda,
+                     int32_t* transp, int32_t* nb_transp, int64_t* first_zero_pivot);
+int eamd_host_sgetrs(eamd_handle_t h, char trans, int64_t n, int64_t nrhs, const float* LU,
+                     int64_t lda, const int32_t* transp, float* B, int64_t ldb);
+int eamd_host_spotrf(eamd_handle_t h, char uplo, int64_t n, float* A, int64_t lda,
+                     int64_t* fail_col);
+int eamd_host_spotrs(eamd_handle_t h, char uplo, int64_t n, int64_t nrhs, const float* A,
+                     int64_t lda, float* B, int64_t ldb);
 int eamd_host_dgemm(eamd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
                     double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                     double beta, double* C, int64_t ldc);

@@ -1,5 +1,6 @@
 /* eigen_plugin.h -- Eigen's own back-end plug points, specialised for
- * double to run on the MI355X through the eigen_amd C ABI.
+ * double (and, for the LU and Cholesky factorizations, float: FP32 storage,
+ * FP64 arithmetic) to run on the MI355X through the eigen_amd C ABI.
  *
  * Include it after <Eigen/Dense> (or after <Eigen/LU> and <Eigen/Cholesky>)
  * and before the first use of the decompositions, exactly where Eigen's
@@ -9,8 +10,12 @@
  *   PartialPivLU<MatrixXd>(A)  -> partial_lu_impl<double, ColMajor, int>::blocked_lu
  *                                 (PartialPivLU.h:426-496; same plug point and
  *                                 conventions as PartialPivLU_LAPACKE.h:42-77)
+ *   PartialPivLU<MatrixXf>(A)  -> partial_lu_impl<float, ColMajor, int>::blocked_lu
+ *                                 (as PartialPivLU_LAPACKE.h:75 binds sgetrf)
  *   LLT<MatrixXd, Lower|Upper>  -> llt_inplace<double, Lower|Upper>::blocked
  *                                 (LLT.h:330-360; as LLT_LAPACKE.h:42-93)
+ *   LLT<MatrixXf, Lower|Upper>  -> llt_inplace<float, Lower|Upper>::blocked
+ *                                 (as LLT_LAPACKE.h:91 binds spotrf)
  *   C.noalias() += A * B        -> general_matrix_matrix_product<..., double, ..., ColMajor>::run
  *                                 (GeneralMatrixMatrix.h:49-199; as
  *                                 GeneralMatrixMatrix_BLAS.h:49-116), when
@@ -93,16 +98,37 @@ inline void check(int status, const char* what) {
 
 /* A row-major block seen as the column-major transpose: copy it out as
  * column-major, run, copy back. */
-inline void to_colmajor(const double* src, Eigen::Index rows, Eigen::Index cols,
-                        Eigen::Index stride, std::vector<double>& dst) {
+template <typename T>
+inline void to_colmajor(const T* src, Eigen::Index rows, Eigen::Index cols,
+                        Eigen::Index stride, std::vector<T>& dst) {
   dst.resize((size_t)rows * (size_t)cols);
   for (Eigen::Index i = 0; i < rows; ++i)
     for (Eigen::Index j = 0; j < cols; ++j) dst[(size_t)j * rows + i] = src[i * stride + j];
 }
-inline void from_colmajor(const std::vector<double>& src, Eigen::Index rows, Eigen::Index cols,
-                          double* dst, Eigen::Index stride) {
+template <typename T>
+inline void from_colmajor(const std::vector<T>& src, Eigen::Index rows, Eigen::Index cols,
+                          T* dst, Eigen::Index stride) {
   for (Eigen::Index i = 0; i < rows; ++i)
     for (Eigen::Index j = 0; j < cols; ++j) dst[i * stride + j] = src[(size_t)j * rows + i];
+}
+
+/* The host-buffer LU and Cholesky entry points by scalar type (FP32 ones
+ * compute in FP64 and round their results once). */
+inline int host_getrf(eamd_handle_t h, int64_t m, int64_t n, double* A, int64_t lda,
+                      int32_t* t, int32_t* nbt, int64_t* fz) {
+  return eamd_host_dgetrf(h, m, n, A, lda, t, nbt, fz);
+}
+inline int host_getrf(eamd_handle_t h, int64_t m, int64_t n, float* A, int64_t lda,
+                      int32_t* t, int32_t* nbt, int64_t* fz) {
+  return eamd_host_sgetrf(h, m, n, A, lda, t, nbt, fz);
+}
+inline int host_potrf(eamd_handle_t h, char uplo, int64_t n, double* A, int64_t lda,
+                      int64_t* fail) {
+  return eamd_host_dpotrf(h, uplo, n, A, lda, fail);
+}
+inline int host_potrf(eamd_handle_t h, char uplo, int64_t n, float* A, int64_t lda,
+                      int64_t* fail) {
+  return eamd_host_spotrf(h, uplo, n, A, lda, fail);
 }
 
 }  // namespace eigen_amd_plugin
@@ -110,14 +136,14 @@ inline void from_colmajor(const std::vector<double>& src, Eigen::Index rows, Eig
 namespace Eigen {
 namespace internal {
 
-/* PartialPivLU<Matrix<double,...>>: returns the first exactly-zero pivot or
- * -1; transpositions are 0-based, nb_transpositions counts t[i] != i. */
-template <int StorageOrder>
-struct partial_lu_impl<double, StorageOrder, int> {
-  static Index blocked_lu(Index rows, Index cols, double* lu_data, Index luStride,
-                          int* row_transpositions, int& nb_transpositions,
-                          Index maxBlockSize = 256) {
-    EIGEN_UNUSED_VARIABLE(maxBlockSize);
+/* PartialPivLU<Matrix<double|float,...>>: returns the first exactly-zero
+ * pivot or -1; transpositions are 0-based, nb_transpositions counts
+ * t[i] != i.  PartialPivLU_LAPACKE.h:74-75 binds both scalar types at this
+ * seam. */
+template <typename Scalar, int StorageOrder>
+struct eigen_amd_lu {
+  static Index blocked_lu(Index rows, Index cols, Scalar* lu_data, Index luStride,
+                          int* row_transpositions, int& nb_transpositions) {
     int32_t nbt = 0;
     int64_t first_zero = -1;
     /* an empty matrix has no pivots; its stride may be 0, which the C ABI's
@@ -129,16 +155,16 @@ struct partial_lu_impl<double, StorageOrder, int> {
     ++eigen_amd_plugin::call_counts().getrf;
     if (StorageOrder == ColMajor) {
       eigen_amd_plugin::check(
-          eamd_host_dgetrf(eigen_amd_plugin::handle(), rows, cols, lu_data, luStride,
-                           row_transpositions, &nbt, &first_zero),
-          "eamd_host_dgetrf");
+          eigen_amd_plugin::host_getrf(eigen_amd_plugin::handle(), rows, cols, lu_data, luStride,
+                                       row_transpositions, &nbt, &first_zero),
+          "eamd_host_getrf");
     } else {
-      std::vector<double> tmp;
+      std::vector<Scalar> tmp;
       eigen_amd_plugin::to_colmajor(lu_data, rows, cols, luStride, tmp);
       eigen_amd_plugin::check(
-          eamd_host_dgetrf(eigen_amd_plugin::handle(), rows, cols, tmp.data(), rows,
-                           row_transpositions, &nbt, &first_zero),
-          "eamd_host_dgetrf");
+          eigen_amd_plugin::host_getrf(eigen_amd_plugin::handle(), rows, cols, tmp.data(), rows,
+                                       row_transpositions, &nbt, &first_zero),
+          "eamd_host_getrf");
       eigen_amd_plugin::from_colmajor(tmp, rows, cols, lu_data, luStride);
     }
     nb_transpositions = nbt;
@@ -146,7 +172,30 @@ struct partial_lu_impl<double, StorageOrder, int> {
   }
 };
 
-/* LLT<Matrix<double,...>, Lower|Upper>: the failing column or -1. */
+template <int StorageOrder>
+struct partial_lu_impl<double, StorageOrder, int> {
+  static Index blocked_lu(Index rows, Index cols, double* lu_data, Index luStride,
+                          int* row_transpositions, int& nb_transpositions,
+                          Index maxBlockSize = 256) {
+    EIGEN_UNUSED_VARIABLE(maxBlockSize);
+    return eigen_amd_lu<double, StorageOrder>::blocked_lu(rows, cols, lu_data, luStride,
+                                                          row_transpositions, nb_transpositions);
+  }
+};
+
+template <int StorageOrder>
+struct partial_lu_impl<float, StorageOrder, int> {
+  static Index blocked_lu(Index rows, Index cols, float* lu_data, Index luStride,
+                          int* row_transpositions, int& nb_transpositions,
+                          Index maxBlockSize = 256) {
+    EIGEN_UNUSED_VARIABLE(maxBlockSize);
+    return eigen_amd_lu<float, StorageOrder>::blocked_lu(rows, cols, lu_data, luStride,
+                                                         row_transpositions, nb_transpositions);
+  }
+};
+
+/* LLT<Matrix<double|float,...>, Lower|Upper>: the failing column or -1
+ * (LLT_LAPACKE.h:90-91 binds both scalar types at this seam). */
 template <int UpLo>
 struct eigen_amd_llt {
   template <typename MatrixType>
@@ -158,16 +207,18 @@ struct eigen_amd_llt {
     const char uplo = UpLo == Lower ? 'L' : 'U';
     ++eigen_amd_plugin::call_counts().potrf;
     if (!(MatrixType::Flags & RowMajorBit)) {
-      eigen_amd_plugin::check(eamd_host_dpotrf(eigen_amd_plugin::handle(), uplo, n,
-                                               &m.coeffRef(0, 0), m.outerStride(), &fail),
-                              "eamd_host_dpotrf");
+      eigen_amd_plugin::check(eigen_amd_plugin::host_potrf(eigen_amd_plugin::handle(), uplo, n,
+                                                           &m.coeffRef(0, 0), m.outerStride(),
+                                                           &fail),
+                              "eamd_host_potrf");
     } else {
       /* row-major: the stored array is the column-major transpose, whose
        * opposite triangle holds the same factor */
-      eigen_amd_plugin::check(eamd_host_dpotrf(eigen_amd_plugin::handle(),
-                                               UpLo == Lower ? 'U' : 'L', n, &m.coeffRef(0, 0),
-                                               m.outerStride(), &fail),
-                              "eamd_host_dpotrf");
+      eigen_amd_plugin::check(eigen_amd_plugin::host_potrf(eigen_amd_plugin::handle(),
+                                                           UpLo == Lower ? 'U' : 'L', n,
+                                                           &m.coeffRef(0, 0), m.outerStride(),
+                                                           &fail),
+                              "eamd_host_potrf");
     }
     return (Index)fail;
   }
@@ -197,6 +248,33 @@ struct llt_inplace<double, Upper> {
                           const typename MatrixType::RealScalar& sigma) {
     Transpose<MatrixType> matt(mat);
     return llt_inplace<double, Lower>::rankUpdate(matt, vec.conjugate(), sigma);
+  }
+};
+
+template <>
+struct llt_inplace<float, Lower> {
+  template <typename MatrixType>
+  static Index blocked(MatrixType& m) {
+    return eigen_amd_llt<Lower>::potrf(m);
+  }
+  template <typename MatrixType, typename VectorType>
+  static Index rankUpdate(MatrixType& mat, const VectorType& vec,
+                          const typename MatrixType::RealScalar& sigma) {
+    return Eigen::internal::llt_rank_update_lower(mat, vec, sigma);
+  }
+};
+
+template <>
+struct llt_inplace<float, Upper> {
+  template <typename MatrixType>
+  static Index blocked(MatrixType& m) {
+    return eigen_amd_llt<Upper>::potrf(m);
+  }
+  template <typename MatrixType, typename VectorType>
+  static Index rankUpdate(MatrixType& mat, const VectorType& vec,
+                          const typename MatrixType::RealScalar& sigma) {
+    Transpose<MatrixType> matt(mat);
+    return llt_inplace<float, Lower>::rankUpdate(matt, vec.conjugate(), sigma);
   }
 };
 

@@ -59,6 +59,7 @@ void dump(const char* name, const T* p, size_t count) {
   std::fclose(f);
 }
 void dump(const char* name, const MatrixXd& m) { dump(name, m.data(), (size_t)m.size()); }
+void dump(const char* name, const Eigen::MatrixXf& m) { dump(name, m.data(), (size_t)m.size()); }
 
 // the plug points' call counts: getrf, potrf, gemm, syrk, trsm, trsv
 void dump_counts(const char* name) {
@@ -168,6 +169,42 @@ int main(int argc, char** argv) {
     dump("syrk_L.f64", Sl);
     dump("syrk_U.f64", Su);
     dump_counts("cnt_syrk.i64");
+
+    // FP32: PartialPivLU<MatrixXf> and LLT<MatrixXf> (lower, upper) through
+    // the float plug points; their solves stay in Eigen's FP32 substitution
+    {
+      const Index nf = 257;
+      Eigen::MatrixXf Af = normal(nf, nf, 31).cast<float>(), bf = normal(nf, 2, 32).cast<float>();
+      Eigen::PartialPivLU<Eigen::MatrixXf> luf(Af);
+      Eigen::MatrixXf xf = luf.solve(bf);
+      Eigen::VectorXi permf = luf.permutationP().indices();
+      dump("luf_A.f32", Af);
+      dump("luf_b.f32", bf);
+      dump("luf_LU.f32", luf.matrixLU());
+      dump("luf_perm.i32", permf.data(), (size_t)nf);
+      dump("luf_x.f32", xf);
+      Eigen::MatrixXf Gf = normal(nf, nf, 33).cast<float>();
+      Eigen::MatrixXf Sf = Gf * Gf.transpose() / float(nf) + Eigen::MatrixXf::Identity(nf, nf);
+      Eigen::MatrixXf bsf = normal(nf, 3, 34).cast<float>();
+      Eigen::LLT<Eigen::MatrixXf> lltf(Sf);
+      Eigen::MatrixXf xsf = lltf.solve(bsf);
+      Eigen::MatrixXf Lf = lltf.matrixL();
+      Eigen::LLT<Eigen::MatrixXf, Eigen::Upper> lltfu(Sf);
+      Eigen::MatrixXf Uf = lltfu.matrixU();
+      Eigen::MatrixXf Sfbad = Sf;
+      Sfbad(20, 20) = -1.0f;
+      Eigen::LLT<Eigen::MatrixXf> badf(Sfbad);
+      int infof[3] = {lltf.info() == Eigen::Success ? 0 : 1,
+                      lltfu.info() == Eigen::Success ? 0 : 1,
+                      badf.info() == Eigen::NumericalIssue ? 1 : 0};
+      dump("lltf_S.f32", Sf);
+      dump("lltf_b.f32", bsf);
+      dump("lltf_L.f32", Lf);
+      dump("lltf_U.f32", Uf);
+      dump("lltf_x.f32", xsf);
+      dump("lltf_info.i32", infof, 3);
+      dump_counts("cnt_f32.i64");
+    }
 
     // empty matrices: Success, no GPU call needed
     MatrixXd E(0, 0);

@@ -417,3 +417,58 @@ def test_llt_rank_update_batched(gpu):
         assert info[i] == rinfo == -1
         got = out[offL[i]:offL[i] + n * n].reshape(n, n, order="F")
         np.testing.assert_allclose(np.tril(got), np.tril(ref), rtol=1e-12, atol=1e-12)
+
+
+def test_fp32_single_matrix_entry_points(gpu):
+    """eamd_sgetrf/sgetrs and eamd_spotrf/spotrs (FP32 storage, FP64
+    arithmetic) through the C ABI on device buffers: the pivots are the
+    oracle's FP64 pivots on the FP32 values, each factor entry is the FP64
+    factor rounded once, both transposed and plain LU solves and the
+    Cholesky solve reach FP32 backward error, and a non-SPD matrix reports
+    its failing column."""
+    import ctypes as C
+    from eigen_amd import dense
+    lib, h, p = dense.lib(), dense.handle(), dense._p
+    n, nrhs = 300, 5
+    A32 = inputs.normal(7100, n, n).astype(np.float32)
+    A = A32.astype(np.float64)
+    dA = ea.colmajor(torch.from_numpy(np.asfortranarray(A32)))
+    t = torch.empty(n, dtype=torch.int32, device="cuda")
+    nbt, fz = C.c_int32(0), C.c_int64(0)
+    assert lib.eamd_sgetrf(h, n, n, p(dA), n, p(t), C.byref(nbt), C.byref(fz)) == 0
+    lu_r, t_r, nbt_r, fz_r = po.lu(A)
+    np.testing.assert_array_equal(host(t), t_r)
+    assert (nbt.value, fz.value) == (nbt_r, fz_r)
+    np.testing.assert_allclose(host(dA).astype(np.float64), lu_r, rtol=2.0 ** -24,
+                               atol=1e-11 * np.abs(lu_r).max())
+    b32 = inputs.normal(7101, n, nrhs).astype(np.float32)
+    for trans in (b"N", b"T"):
+        dB = ea.colmajor(torch.from_numpy(np.asfortranarray(b32)))
+        assert lib.eamd_sgetrs(h, trans, n, nrhs, p(dA), n, p(t), p(dB), n) == 0
+        op = A if trans == b"N" else A.T
+        x = host(dB).astype(np.float64)
+        assert po.backward_error(op, x, b32.astype(np.float64)) <= 1e-5
+    S32 = inputs.spd(7102, n).astype(np.float32)
+    S = S32.astype(np.float64)
+    l_r, fail_r = po.llt(S)
+    assert fail_r == -1
+    for uplo in (b"L", b"U"):
+        dS = ea.colmajor(torch.from_numpy(np.asfortranarray(S32)))
+        fail = C.c_int64(0)
+        assert lib.eamd_spotrf(h, uplo, n, p(dS), n, C.byref(fail)) == 0
+        assert fail.value == -1
+        F = host(dS).astype(np.float64)
+        got, ref = (np.tril(F), np.tril(l_r)) if uplo == b"L" else (np.triu(F), np.tril(l_r).T)
+        np.testing.assert_allclose(got, ref, rtol=2.0 ** -24, atol=1e-12 * np.abs(ref).max())
+        # the other triangle keeps its input values exactly
+        other = np.triu(F, 1) if uplo == b"L" else np.tril(F, -1)
+        np.testing.assert_array_equal(other, np.triu(S, 1) if uplo == b"L" else np.tril(S, -1))
+        dB = ea.colmajor(torch.from_numpy(np.asfortranarray(b32)))
+        assert lib.eamd_spotrs(h, uplo, n, nrhs, p(dS), n, p(dB), n) == 0
+        assert po.backward_error(S, host(dB).astype(np.float64), b32.astype(np.float64)) <= 1e-5
+    bad = S32.copy()
+    bad[40, 40] = -1.0
+    dS = ea.colmajor(torch.from_numpy(np.asfortranarray(bad)))
+    fail = C.c_int64(0)
+    assert lib.eamd_spotrf(h, b"L", n, p(dS), n, C.byref(fail)) == 0
+    assert fail.value == po.llt(bad.astype(np.float64))[1]

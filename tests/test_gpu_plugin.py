@@ -1,8 +1,8 @@
 """GPU test of the Eigen plug-point specialisations
 (include/eigen_amd/eigen_plugin.h): unmodified Eigen user code
 (tests/adapter/eigen_plugin_check.cpp, compiled with g++ against the Eigen
-headers into oracle/_ref/) runs PartialPivLU<MatrixXd>, LLT<MatrixXd>,
-GEMM and triangular solves through the C ABI; its outputs are checked
+headers into oracle/_ref/) runs PartialPivLU<MatrixXd|MatrixXf>,
+LLT<MatrixXd|MatrixXf>, GEMM and triangular solves through the C ABI; its outputs are checked
 against the oracle and closed forms here."""
 import os
 import subprocess
@@ -137,6 +137,51 @@ def test_plugin_syrk(gpu, out):
     np.testing.assert_array_equal(np.tril(Su, -1), np.tril(C0, -1))
     before, after = _counts(out, "cnt_trsm.i64"), _counts(out, "cnt_syrk.i64")
     assert after["syrk"] - before["syrk"] == 2
+
+
+def test_plugin_fp32_factorizations(gpu, out):
+    """PartialPivLU<MatrixXf> and LLT<MatrixXf> (lower, upper) reach the GPU
+    through the float plug points, which compute in FP64 on the FP32 values:
+    the pivots equal the oracle's FP64 pivots on those values, and every
+    factor entry is the oracle's FP64 factor rounded once (a factorization
+    carried out in FP32 would be off by many FP32 ulps).  Eigen's FP32
+    substitution then solves with them to FP32 backward error, and a
+    non-SPD matrix reports NumericalIssue."""
+    n = 257
+    f32 = np.float32
+    A = _load(out, "luf_A.f32", (n, n), f32).astype(np.float64)
+    b = _load(out, "luf_b.f32", (n, 2), f32).astype(np.float64)
+    LU = _load(out, "luf_LU.f32", (n, n), f32).astype(np.float64)
+    perm = _load(out, "luf_perm.i32", (n,), np.int32)
+    x = _load(out, "luf_x.f32", (n, 2), f32).astype(np.float64)
+    lu_r, t_r, _, _ = po.lu(A)
+    np.testing.assert_array_equal(perm, po.perm_from_transpositions(t_r))
+    # one FP32 rounding (half an ulp, 2^-24 relative) of an FP64 factor that
+    # is itself within 1e-11 of the oracle's
+    np.testing.assert_allclose(LU, lu_r, rtol=2.0 ** -24, atol=1e-11 * np.abs(lu_r).max())
+    assert berr(A, x, b) <= 1e-5
+    S = _load(out, "lltf_S.f32", (n, n), f32).astype(np.float64)
+    bs = _load(out, "lltf_b.f32", (n, 3), f32).astype(np.float64)
+    L = _load(out, "lltf_L.f32", (n, n), f32).astype(np.float64)
+    U = _load(out, "lltf_U.f32", (n, n), f32).astype(np.float64)
+    xs = _load(out, "lltf_x.f32", (n, 3), f32).astype(np.float64)
+    info = _load(out, "lltf_info.i32", (3,), np.int32)
+    np.testing.assert_array_equal(info, [0, 0, 1])
+    l_r, fail = po.llt(S)
+    assert fail == -1
+    l_r = np.tril(l_r)
+    tol = 1e-12 * np.abs(l_r).max()
+    np.testing.assert_allclose(L, l_r, rtol=2.0 ** -24, atol=tol)
+    # Eigen's FP32 G G^T is not exactly symmetric: the Upper factorization
+    # reads the upper triangle, so its reference is the one of that triangle
+    Su = np.triu(S) + np.triu(S, 1).T
+    u_r, fail = po.llt(Su)
+    assert fail == -1
+    u_r = np.tril(u_r).T
+    np.testing.assert_allclose(U, u_r, rtol=2.0 ** -24, atol=tol)
+    assert berr(S, xs, bs) <= 1e-5
+    before, after = _counts(out, "cnt_syrk.i64"), _counts(out, "cnt_f32.i64")
+    assert after["getrf"] - before["getrf"] == 1 and after["potrf"] - before["potrf"] == 3
 
 
 def test_plugin_empty_matrices(gpu, out):
