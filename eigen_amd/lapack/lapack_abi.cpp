@@ -1,5 +1,5 @@
-// lapack_abi.cpp -- dgetrf_/dgetrs_/dpotrf_/dpotrs_/dgemm_/dtrsm_/dsyrk_ over
-// the eigen_amd host-buffer entry points.  Argument checks and info codes
+// lapack_abi.cpp -- dgetrf_/dgetrs_/dpotrf_/dpotrs_ (and their FP32 s*
+// forms), dgemm_/dtrsm_/dsyrk_ over the eigen_amd host-buffer entry points.  Argument checks and info codes
 // follow lapack/lu.cpp:14-89, lapack/cholesky.cpp:14-72 and
 // blas/level3_impl.h:12-179, 357-391;
 // each host thread gets its own handle (handles are thread-confined).
@@ -52,74 +52,142 @@ bool is_uplo(char c) {
 }
 char up(char c) { return (char)std::toupper((unsigned char)c); }
 
-}  // namespace
+// The LU and Cholesky stubs for both real types (the FP32 entry points keep
+// FP32 storage and compute in FP64).
+inline int host_getrf(eamd_handle_t h, int m, int n, double* a, int lda, int* ipiv, int32_t* nbt,
+                      int64_t* fz) {
+  return eamd_host_dgetrf(h, m, n, a, lda, ipiv, nbt, fz);
+}
+inline int host_getrf(eamd_handle_t h, int m, int n, float* a, int lda, int* ipiv, int32_t* nbt,
+                      int64_t* fz) {
+  return eamd_host_sgetrf(h, m, n, a, lda, ipiv, nbt, fz);
+}
+inline int host_getrs(eamd_handle_t h, char tr, int n, int nrhs, const double* a, int lda,
+                      const int32_t* t, double* b, int ldb) {
+  return eamd_host_dgetrs(h, tr, n, nrhs, a, lda, t, b, ldb);
+}
+inline int host_getrs(eamd_handle_t h, char tr, int n, int nrhs, const float* a, int lda,
+                      const int32_t* t, float* b, int ldb) {
+  return eamd_host_sgetrs(h, tr, n, nrhs, a, lda, t, b, ldb);
+}
+inline int host_potrf(eamd_handle_t h, char uplo, int n, double* a, int lda, int64_t* fail) {
+  return eamd_host_dpotrf(h, uplo, n, a, lda, fail);
+}
+inline int host_potrf(eamd_handle_t h, char uplo, int n, float* a, int lda, int64_t* fail) {
+  return eamd_host_spotrf(h, uplo, n, a, lda, fail);
+}
+inline int host_potrs(eamd_handle_t h, char uplo, int n, int nrhs, const double* a, int lda,
+                      double* b, int ldb) {
+  return eamd_host_dpotrs(h, uplo, n, nrhs, a, lda, b, ldb);
+}
+inline int host_potrs(eamd_handle_t h, char uplo, int n, int nrhs, const float* a, int lda,
+                      float* b, int ldb) {
+  return eamd_host_spotrs(h, uplo, n, nrhs, a, lda, b, ldb);
+}
 
-extern "C" {
-
-int dgetrf_(int* m, int* n, double* a, int* lda, int* ipiv, int* info) {
+template <typename T>
+int getrf_t(const char* nm, int* m, int* n, T* a, int* lda, int* ipiv, int* info) {
   *info = 0;
-  if (*m < 0) return illegal("DGETRF", 1, info);
-  if (*n < 0) return illegal("DGETRF", 2, info);
-  if (*lda < std::max(1, *m)) return illegal("DGETRF", 4, info);
+  if (*m < 0) return illegal(nm, 1, info);
+  if (*n < 0) return illegal(nm, 2, info);
+  if (*lda < std::max(1, *m)) return illegal(nm, 4, info);
   if (*m == 0 || *n == 0) return 0;
   eamd_handle_t h = handle();
-  if (!h) return device_failure("DGETRF", -1000, info);
+  if (!h) return device_failure(nm, -1000, info);
   int32_t nbt = 0;
   int64_t fz = -1;
-  const int st = eamd_host_dgetrf(h, *m, *n, a, *lda, ipiv, &nbt, &fz);
-  if (st != 0) return device_failure("DGETRF", st, info);
+  const int st = host_getrf(h, *m, *n, a, *lda, ipiv, &nbt, &fz);
+  if (st != 0) return device_failure(nm, st, info);
   for (int i = 0; i < std::min(*m, *n); ++i) ipiv[i] += 1;
   if (fz >= 0) *info = (int)fz + 1;
   return 0;
 }
 
-int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb,
-            int* info) {
+template <typename T>
+int getrs_t(const char* nm, char* trans, int* n, int* nrhs, T* a, int* lda, int* ipiv, T* b,
+            int* ldb, int* info) {
   *info = 0;
-  if (!is_trans(*trans)) return illegal("DGETRS", 1, info);
-  if (*n < 0) return illegal("DGETRS", 2, info);
-  if (*nrhs < 0) return illegal("DGETRS", 3, info);
-  if (*lda < std::max(1, *n)) return illegal("DGETRS", 5, info);
-  if (*ldb < std::max(1, *n)) return illegal("DGETRS", 8, info);
+  if (!is_trans(*trans)) return illegal(nm, 1, info);
+  if (*n < 0) return illegal(nm, 2, info);
+  if (*nrhs < 0) return illegal(nm, 3, info);
+  if (*lda < std::max(1, *n)) return illegal(nm, 5, info);
+  if (*ldb < std::max(1, *n)) return illegal(nm, 8, info);
   if (*n == 0 || *nrhs == 0) return 0;
   eamd_handle_t h = handle();
-  if (!h) return device_failure("DGETRS", -1000, info);
+  if (!h) return device_failure(nm, -1000, info);
   std::vector<int32_t> t(ipiv, ipiv + *n);
   for (auto& v : t) v -= 1;
   const char tr = up(*trans) == 'N' ? 'N' : 'T';  // real: adjoint = transpose
-  const int st = eamd_host_dgetrs(h, tr, *n, *nrhs, a, *lda, t.data(), b, *ldb);
-  if (st != 0) return device_failure("DGETRS", st, info);
+  const int st = host_getrs(h, tr, *n, *nrhs, a, *lda, t.data(), b, *ldb);
+  if (st != 0) return device_failure(nm, st, info);
   return 0;
 }
 
-int dpotrf_(char* uplo, int* n, double* a, int* lda, int* info) {
+template <typename T>
+int potrf_t(const char* nm, char* uplo, int* n, T* a, int* lda, int* info) {
   *info = 0;
-  if (!is_uplo(*uplo)) return illegal("DPOTRF", 1, info);
-  if (*n < 0) return illegal("DPOTRF", 2, info);
-  if (*lda < std::max(1, *n)) return illegal("DPOTRF", 4, info);
+  if (!is_uplo(*uplo)) return illegal(nm, 1, info);
+  if (*n < 0) return illegal(nm, 2, info);
+  if (*lda < std::max(1, *n)) return illegal(nm, 4, info);
   if (*n == 0) return 0;
   eamd_handle_t h = handle();
-  if (!h) return device_failure("DPOTRF", -1000, info);
+  if (!h) return device_failure(nm, -1000, info);
   int64_t fail = -1;
-  const int st = eamd_host_dpotrf(h, up(*uplo), *n, a, *lda, &fail);
-  if (st != 0) return device_failure("DPOTRF", st, info);
+  const int st = host_potrf(h, up(*uplo), *n, a, *lda, &fail);
+  if (st != 0) return device_failure(nm, st, info);
   if (fail >= 0) *info = (int)fail + 1;
   return 0;
 }
 
-int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, int* info) {
+template <typename T>
+int potrs_t(const char* nm, char* uplo, int* n, int* nrhs, T* a, int* lda, T* b, int* ldb,
+            int* info) {
   *info = 0;
-  if (!is_uplo(*uplo)) return illegal("DPOTRS", 1, info);
-  if (*n < 0) return illegal("DPOTRS", 2, info);
-  if (*nrhs < 0) return illegal("DPOTRS", 3, info);
-  if (*lda < std::max(1, *n)) return illegal("DPOTRS", 5, info);
-  if (*ldb < std::max(1, *n)) return illegal("DPOTRS", 7, info);
+  if (!is_uplo(*uplo)) return illegal(nm, 1, info);
+  if (*n < 0) return illegal(nm, 2, info);
+  if (*nrhs < 0) return illegal(nm, 3, info);
+  if (*lda < std::max(1, *n)) return illegal(nm, 5, info);
+  if (*ldb < std::max(1, *n)) return illegal(nm, 7, info);
   if (*n == 0 || *nrhs == 0) return 0;
   eamd_handle_t h = handle();
-  if (!h) return device_failure("DPOTRS", -1000, info);
-  const int st = eamd_host_dpotrs(h, up(*uplo), *n, *nrhs, a, *lda, b, *ldb);
-  if (st != 0) return device_failure("DPOTRS", st, info);
+  if (!h) return device_failure(nm, -1000, info);
+  const int st = host_potrs(h, up(*uplo), *n, *nrhs, a, *lda, b, *ldb);
+  if (st != 0) return device_failure(nm, st, info);
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgetrf_(int* m, int* n, double* a, int* lda, int* ipiv, int* info) {
+  return getrf_t("DGETRF", m, n, a, lda, ipiv, info);
+}
+int sgetrf_(int* m, int* n, float* a, int* lda, int* ipiv, int* info) {
+  return getrf_t("SGETRF", m, n, a, lda, ipiv, info);
+}
+
+int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, double* b, int* ldb,
+            int* info) {
+  return getrs_t("DGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info);
+}
+int sgetrs_(char* trans, int* n, int* nrhs, float* a, int* lda, int* ipiv, float* b, int* ldb,
+            int* info) {
+  return getrs_t("SGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info);
+}
+
+int dpotrf_(char* uplo, int* n, double* a, int* lda, int* info) {
+  return potrf_t("DPOTRF", uplo, n, a, lda, info);
+}
+int spotrf_(char* uplo, int* n, float* a, int* lda, int* info) {
+  return potrf_t("SPOTRF", uplo, n, a, lda, info);
+}
+
+int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, int* info) {
+  return potrs_t("DPOTRS", uplo, n, nrhs, a, lda, b, ldb, info);
+}
+int spotrs_(char* uplo, int* n, int* nrhs, float* a, int* lda, float* b, int* ldb, int* info) {
+  return potrs_t("SPOTRS", uplo, n, nrhs, a, lda, b, ldb, info);
 }
 
 int dgemm_(const char* opa, const char* opb, const int* m, const int* n, const int* k,

@@ -22,6 +22,13 @@ int dgetrs_(char* trans, int* n, int* nrhs, double* a, int* lda, int* ipiv, doub
 int dpotrf_(char* uplo, int* n, double* a, int* lda, int* info);
 /* solve with the factor of dpotrf_ (lapack/cholesky.cpp:41-72) */
 int dpotrs_(char* uplo, int* n, int* nrhs, double* a, int* lda, double* b, int* ldb, int* info);
+/* The same four for FP32 arrays (lapack/lu.cpp and lapack/cholesky.cpp are
+ * compiled for float too); FP32 storage, FP64 arithmetic inside */
+int sgetrf_(int* m, int* n, float* a, int* lda, int* ipiv, int* info);
+int sgetrs_(char* trans, int* n, int* nrhs, float* a, int* lda, int* ipiv, float* b, int* ldb,
+            int* info);
+int spotrf_(char* uplo, int* n, float* a, int* lda, int* info);
+int spotrs_(char* uplo, int* n, int* nrhs, float* a, int* lda, float* b, int* ldb, int* info);
 /* C = alpha op(A) op(B) + beta C (blas/level3_impl.h:12-76) */
 int dgemm_(const char* opa, const char* opb, const int* m, const int* n, const int* k,
            const double* alpha, const double* a, const int* lda, const double* b, const int* ldb,

@@ -1,5 +1,6 @@
 """GPU tests of the Fortran-ABI layer (libeigen_amd_lapack.so): dgetrf_,
-dgetrs_, dpotrf_, dpotrs_, dgemm_, dtrsm_, dsyrk_ on host column-major arrays,
+dgetrs_, dpotrf_, dpotrs_ (and their FP32 s* forms), dgemm_, dtrsm_, dsyrk_
+on host column-major arrays,
 against the oracle (1-based ipiv equal to the oracle's transpositions + 1,
 info rules of lapack/lu.cpp:14-41 and lapack/cholesky.cpp:14-36)."""
 import ctypes as C
@@ -83,6 +84,57 @@ def test_dpotrf_dpotrs(gpu, L):
     bad[10, 10] = -1.0
     L.dpotrf_(C.c_char_p(b"L"), _i(n), _d(bad), _i(n), C.byref(info))
     assert info.value > 0
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def test_sgetrf_sgetrs_spotrf_spotrs(gpu, L):
+    """The FP32 stubs (FP32 arrays, FP64 arithmetic inside): ipiv equal to
+    the oracle's FP64 transpositions on the FP32 values + 1, each factor
+    entry that factor rounded once, FP32-level backward errors, and the
+    info rules of the FP64 stubs."""
+    n = 150
+    a0 = inputs.normal(141, n, n).astype(np.float32)
+    a = np.asfortranarray(a0.copy())
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = C.c_int(-7)
+    L.sgetrf_(_i(n), _i(n), _f(a), _i(n), ipiv.ctypes.data_as(C.POINTER(C.c_int)), C.byref(info))
+    assert info.value == 0
+    A = a0.astype(np.float64)
+    lu_r, t_r, _, _ = po.lu(A)
+    np.testing.assert_array_equal(ipiv, t_r + 1)
+    np.testing.assert_allclose(a.astype(np.float64), lu_r, rtol=2.0 ** -24,
+                               atol=1e-11 * np.abs(lu_r).max())
+    b0 = np.asfortranarray(inputs.normal(142, n, 3).astype(np.float32))
+    for trans, op in ((b"N", A), (b"T", A.T)):
+        b = np.asfortranarray(b0.copy())
+        L.sgetrs_(C.c_char_p(trans), _i(n), _i(3), _f(a), _i(n),
+                  ipiv.ctypes.data_as(C.POINTER(C.c_int)), _f(b), _i(n), C.byref(info))
+        assert info.value == 0
+        assert berr(op, b.astype(np.float64), b0.astype(np.float64)) <= 1e-5
+    z = np.asfortranarray(a0.copy())
+    z[:, 6] = 0.0
+    L.sgetrf_(_i(n), _i(n), _f(z), _i(n), ipiv.ctypes.data_as(C.POINTER(C.c_int)), C.byref(info))
+    assert info.value == 7
+    s0 = inputs.spd(143, n).astype(np.float32)
+    for uplo in (b"L", b"U"):
+        s = np.asfortranarray(s0.copy())
+        L.spotrf_(C.c_char_p(uplo), _i(n), _f(s), _i(n), C.byref(info))
+        assert info.value == 0
+        f = np.tril(s) if uplo == b"L" else np.triu(s).T
+        f = f.astype(np.float64)
+        S = s0.astype(np.float64)
+        np.testing.assert_allclose(f @ f.T, S, rtol=0, atol=1e-6 * np.abs(S).max())
+        b = np.asfortranarray(b0.copy())
+        L.spotrs_(C.c_char_p(uplo), _i(n), _i(3), _f(s), _i(n), _f(b), _i(n), C.byref(info))
+        assert info.value == 0
+        assert berr(S, b.astype(np.float64), b0.astype(np.float64)) <= 1e-5
+    bad = np.asfortranarray(s0.copy())
+    bad[10, 10] = -1.0
+    L.spotrf_(C.c_char_p(b"L"), _i(n), _f(bad), _i(n), C.byref(info))
+    assert info.value == 11
 
 
 def test_dgemm_dtrsm(gpu, L):
