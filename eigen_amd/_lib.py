@@ -69,6 +69,7 @@ _SIGS = {
     "eamd_dbackward_error": ([vp, c_int64, c_int64, vp, c_int64, vp, c_int64, vp, c_int64,
                               C.POINTER(c_dbl)], c_int),
     "eamd_spotrf_solve_batched": ([vp, c_int64, vp, vp, vp, vp, vp, vp], c_int),
+    "eamd_dldlt_rank_update_batched": ([vp, c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp], c_int),
     "eamd_dllt_rank_update_batched": ([vp, c_int64, vp, vp, vp, vp, vp, vp, C.c_int32, vp], c_int),
     "eamd_dldlt_batched": ([vp, c_int64, vp, vp, C.c_int32, vp, vp, vp, vp, vp], c_int),
     "eamd_dldlt_solve_batched": ([vp, c_int64, vp, vp, C.c_int32, vp, vp, vp, vp, vp], c_int),

@@ -135,6 +135,22 @@ def ldlt_solve_batched(offA, n, A, offT, T, offb, b, max_n=None):
     return b
 
 
+def ldlt_rank_update_batched(offA, n, A, offT, T, offw, w, sigma=None):
+    """LDLT::rankUpdate (LDLT.h:403-449, 529-555) for ldlt_batched's factors:
+    factor i (A at offA[i], transpositions at offT[i]) updated in place to
+    the factor of A_i + sigma_i w_i w_i^T, w_i (FP64, n[i] values at offw[i])
+    read only; sigma (FP64 per factor) defaults to 1.  The transpositions
+    and sign are unchanged, as in the reference.  Returns info (0, or -2 for
+    n outside 0..256)."""
+    info = torch.empty(n.numel(), dtype=torch.int32, device=n.device)
+    check("eamd_dldlt_rank_update_batched",
+          lib().eamd_dldlt_rank_update_batched(handle(), n.numel(), _p(offA), _p(n), _p(A),
+                                               _p(offT), _p(T), _p(offw), _p(w),
+                                               _p(sigma) if sigma is not None else None,
+                                               _p(info)))
+    return info
+
+
 def dgesv_ir_batched(offA, n, A, offB, nrhs, B):
     """Batched FP32-LU + FP64-refinement solve; returns (X, info, iters)."""
     X = torch.empty_like(B)

@@ -772,6 +772,25 @@ int eamd_dldlt_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA
   return ldlt_solve_batched(h, count, offA, n, max_n, A, offT, transpositions, offb, b);
 }
 
+int eamd_dldlt_rank_update_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
+                                   const int32_t* n, double* A, const int64_t* offT,
+                                   const int32_t* transpositions, const int64_t* offw,
+                                   const double* w, const double* sigma, int32_t* info) {
+  if (count < 0) return -2;
+  CHECK_HANDLE(h);
+  if (count == 0) return kOk;
+  if (!offA) return -3;
+  if (!n) return -4;
+  if (!A) return -5;
+  if (!offT) return -6;
+  if (!transpositions) return -7;
+  if (!offw) return -8;
+  if (!w) return -9;
+  if (!info) return -11;
+  return ldlt_rank_update_batched(h, count, offA, n, A, offT, transpositions, offw, w, sigma,
+                                  info);
+}
+
 int eamd_spotrf_solve_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
                               const int32_t* n, const float* A, const int64_t* offb, float* b,
                               int32_t* info) {

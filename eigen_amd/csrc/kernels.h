@@ -38,6 +38,10 @@ int ldlt_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32
 int ldlt_solve_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n,
                        int max_n, const double* A, const int64_t* offT, const int32_t* T,
                        const int64_t* offb, double* b);
+int ldlt_rank_update_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n,
+                             double* A, const int64_t* offT, const int32_t* T,
+                             const int64_t* offw, const double* w, const double* sigma,
+                             int32_t* info);
 
 // ---- level 3 (gemm_f64.hip) ----
 // C = alpha*op(A)*op(B) + beta*C, column-major.  tri: 0 full C, 1 only

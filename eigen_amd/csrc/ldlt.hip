@@ -314,7 +314,104 @@ __global__ __launch_bounds__(256) void ldlt_solve_batch_kernel(
 
 int group_width(int max_n) { return max_n <= 16 ? 16 : (max_n <= 32 ? 32 : 64); }
 
+// LDLT rank-one update in place: one wave per factor, row i on lane i % 64
+// (up to four rows per lane, n <= 256).  Restates
+// internal::ldlt_inplace<Lower>::update (LDLT.h:440-449): w is permuted by
+// the factor's transpositions in order (tmp = P w), then updateInPlace
+// (LDLT.h:404-437) runs column by column: d_j += sigma w_j^2 / alpha,
+// alpha += sigma w_j^2 / d_j, w_i -= w_j l_ij, and, if gamma = d_j alpha +
+// sigma w_j^2 is not zero, l_ij += (sigma w_j / gamma) w_i below the
+// diagonal, stopping once alpha is not finite.  Column j's entries are read
+// and written by their own lanes in step j only, and d_j is read by all
+// lanes before lane j % 64 writes it.  info: 0, or -2 for a size outside
+// 0..256.
+constexpr int kLdltRankMaxN = 256;
+
+// element idx of a vector kept as four register slots per lane (slot idx /
+// 64 of lane idx % 64), as a wave-uniform value
+__device__ __forceinline__ double lane_get(const double (&t)[4], int idx) {
+  const int k = idx >> 6;
+  const double sel = k == 0 ? t[0] : (k == 1 ? t[1] : (k == 2 ? t[2] : t[3]));
+  const long long b = __double_as_longlong(sel);
+  return __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(b >> 32), idx & 63) << 32) |
+                              (unsigned)__builtin_amdgcn_readlane((int)b, idx & 63));
+}
+
+__global__ __launch_bounds__(256) void ldlt_rank_update_kernel(
+    int64_t count, const int64_t* __restrict__ offA, const int32_t* __restrict__ nv,
+    double* __restrict__ Aall, const int64_t* __restrict__ offT, const int32_t* __restrict__ Tall,
+    const int64_t* __restrict__ offw, const double* __restrict__ Wall,
+    const double* __restrict__ sigmas, int32_t* __restrict__ info) {
+  constexpr int K = kLdltRankMaxN / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (s >= count) return;  // whole waves
+  const int n = nv[s];
+  if (n < 0 || n > kLdltRankMaxN) {
+    if (lane == 0) info[s] = -2;
+    return;
+  }
+  double* A = Aall + offA[s];
+  const int32_t* tr = Tall + offT[s];
+  const double* w = Wall + offw[s];
+  const double sigma = sigmas ? sigmas[s] : 1.0;
+  double t[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = lane + 64 * k;
+    t[k] = i < n ? w[i] : 0.0;
+  }
+  // tmp = P w: transposition k swaps entries k and tr[k], k = 0, 1, ...
+  for (int k = 0; k < n; ++k) {
+    const int j = tr[k];
+    if (j == k || j < 0 || j >= n) continue;
+    const double vk = lane_get(t, k), vj = lane_get(t, j);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = lane + 64 * q;
+      t[q] = i == k ? vj : (i == j ? vk : t[q]);
+    }
+  }
+  double alpha = 1.0;
+  for (int j = 0; j < n; ++j) {
+    if (!__builtin_isfinite(alpha)) break;
+    const double dj = A[j + (int64_t)j * n];
+    const double wj = lane_get(t, j);
+    const double swj2 = sigma * wj * wj;
+    const double gamma = dj * alpha + swj2;
+    if (lane == (j & 63)) A[j + (int64_t)j * n] = dj + swj2 / alpha;
+    alpha += swj2 / dj;
+    const double c = sigma * wj / gamma;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = lane + 64 * q;
+      if (i > j && i < n) {
+        const double lij = A[i + (int64_t)j * n];
+        t[q] -= wj * lij;
+        if (gamma != 0.0) A[i + (int64_t)j * n] = lij + c * t[q];
+      }
+    }
+  }
+  if (lane == 0) info[s] = 0;
+}
+
 }  // namespace
+
+int ldlt_rank_update_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n,
+                             double* A, const int64_t* offT, const int32_t* T,
+                             const int64_t* offw, const double* w, const double* sigma,
+                             int32_t* info) {
+  if (count <= 0) return kOk;
+  const int64_t blocks = (count + 3) / 4;  // four waves (factors) per workgroup
+  if (blocks > INT32_MAX) {
+    set_error("ldlt rank update: %lld factors exceed one launch", (long long)count);
+    return kErrUnsupported;
+  }
+  hipLaunchKernelGGL(ldlt_rank_update_kernel, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                     count, offA, n, A, offT, T, offw, w, sigma, info);
+  EAMD_HIP(hipGetLastError());
+  return kOk;
+}
 
 int ldlt_batched(eamd_handle* h, int64_t count, const int64_t* offA, const int32_t* n, int max_n,
                  double* A, const int64_t* offT, int32_t* T, int32_t* sign, int32_t* info) {

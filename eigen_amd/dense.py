@@ -386,6 +386,27 @@ class LDLT:
         batched.ldlt_solve_batched(z, nn, self._m, z, self._t, offb, x, max_n=n)
         return x.reshape(-1) if vec else x
 
+    def rankUpdate(self, w, sigma=1.0):
+        """The factor of A + sigma w w^T in place (LDLT.h:529-555, through
+        ldlt_inplace<Lower>::update, :403-449); the transpositions and the
+        sign stay as they are.  Only a computed factor is updated (the
+        reference also starts from an empty one)."""
+        from . import batched
+        if self._m is None:
+            raise ValueError("LDLT.rankUpdate: compute() first")
+        n = self._m.shape[0]
+        dev = self._m.device
+        wv = colmajor(w, torch.float64).reshape(-1).contiguous()
+        if wv.numel() != n:
+            raise ValueError("LDLT.rankUpdate: w has %d entries, the factor %d" % (wv.numel(), n))
+        z = torch.zeros(1, dtype=torch.int64, device=dev)
+        nn = torch.tensor([n], dtype=torch.int32, device=dev)
+        sg = torch.tensor([float(sigma)], dtype=torch.float64, device=dev)
+        info = batched.ldlt_rank_update_batched(z, nn, self._m, z, self._t, z, wv, sg)
+        if int(info[0]) != 0:
+            raise RuntimeError("LDLT.rankUpdate: info %d" % int(info[0]))
+        return self
+
     def reconstructedMatrix(self):
         """P^T L D L^T P (LDLT.h:620-643)."""
         L = self.matrixL()

@@ -238,6 +238,20 @@ int eamd_dldlt_batched(eamd_handle_t h, int64_t count, const int64_t* offA, cons
                        int32_t max_n, double* A, const int64_t* offT, int32_t* transpositions,
                        int32_t* sign, int32_t* info);
 
+/* Replaces LDLT<MatrixXd, Lower>::rankUpdate(w, sigma) (LDLT.h:529-555,
+ * through internal::ldlt_inplace<Lower>::update and updateInPlace,
+ * LDLT.h:403-449) for many factors of eamd_dldlt_batched in one launch:
+ * factor i (n[i] x n[i] at A + offA[i]: L strictly below the diagonal, D on
+ * it) with its transpositions (transpositions + offT[i]) becomes the factor
+ * of P^T L D L^T P + sigma[i] w w^T, w at w + offw[i] (n[i] values, read
+ * only); sigma null means 1 for every factor.  The transpositions and the
+ * sign are unchanged, as in the reference.  info[i] 0, or -2 for n[i]
+ * outside 0..256.  Device arrays; asynchronous on the handle's stream. */
+int eamd_dldlt_rank_update_batched(eamd_handle_t h, int64_t count, const int64_t* offA,
+                                   const int32_t* n, double* A, const int64_t* offT,
+                                   const int32_t* transpositions, const int64_t* offw,
+                                   const double* w, const double* sigma, int32_t* info);
+
 /* Replaces LDLT<MatrixXd, Lower>::solve(b) (LDLT.h:559-593) for each system
  * factored by eamd_dldlt_batched: b + offb[i] (n[i] values) is overwritten
  * with P^T L^-T D^+ L^-1 P b, D^+ zeroing the rows whose |d| is not above

@@ -669,6 +669,45 @@ long orc_ldlt_d(long n, double* A, long ld, int* trans, int* sign) {
 /* x <- A^{-1} b from the LDLT factors, nrhs columns (LDLT.h:559-593):
  * x = P b (the transpositions in order), L^{-1}, D^+ (rows whose |d| is not
  * above 1 / DBL_MAX are set to zero), L^{-T}, then P^T (in reverse order). */
+/* LDLT rank-one update in place on ldlt's factor (strictly lower L, D on the
+ * diagonal, column-major, ld; trans as orc_ldlt_d returns it): w (n values,
+ * read only) is permuted by the transpositions in order k = 0, 1, ...
+ * (Transpositions * vector, ProductEvaluators.h:1051-1056), then column by
+ * column d_j += sigma w_j^2 / alpha, alpha += sigma w_j^2 / d_j,
+ * w_i -= w_j l_ij and, if gamma = d_j alpha + sigma w_j^2 != 0,
+ * l_ij += (sigma w_j / gamma) w_i, stopping once alpha is not finite.
+ * Restates internal::ldlt_inplace<Lower>::update / updateInPlace
+ * (LDLT.h:403-449), as LDLT::rankUpdate (LDLT.h:529-555) calls it on a
+ * computed factor. */
+void orc_ldlt_rank_update_d(long n, double* A, long ld, const int* trans, const double* w,
+                            double sigma) {
+  double* t = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (long i = 0; i < n; ++i) t[i] = w[i];
+  for (long k = 0; k < n; ++k) {
+    const long j = trans[k];
+    if (j != k) {
+      const double x = t[k];
+      t[k] = t[j];
+      t[j] = x;
+    }
+  }
+  double alpha = 1.0;
+  for (long j = 0; j < n; ++j) {
+    if (!isfinite(alpha)) break;
+    const double dj = A[j + j * ld], wj = t[j];
+    const double swj2 = sigma * wj * wj;
+    const double gamma = dj * alpha + swj2;
+    A[j + j * ld] += swj2 / alpha;
+    alpha += swj2 / dj;
+    for (long i = j + 1; i < n; ++i) t[i] -= wj * A[i + j * ld];
+    if (gamma != 0.0) {
+      const double c = sigma * wj / gamma;
+      for (long i = j + 1; i < n; ++i) A[i + j * ld] += c * t[i];
+    }
+  }
+  free(t);
+}
+
 void orc_ldlt_solve_d(long n, long nrhs, const double* A, long ld, const int* trans, double* B,
                       long ldb) {
   const double tol = 1.0 / 1.7976931348623157e308;

@@ -66,6 +66,8 @@ def orc():
         lib.orc_ldlt_d.argtypes = [L, _dp, L, _ip, C.POINTER(C.c_int)]
         lib.orc_ldlt_d.restype = L
         lib.orc_ldlt_solve_d.argtypes = [L, L, _dp, L, _ip, _dp, L]
+        lib.orc_ldlt_rank_update_d.argtypes = [L, _dp, L, _ip, _dp, C.c_double]
+        lib.orc_ldlt_rank_update_d.restype = None
         lib.orc_batched_lu_solve_d.argtypes = [L, _i64p, _ip, _d1, _i64p, L, _d1, _ip]
         _ORC = lib
     return _ORC
@@ -106,6 +108,8 @@ def ref():
         lib.eref_ldlt_d.argtypes = [L, _dp, L, _ip, C.POINTER(C.c_int)]
         lib.eref_ldlt_d.restype = L
         lib.eref_ldlt_solve_d.argtypes = [L, L, _dp, L, _dp, L]
+        lib.eref_ldlt_rank_update_d.argtypes = [L, _dp, L, _dp, C.c_double, _ip]
+        lib.eref_ldlt_rank_update_d.restype = None
         lib.eref_gemm_d.argtypes = [L, L, L, _dp, L, _dp, L, _dp, L]
         lib.eref_trsm_d.argtypes = [C.c_char, C.c_char, C.c_char, C.c_char, L, L, _dp, L, _dp, L]
         lib.eref_syrk_lower_d.argtypes = [L, L, C.c_double, _dp, L, _dp, L]
@@ -203,6 +207,27 @@ def ldlt_solve_reference(a, b):
     n = a.shape[0]
     ref().eref_ldlt_solve_d(n, b.shape[1], a, max(n, 1), b, max(n, 1))
     return b.reshape(-1) if vec else b
+
+
+def ldlt_rank_update(f, t, w, sigma):
+    """LDLT rank-one update (LDLT.h:403-449) of ldlt()'s factor f with
+    transpositions t, on a copy: the factor of P^T L D L^T P + sigma w w^T."""
+    f = np.array(f, dtype=np.float64, order="F", copy=True)
+    n = f.shape[0]
+    orc().orc_ldlt_rank_update_d(n, f, max(n, 1), np.ascontiguousarray(t, dtype=np.int32),
+                                 np.ascontiguousarray(w, dtype=np.float64), float(sigma))
+    return f
+
+
+def ldlt_rank_update_reference(a, w, sigma):
+    """Eigen's LDLT<MatrixXd, Lower>(a).rankUpdate(w, sigma), run by
+    oracle/_ref: (matrixLDLT with a's strict upper part kept, transpositions)."""
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    n = f.shape[0]
+    t = np.zeros(max(n, 1), dtype=np.int32)
+    ref().eref_ldlt_rank_update_d(n, f, max(n, 1), np.ascontiguousarray(w, dtype=np.float64),
+                                  float(sigma), t)
+    return f, t[:n]
 
 
 def llt(a, dtype=np.float64):

@@ -138,6 +138,21 @@ long eref_ldlt_d(long n, double* a, long lda, int* trans, int* sign) {
   return f.info() == Success ? 1 : 0;
 }
 
+// LDLT<MatrixXd, Lower>(A).rankUpdate(w, sigma) (LDLT.h:529-555): the lower
+// triangle of A (column-major, lda) replaced by the updated matrixLDLT()'s,
+// the transpositions' indices in trans
+void eref_ldlt_rank_update_d(long n, double* a, long lda, const double* w, double sigma,
+                             int* trans) {
+  MapD A(a, n, n, OuterStride<>(lda));
+  LDLT<MatrixXd, Lower> f(n);
+  f.compute(MatrixXd(A));
+  f.rankUpdate(Map<const VectorXd>(w, n), sigma);
+  const MatrixXd& m = f.matrixLDLT();
+  for (long j = 0; j < n; ++j)
+    for (long i = j; i < n; ++i) A(i, j) = m(i, j);
+  for (long k = 0; k < n; ++k) trans[k] = (int)f.transpositionsP().indices()(k);
+}
+
 // LDLT<MatrixXd, Lower>::compute(A).solve(B) (LDLT.h:559-593), B in place
 void eref_ldlt_solve_d(long n, long nrhs, const double* a, long lda, double* b, long ldb) {
   MapD A(const_cast<double*>(a), n, n, OuterStride<>(lda));

@@ -30,6 +30,11 @@ these fixtures are Eigen's outputs on seeded inputs (tests/inputs.py):
             info, the factor (n <= 24; diagonal, row and column sums else) and
             the solution of A x = b; plus small exact semidefinite and failing
             cases (`make_golden.py ldlt`).
+  ldltrank.npz  LDLT<MatrixXd, Lower>::rankUpdate (LDLT.h:403-449, 529-555)
+            of Eigen's LDLT of the ldlt.npz matrices (n = 5, 24, 64) by a
+            seeded w with sigma = +1 and -0.5: the updated factor (n <= 24;
+            diagonal, row and column sums else) and transpositions
+            (`make_golden.py ldltrank`).
 Inputs are not stored (they are regenerated from the seeds); their SHA-256
 is, so a diverging generator is detected.
 """
@@ -288,6 +293,32 @@ def ldlt_main():
     print("ldlt fixtures written to", HERE)
 
 
+LDLT_RANK_SIZES = [5, 24, 64]
+LDLT_RANK_SIGMAS = {"p": 1.0, "m": -0.5}
+
+
+def ldltrank_main():
+    out = {}
+    for kind in LDLT_KINDS:
+        for n in LDLT_RANK_SIZES:
+            a = inputs.ldlt_matrix(kind, n)
+            w = inputs.normal(9000 + n, n)
+            for tag, sigma in LDLT_RANK_SIGMAS.items():
+                key = f"{kind}{n}{tag}"
+                out[key + "_sha"] = np.array(inputs.sha256(a) + inputs.sha256(w))
+                f, t = po.ldlt_rank_update_reference(a, w, sigma)
+                lf = np.tril(f)
+                if n <= 24:
+                    out[key + "_f"] = lf
+                else:
+                    out[key + "_diag"] = np.diag(lf).copy()
+                    out[key + "_rows"] = lf.sum(axis=1)
+                    out[key + "_cols"] = lf.sum(axis=0)
+                out[key + "_t"] = t.copy()
+    np.savez_compressed(os.path.join(HERE, "ldltrank.npz"), **out)
+    print("ldlt rank-update fixtures written to", HERE)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "rcond":
         rcond_main()
@@ -295,5 +326,7 @@ if __name__ == "__main__":
         rankupdate_main()
     elif len(sys.argv) > 1 and sys.argv[1] == "ldlt":
         ldlt_main()
+    elif len(sys.argv) > 1 and sys.argv[1] == "ldltrank":
+        ldltrank_main()
     else:
         main()

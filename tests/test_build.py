@@ -49,5 +49,5 @@ def test_no_kernel_uses_scratch(tmp_path):
     # the batched kernels are all there (the check above covers them)
     names = " ".join(n for n, _, _ in meta)
     for k in ("llt_batch_persistent", "lu_ir_batch_kernel", "ldlt_batch_kernel",
-              "ldlt_solve_batch_kernel", "llt_rank_update_kernel"):
+              "ldlt_solve_batch_kernel", "llt_rank_update_kernel", "ldlt_rank_update_kernel"):
         assert k in names, k

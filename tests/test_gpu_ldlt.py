@@ -277,3 +277,56 @@ def test_ldlt_graph_replays():
         assert torch.equal(A, Ae) and torch.equal(T, Te), f"replay {rep}"
         assert torch.equal(b, be) and torch.equal(info, ie) and torch.equal(sign, se)
         del junk
+
+
+def test_ldlt_rank_update_batched_against_reference_fixture():
+    """LDLT::rankUpdate (LDLT.h:403-449, 529-555) of the batched GPU factors
+    of every ldltrank.npz case in one launch: the updated factor within
+    1e-10 (indefinite) / 1e-13 (definite) of Eigen's, its strict upper part
+    and transpositions untouched, and the oracle's update of the same GPU
+    factor matched to 1e-12; then LDLT.rankUpdate on a 200 x 200 factor
+    (beyond the batched factorization's 64, within the update's 256) made
+    by the oracle, against the oracle's update."""
+    import os
+    from tests.test_oracle import LDLT_RANK_SIGMAS, LDLT_RANK_SIZES
+    gr = np.load(os.path.join(os.path.dirname(__file__), "golden", "ldltrank.npz"),
+                 allow_pickle=False)
+    cases = [(kind, k, tag) for kind in LDLT_KINDS for k in LDLT_RANK_SIZES
+             for tag in sorted(LDLT_RANK_SIGMAS)]
+    mats = [inputs.ldlt_matrix(kind, k) for kind, k, _ in cases]
+    ws = [inputs.normal(9000 + k, k) for _, k, _ in cases]
+    offA, n, A, T, offT, sign, info, off, sizes = run_ldlt(mats)
+    f0 = [unpack(A, off, sizes, i) for i in range(len(cases))]
+    Th0 = T.cpu().numpy().copy()
+    ow = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    offw = torch.from_numpy(ow).cuda()
+    W = torch.from_numpy(np.concatenate(ws)).cuda()
+    sig = torch.tensor([LDLT_RANK_SIGMAS[tag] for _, _, tag in cases], **D64)
+    print("ldlt rank update: batch of", len(cases), flush=True)
+    rinfo = batched.ldlt_rank_update_batched(offA, n, A, offT, T, offw, W, sig)
+    assert (rinfo.cpu().numpy() == 0).all()
+    np.testing.assert_array_equal(T.cpu().numpy(), Th0)
+    np.testing.assert_array_equal(W.cpu().numpy(), np.concatenate(ws))
+    ot = offT.cpu().numpy()
+    for i, (kind, k, tag) in enumerate(cases):
+        key = f"{kind}{k}{tag}"
+        f = unpack(A, off, sizes, i)
+        ldlt_check_factor(gr, key, f, 1e-10 if kind == "sym" else 1e-13)
+        np.testing.assert_array_equal(np.triu(f, 1), np.triu(mats[i], 1))
+        ref = po.ldlt_rank_update(f0[i], Th0[ot[i]:ot[i] + k], ws[i], LDLT_RANK_SIGMAS[tag])
+        scale = max(1.0, np.abs(np.tril(ref)).max())
+        assert np.abs(np.tril(f) - np.tril(ref)).max() <= 1e-12 * scale, key
+    k = 200
+    a = inputs.ldlt_matrix("pos", k)
+    fo, to, _, ok = po.ldlt(a)
+    assert ok
+    w = inputs.normal(9200, k)
+    F = ea.LDLT()
+    F._m = ea.colmajor(torch.from_numpy(np.asfortranarray(fo)))
+    F._t = torch.from_numpy(np.ascontiguousarray(to)).cuda()
+    F._sign, F._info = 0, ea.Success
+    print("ldlt rank update: n = 200", flush=True)
+    F.rankUpdate(torch.from_numpy(w), -0.5)
+    ref = po.ldlt_rank_update(fo, to, w, -0.5)
+    got = F.matrixLDLT().cpu().numpy()
+    assert np.abs(np.tril(got) - np.tril(ref)).max() <= 1e-12 * max(1.0, np.abs(ref).max())

@@ -529,3 +529,35 @@ def test_ldlt_brute_force_small():
             w[k + 1:, k + 1:] -= np.outer(L[k + 1:, k], w[k, k + 1:])
         np.testing.assert_allclose(np.diag(f), D, rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose(np.tril(f, -1), np.tril(L, -1), rtol=1e-10, atol=1e-12)
+
+
+LDLT_RANK_SIZES = [5, 24, 64]
+LDLT_RANK_SIGMAS = {"p": 1.0, "m": -0.5}
+
+
+@pytest.fixture(scope="module")
+def gldr():
+    return np.load(os.path.join(G, "ldltrank.npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("kind", LDLT_KINDS)
+@pytest.mark.parametrize("n", LDLT_RANK_SIZES)
+@pytest.mark.parametrize("tag", sorted(LDLT_RANK_SIGMAS))
+def test_ldlt_rank_update_matches_reference_fixture(gldr, kind, n, tag):
+    """The oracle's LDLT rank-one update (LDLT.h:403-449) of its own LDLT
+    factor against Eigen's LDLT(A).rankUpdate(w, sigma) (transpositions
+    equal, factor within 1e-10 / 1e-13 of its largest entry), and,
+    independently of the reference, P^T L' D' L'^T P = A + sigma w w^T."""
+    sigma = LDLT_RANK_SIGMAS[tag]
+    a = inputs.ldlt_matrix(kind, n)
+    w = inputs.normal(9000 + n, n)
+    key = f"{kind}{n}{tag}"
+    assert str(gldr[key + "_sha"]) == inputs.sha256(a) + inputs.sha256(w)
+    f, t, _, _ = po.ldlt(a)
+    fo = po.ldlt_rank_update(f, t, w, sigma)
+    np.testing.assert_array_equal(t, gldr[key + "_t"])
+    ldlt_check_factor(gldr, key, fo, 1e-10 if kind == "sym" else 1e-13)
+    target = a + sigma * np.outer(w, w)
+    err = np.abs(ldlt_reconstruct(fo, t) - target).max()
+    assert err <= 1e-12 * max(1.0, np.abs(target).max()) * n
+    np.testing.assert_array_equal(np.triu(fo, 1), np.triu(a, 1))
