@@ -24,6 +24,8 @@ namespace eamd {
 
 namespace {
 
+// polls a wait may take before it gives up (and reports a time-out) unless
+// EAMD_TRSV_SPIN sets another limit
 constexpr int kSpin = 1 << 26;
 
 template <typename T>
@@ -49,7 +51,8 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
                                                          int64_t lda, T* __restrict__ X,
                                                          int64_t ldx, int c0, int nrhs,
                                                          u64* __restrict__ xw, unsigned epoch,
-                                                         int* __restrict__ err, int nch) {
+                                                         int* __restrict__ err, int nch,
+                                                         int spin_limit) {
   static_assert((BS == 64 || BS == 128) && (NRC == 16 || NRC == 4 || NRC == 1),
                 "four threads per row of a 64- or 128-row block");
   constexpr int NT = 4 * BS;   // threads
@@ -223,8 +226,10 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
           }
         }
       if (all) break;
-      if ((++spins & 255) == 0 &&
-          (spins > kSpin ||
+      // the limit every poll, another workgroup's give-up every 256th
+      ++spins;
+      if (spins > spin_limit ||
+          ((spins & 255) == 0 &&
            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
         bad = 1;
         break;
@@ -326,8 +331,9 @@ __global__ __launch_bounds__(4 * BS) void trsv_chain_kernel(int n, const T* __re
         return m;
       };
       for (int spins = 0; __ballot(missing()) != 0ull;) {
-        if ((++spins & 255) == 0 &&
-            (spins > kSpin ||
+        ++spins;
+        if (spins > spin_limit ||
+            ((spins & 255) == 0 &&
              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           return;
@@ -380,6 +386,7 @@ template <typename T>
 int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, bool lower,
                 bool unit, T* X, int64_t ldx, int nrhs) {
   if (n <= 0 || nrhs <= 0) return kOk;
+  static const int spin_limit = env_int("EAMD_TRSV_SPIN", kSpin);
   constexpr int BS = 64, NRC = 4, kChains = 4;
   const int nblk = ceil_div(n, BS);
   // error word | exchange words (kChains chains of NRC values per row, 32
@@ -407,7 +414,7 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     const unsigned epoch = (unsigned)++h->trsv_epoch;
 #define EAMD_TRSV1(RM, LO, UN)                                                                   \
   hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS1, 1>), dim3(nblk1), dim3(4 * BS1), 0, \
-                     h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err, 1)
+                     h->stream, n, A, lda, X, ldx, 0, 1, xw, epoch, err, 1, spin_limit)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV1(true, true, true); else EAMD_TRSV1(true, true, false); }
       else { if (unit) EAMD_TRSV1(true, false, true); else EAMD_TRSV1(true, false, false); }
@@ -430,7 +437,7 @@ int trsv_device(eamd_handle* h, int n, const T* A, int64_t lda, bool rowmajor, b
     const int nch = std::min(kChains, ceil_div(nrhs - c0, NRC));
 #define EAMD_TRSV(RM, LO, UN)                                                                  \
   hipLaunchKernelGGL((trsv_chain_kernel<T, RM, LO, UN, BS, NRC>), dim3(nblk * nch), dim3(256), \
-                     0, h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err, nch)
+                     0, h->stream, n, A, lda, X, ldx, c0, nrhs, xw, epoch, err, nch, spin_limit)
     if (rowmajor) {
       if (lower) { if (unit) EAMD_TRSV(true, true, true); else EAMD_TRSV(true, true, false); }
       else { if (unit) EAMD_TRSV(true, false, true); else EAMD_TRSV(true, false, false); }

@@ -68,3 +68,39 @@ def test_switch(gpu, env):
         assert conv == 1 and be <= 1e-14, (prec, res)
     assert res["lu_pivots_ok"], res
     assert res["lu"] <= 1e-14 and res["llt"] <= 1e-14, res
+
+
+GIVE_UP_CHILD = r"""
+import json, sys
+import torch
+sys.path.insert(0, sys.argv[1])
+import eigen_amd as ea
+from tests import inputs
+torch.cuda.set_device(0)
+n = 4096
+lu = ea.PartialPivLU(ea.colmajor(torch.from_numpy(inputs.normal(91, n, n)).cuda()))
+out = {}
+try:
+    lu.solve(torch.from_numpy(inputs.normal(92, n)).cuda())
+    out["raised"] = False
+except ea.EamdError as e:
+    out["raised"], out["msg"] = True, str(e)
+torch.cuda.synchronize()
+print("RESULT " + json.dumps(out), flush=True)
+"""
+
+
+def test_trsv_wait_gives_up_and_reports(gpu):
+    """The triangular-solve chain's bounded wait, forced from the host:
+    with EAMD_TRSV_SPIN=0 every workgroup whose predecessor has not yet
+    published gives up at its first poll (its kernel returns normally and
+    the others see the error word), and the solve reports the time-out as
+    an error instead of hanging or returning a partial result."""
+    print("trsv give-up: EAMD_TRSV_SPIN=0", flush=True)
+    r = subprocess.run([sys.executable, "-c", GIVE_UP_CHILD, ROOT],
+                       env={**os.environ, "EAMD_TRSV_SPIN": "0"},
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("RESULT ")][-1]
+    res = json.loads(line[len("RESULT "):])
+    assert res["raised"] and "timed out" in res["msg"], res
