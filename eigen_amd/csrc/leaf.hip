@@ -290,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
     int* __restrict__ lperm_n, int* __restrict__ lpos, int* __restrict__ lsrc,
     int* __restrict__ pperm_n, int* __restrict__ ppos, int* __restrict__ psrc, int row0,
     int pnb, int* __restrict__ zero_col, u64* __restrict__ xbuf, int* __restrict__ err,
-    unsigned epoch_base, int piv_offset, int* __restrict__ fz_global) {
+    unsigned epoch_base, int piv_offset, int* __restrict__ fz_global, int spin_limit) {
   __builtin_amdgcn_s_setprio(3);  // on the critical path (see tsl_kernel)
   constexpr int NW = Words<T>::n;
   constexpr int SW = slot_words<T>();
@@ -437,8 +437,9 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
             }
           }
         if (all) break;
-        if ((spins & 255) == 255 &&
-            (spins > kLeafSpin ||
+        // the limit every poll, another workgroup's give-up every 256th
+        if (spins >= spin_limit ||
+            ((spins & 255) == 255 &&
              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           bad = 1;
           break;
@@ -525,7 +526,19 @@ __global__ __launch_bounds__(BLOCK) void leaf_lu_kernel(
   }
 #pragma unroll 1
   for (int r = ngroups; r < 4; ++r) rotate8(v);
-  if (s_abort) return;
+  if (s_abort) {
+    // gave up (another workgroup's words never came; the call reports a
+    // time-out): empty permutation lists and no zero pivot, so the kernels
+    // that apply the lists after this leaf do nothing instead of indexing
+    // rows with stale entries.  Workgroup 0 always takes this path when any
+    // workgroup does (it needs every workgroup's words for each column).
+    if (g == 0 && tid == 0) {
+      *lperm_n = 0;
+      *pperm_n = 0;
+      *zero_col = -1;
+    }
+    return;
+  }
   __syncthreads();
   // ---- bookkeeping (workgroup 0, wave 0): the leaf-level and panel-level
   // permutation lists, replaying the interchanges in column order
@@ -576,10 +589,13 @@ int leaf_lu(eamd_handle* h, T* P, int64_t ldp, int m, int w, int* piv, int* lper
   unsigned epoch_base = h->panel_epoch;
   h->panel_epoch += (unsigned)w + 1u;
   char* b = static_cast<char*>(base);
+  // polls an exchange may take before the leaf gives up (the call then
+  // reports a time-out); EAMD_LEAF_SPIN sets another limit
+  static const int spin_limit = env_int("EAMD_LEAF_SPIN", kLeafSpin);
   hipLaunchKernelGGL((leaf_lu_kernel<T, BLOCK>), dim3(G), dim3(BLOCK), 0, h->stream, P, ldp, m, w,
                      G, piv, lperm_n, lpos, lsrc, pperm_n, ppos, psrc, row0, pnb, zero_col,
                      reinterpret_cast<u64*>(b + 256), reinterpret_cast<int*>(b), epoch_base,
-                     piv_offset, fz_global);
+                     piv_offset, fz_global, spin_limit);
   EAMD_HIP(hipGetLastError());
   return kOk;
 }

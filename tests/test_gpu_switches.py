@@ -104,3 +104,40 @@ def test_trsv_wait_gives_up_and_reports(gpu):
     line = [ln for ln in r.stdout.splitlines() if ln.startswith("RESULT ")][-1]
     res = json.loads(line[len("RESULT "):])
     assert res["raised"] and "timed out" in res["msg"], res
+
+
+LEAF_GIVE_UP_CHILD = r"""
+import json, sys
+import torch
+sys.path.insert(0, sys.argv[1])
+import eigen_amd as ea
+from tests import inputs
+torch.cuda.set_device(0)
+n = 4096
+A = ea.colmajor(torch.from_numpy(inputs.normal(93, n, n)).cuda())
+out = {}
+try:
+    ea.PartialPivLU(A)
+    out["raised"] = False
+except ea.EamdError as e:
+    out["raised"], out["msg"] = True, str(e)
+torch.cuda.synchronize()
+print("RESULT " + json.dumps(out), flush=True)
+"""
+
+
+def test_leaf_exchange_gives_up_and_reports(gpu):
+    """The FP64 LU leaf's grid exchange, forced from the host: with
+    EAMD_LEAF_SPIN=0 a workgroup gives up at its first poll that finds
+    another workgroup's words missing (every kernel returns normally; the
+    aborted leaves leave empty permutation lists, so the interchanges after
+    them do nothing), and the factorization reports the time-out as an
+    error."""
+    print("leaf give-up: EAMD_LEAF_SPIN=0", flush=True)
+    r = subprocess.run([sys.executable, "-c", LEAF_GIVE_UP_CHILD, ROOT],
+                       env={**os.environ, "EAMD_LEAF_SPIN": "0"},
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("RESULT ")][-1]
+    res = json.loads(line[len("RESULT "):])
+    assert res["raised"] and "timed out" in res["msg"], res
